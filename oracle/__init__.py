@@ -1,0 +1,182 @@
+"""TEST INFRASTRUCTURE ONLY: the CPU oracle for the unsplit hydro step.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this package, and only as the checker.  The product
+package ``ramses_amd`` never imports it.
+
+Two CPU implementations are exposed through ctypes:
+
+* ``Oracle``  -- ``libhydro_oracle.so``, the C restatement in ``hydro_ref.c``
+  (built by ``make -C oracle``), and
+* ``Reference`` -- ``_ref/libramses_ref.so``, the reference's own Fortran
+  hot-path routines (hydro/umuscl.f90, hydro/godunov_utils.f90) compiled from
+  the reference sources by ``make -C oracle ref``.  It is present only where
+  it was built.
+
+All arrays are numpy float64 in the reference's Fortran layouts, passed as
+C-ordered numpy arrays whose axes are the Fortran axes reversed:
+``uin[nvar][k][j][i][l]`` (6x6x6 cells, ``l`` < nvector fastest),
+``flux[ndim][nvar][k][j][i][l]`` (3x3x3 face slots), ``tmp[ndim][2][k][j][i][l]``.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(_HERE, "libhydro_oracle.so")
+REF_LIB = os.path.join(_HERE, "_ref", "libramses_ref.so")
+
+NVAR, NDIM = 5, 3
+RIEMANN = {"llf": 0, "hllc": 1, "hll": 2}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [("gamma", ctypes.c_double), ("smallr", ctypes.c_double),
+                ("smallc", ctypes.c_double), ("slope_theta", ctypes.c_double),
+                ("courant_factor", ctypes.c_double), ("difmag", ctypes.c_double),
+                ("slope_type", ctypes.c_int), ("riemann", ctypes.c_int)]
+
+
+def stencil_shape(nv):
+    return (NVAR, 6, 6, 6, nv)
+
+
+def grav_shape(nv):
+    return (NDIM, 6, 6, 6, nv)
+
+
+def flux_shape(nv):
+    return (NDIM, NVAR, 3, 3, 3, nv)
+
+
+def tmp_shape(nv):
+    return (NDIM, 2, 3, 3, 3, nv)
+
+
+class Oracle:
+    """ctypes front end of the C restatement (hydro_ref.c)."""
+
+    def __init__(self, path=ORACLE_LIB):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is not built (make -C oracle)")
+        lib = ctypes.CDLL(path)
+        lib.hr_set_params.argtypes = [ctypes.POINTER(_Params)]
+        lib.hr_set_params.restype = ctypes.c_int
+        lib.hr_unsplit.argtypes = [_dp, _dp, _dp, _dp] + [ctypes.c_double] * 4 + [ctypes.c_int] * 2
+        lib.hr_unsplit.restype = ctypes.c_int
+        lib.hr_unsplit_batched.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_long, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        lib.hr_unsplit_batched.restype = ctypes.c_int
+        lib.hr_cmpdt.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        lib.hr_cmpdt.restype = ctypes.c_double
+        lib.hr_uniform_godunov.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_int, ctypes.c_int]
+        lib.hr_uniform_godunov.restype = ctypes.c_int
+        lib.hr_uniform_courant.argtypes = [_dp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_int, _dp, _dp, _dp]
+        lib.hr_uniform_courant.restype = ctypes.c_double
+        lib.hr_uniform_step.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                        ctypes.c_int]
+        lib.hr_uniform_step.restype = ctypes.c_double
+        self.lib = lib
+
+    def set_params(self, gamma=1.4, smallr=1e-10, smallc=1e-10, slope_type=1, slope_theta=1.5,
+                   riemann="llf", courant_factor=0.5, difmag=0.0):
+        p = _Params(gamma, smallr, smallc, slope_theta, courant_factor, difmag, slope_type,
+                    RIEMANN[riemann])
+        rc = self.lib.hr_set_params(ctypes.byref(p))
+        if rc != 0:
+            raise ValueError(f"oracle rejected the configuration (code {rc})")
+
+    def unsplit(self, uin, gravin, flux, tmp, dx, dt, ngrid, dy=None, dz=None):
+        nv = uin.shape[-1]
+        rc = self.lib.hr_unsplit(_ptr(uin), _ptr(gravin), _ptr(flux), _ptr(tmp), dx,
+                                 dx if dy is None else dy, dx if dz is None else dz, dt, ngrid, nv)
+        if rc:
+            raise RuntimeError(f"hr_unsplit failed ({rc})")
+
+    def unsplit_batched(self, uin, gravin, flux, tmp, dx, dt, ngrid_last=None, nthreads=0):
+        """uin[g][nvar][6][6][6][nv]; all groups full except the last."""
+        ng, nv = uin.shape[0], uin.shape[-1]
+        rc = self.lib.hr_unsplit_batched(_ptr(uin), _ptr(gravin), _ptr(flux), _ptr(tmp), dx, dt,
+                                         ng, nv if ngrid_last is None else ngrid_last, nv, nthreads)
+        if rc:
+            raise RuntimeError(f"hr_unsplit_batched failed ({rc})")
+
+    def cmpdt(self, uu, gg, dx, ncell):
+        """uu[nvar][nv] is overwritten like the reference does; returns dt."""
+        return self.lib.hr_cmpdt(_ptr(uu), _ptr(gg), dx, ncell, uu.shape[-1])
+
+    def uniform_godunov(self, u, dx, dt, nv=32, nthreads=0):
+        n = u.shape[-1]
+        unew = np.empty_like(u)
+        rc = self.lib.hr_uniform_godunov(_ptr(u), _ptr(unew), n, dx, dt, nv, nthreads)
+        if rc:
+            raise RuntimeError(f"hr_uniform_godunov failed ({rc})")
+        return unew
+
+    def uniform_courant(self, u, dx, dt_start, nv=32):
+        n = u.shape[-1]
+        m, e, ei = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        dt = self.lib.hr_uniform_courant(_ptr(u), n, dx, dt_start, nv, ctypes.byref(m),
+                                         ctypes.byref(e), ctypes.byref(ei))
+        return dt, m.value, e.value, ei.value
+
+    def uniform_step(self, u, dx, nv=32, nthreads=0):
+        n = u.shape[-1]
+        unew = np.empty_like(u)
+        dt = self.lib.hr_uniform_step(_ptr(u), _ptr(unew), n, dx, nv, nthreads)
+        if dt < 0:
+            raise RuntimeError("hr_uniform_step failed")
+        return unew, dt
+
+
+class Reference:
+    """ctypes front end of the reference's own compiled routines (oracle/_ref)."""
+
+    def __init__(self, path=REF_LIB):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is not built (make -C oracle ref)")
+        lib = ctypes.CDLL(path)
+        lib.ref_set_hydro_params.argtypes = [ctypes.c_double] * 3 + [ctypes.c_int, ctypes.c_double,
+                                                                       ctypes.c_int, ctypes.c_double,
+                                                                       ctypes.c_double]
+        lib.ref_unsplit.argtypes = [_dp, _dp, _dp, _dp, _dp] + [ctypes.c_double] * 4 + [ctypes.c_int]
+        lib.ref_cmpdt.argtypes = [_dp, _dp, ctypes.c_double, _dp, ctypes.c_int]
+        nv, nvar, nd = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        lib.ref_shape(ctypes.byref(nv), ctypes.byref(nvar), ctypes.byref(nd))
+        self.nvector, self.nvar, self.ndim = nv.value, nvar.value, nd.value
+        self.lib = lib
+        self._pin = np.zeros((6, 6, 6, self.nvector))
+        self._grav0 = np.zeros(grav_shape(self.nvector))
+
+    def set_params(self, gamma=1.4, smallr=1e-10, smallc=1e-10, slope_type=1, slope_theta=1.5,
+                   riemann="llf", courant_factor=0.5, difmag=0.0):
+        self.lib.ref_set_hydro_params(gamma, smallr, smallc, slope_type, slope_theta,
+                                      RIEMANN[riemann], courant_factor, difmag)
+
+    def unsplit(self, uin, gravin, flux, tmp, dx, dt, ngrid, dy=None, dz=None):
+        assert uin.shape == stencil_shape(self.nvector)
+        g = self._grav0 if gravin is None else gravin
+        self.lib.ref_unsplit(_ptr(uin), _ptr(g), _ptr(self._pin), _ptr(flux), _ptr(tmp), dx,
+                             dx if dy is None else dy, dx if dz is None else dz, dt, ngrid)
+
+    def cmpdt(self, uu, gg, dx, ncell):
+        dt = ctypes.c_double()
+        g = np.zeros((NDIM, self.nvector)) if gg is None else gg
+        self.lib.ref_cmpdt(_ptr(uu), _ptr(g), dx, ctypes.byref(dt), ncell)
+        return dt.value
+
+
+def reference_available():
+    return os.path.exists(REF_LIB)
