@@ -1,0 +1,122 @@
+/*
+ * ramses_hip -- MI355X (gfx950) implementation of the RAMSES unsplit
+ * MUSCL-Hancock hydro step behind the reference's own call boundary.
+ *
+ * C ABI only: plain pointers, sizes and scalars; no C++ or torch types.
+ * Every entry point cites the reference interface it replaces.  Array
+ * arguments use the reference's Fortran layouts (column-major, the oct lane
+ * `l` fastest with leading dimension nvector):
+ *
+ *   uin   (nvector, -1:4, -1:4, -1:4, nvar)        hydro/umuscl.f90:33
+ *   gravin(nvector, -1:4, -1:4, -1:4, ndim)        hydro/umuscl.f90:34
+ *   pin   (nvector, -1:4, -1:4, -1:4)   (unused)   hydro/umuscl.f90:32
+ *   flux  (nvector, 1:3, 1:3, 1:3, nvar, ndim)     hydro/umuscl.f90:37
+ *   tmp   (nvector, 1:3, 1:3, 1:3, 2, ndim)        hydro/umuscl.f90:38
+ *
+ * NDIM=3, NVAR=5, NENER=0.  Only the 36 defined faces per oct and only lanes
+ * l < ngrid of flux/tmp are written, as in the reference; every other entry
+ * keeps the caller's contents.
+ *
+ * Status codes: 0 success, < 0 invalid argument or configuration,
+ * > 0 HIP runtime error (the hipError_t value).  ramses_last_error() returns
+ * a message for the last failure on the calling thread.
+ */
+#ifndef RAMSES_HIP_H
+#define RAMSES_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RAMSES_RIEMANN_LLF 0
+#define RAMSES_RIEMANN_HLLC 1
+#define RAMSES_RIEMANN_HLL 2
+
+#define RAMSES_FP64 0        /* every operation in FP64, bitwise the reference */
+#define RAMSES_FP32_MIXED 1  /* FP32 primitives/slopes/trace/Riemann; FP64 state,
+                                E-kinetic recovery, flux accumulation and dt */
+
+/* Solver configuration.  Replaces the module globals the reference reads in
+ * every routine of the path: gamma, smallr, smallc, slope_type, slope_theta,
+ * riemann, scheme, difmag, courant_factor (hydro/hydro_parameters.f90:74-85)
+ * and the compile-time nvector (amr/amr_parameters.f90:49-53). */
+typedef struct ramses_hydro_params {
+  double gamma;          /* ratio of specific heats, > 1 */
+  double smallr;         /* density floor */
+  double smallc;         /* sound-speed floor */
+  double slope_theta;    /* slope_type 8 parameter */
+  double courant_factor; /* CFL number used by the dt reduction */
+  double difmag;         /* must be 0: artificial diffusion (cmpdivu/consup) is not built */
+  int slope_type;        /* 0,1,2,3,7,8 (the NDIM=3 branches of uslope) */
+  int riemann;           /* RAMSES_RIEMANN_* */
+  int precision;         /* RAMSES_FP64 or RAMSES_FP32_MIXED */
+  int nvector;           /* leading dimension of the Fortran arrays (32 in the reference build) */
+} ramses_hydro_params;
+
+/* Validate and install the configuration used by every later call.
+ * Replaces the namelist-read module globals (hydro/read_hydro_params.f90:43-53);
+ * an unsupported value is rejected here, never mid-kernel (the reference
+ * stops inside the kernel: hydro/umuscl.f90:801-803,1474-1477). */
+int ramses_hydro_config(const ramses_hydro_params *p);
+
+/* Message for the last non-zero status returned on this thread. */
+const char *ramses_last_error(void);
+
+/* Link-level drop-in for the reference's external subroutine
+ *   subroutine unsplit(uin,gravin,pin,flux,tmp,dx,dy,dz,dt,ngrid)
+ * (hydro/umuscl.f90:22, called from hydro/godunov_fine.f90:681).  All
+ * arguments by reference, host memory.  On failure it prints the message and
+ * aborts, as the reference's `stop` does. */
+void unsplit_(const double *uin, const double *gravin, const double *pin, double *flux, double *tmp,
+              const double *dx, const double *dy, const double *dz, const double *dt, const int *ngrid);
+
+/* ISO_C_BINDING form of the same call without `pin` (the 9-argument form of
+ * rhd/umuscl.f90:31 and patch/6eq/umuscl.f90:22).  Host memory; gravin may
+ * be NULL (gravity zero). */
+int ramses_unsplit(const double *uin, const double *gravin, double *flux, double *tmp, double dx,
+                   double dy, double dz, double dt, int ngrid);
+
+/* Device-resident batched form: `ngroups` consecutive unsplit array sets in
+ * device memory (each laid out exactly like one call above, group g at
+ * offset g*nvector*216*5 for uin, etc.), every group full except the last
+ * which has `ngrid_last` lanes.  Equivalent to the godunov_fine sweep loop
+ * (hydro/godunov_fine.f90:24-31) calling unsplit once per batch.  Enqueued
+ * on `stream` (a hipStream_t; NULL = default stream), asynchronous. */
+int ramses_unsplit_device(const double *uin, const double *gravin, double *flux, double *tmp, double dx,
+                          double dy, double dz, double dt, long ngroups, int ngrid_last, void *stream);
+
+/* cmpdt (hydro/godunov_utils.f90:5-120) over `ncell` cells of a (nvector,
+ * nvar) host array `uu` and (nvector, ndim) gravity `gg` (NULL = zero).
+ * Unlike the reference `uu` is not overwritten.  Returns the status and
+ * writes the time step to *dt. */
+int ramses_cmpdt(const double *uu, const double *gg, double dx, int ncell, double *dt);
+
+/* ---------------------------------------------------------------------
+ * Resident uniform level (a fully refined periodic level, levelmin ==
+ * levelmax): the godunov_fine -> godfine1 sweep (hydro/godunov_fine.f90:5-35,
+ * 486-911), set_unew/set_uold (:40-232) and courant_fine
+ * (hydro/courant_fine.f90:1-159) over conserved state kept in device memory
+ * as u[var][z][y][x], var-major SoA, n^3 cells per variable.
+ * --------------------------------------------------------------------- */
+
+/* courant_fine over the level: writes min(dt_start, min_cell cmpdt) to
+ * *d_dt (device memory, one double).  u: device state. */
+int ramses_uniform_courant(const double *u, int n, double dx, double dt_start, double *d_dt, void *stream);
+
+/* godfine1 over every oct of the level with the time step read from device
+ * memory *d_dt: unew = u + sum_dim (flux_left - flux_right).  When
+ * d_dt_next is not NULL the courant reduction of the new state is fused into
+ * the same kernel and min-combined into *d_dt_next, which the caller must
+ * have set to the starting value (ramses_uniform_dt_init). */
+int ramses_uniform_godunov(const double *u, double *unew, int n, double dx, const double *d_dt,
+                           double *d_dt_next, void *stream);
+
+/* Writes `value` into one device double with a kernel (graph-capture safe):
+ * the per-step reset of the dt accumulator, dtnew(ilevel)=boxlen/smallc in
+ * newdt_fine (pm/newdt_fine.f90:51). */
+int ramses_uniform_dt_init(double *d_dt, double value, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RAMSES_HIP_H */

@@ -1,0 +1,244 @@
+// C ABI of the library (include/ramses_hip.h): configuration, error
+// reporting, and the host-memory / device-memory forms of `unsplit`.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "../../include/ramses_hip.h"
+#include "hydro_device.hpp"
+
+namespace ramses {
+struct UnsplitArgs;
+hipError_t launch_unsplit(const UnsplitArgs& a, bool fp32, hipStream_t stream);
+hipError_t launch_cmpdt(const double* uu, const double* gg, int nv, int ncell, double dx,
+                        const HydroParams& hp, double* d_dt, hipStream_t stream);
+hipError_t launch_set_double(double* p, double v, hipStream_t s);
+hipError_t launch_courant(const double* u, int n, double dx, const HydroParams& hp, double* d_dt, hipStream_t s);
+hipError_t launch_uniform_godunov(const double* u, double* unew, int n, double dx, const double* d_dt,
+                                  double* d_dt_next, const HydroParams& hp, bool fp32, hipStream_t s);
+}  // namespace ramses
+
+#include "unsplit_args.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Config {
+  ramses::HydroParams hp{1.4, 1e-10, 1e-10, 1.5, 0.5, 1, ramses::kLLF};
+  int precision = RAMSES_FP64;
+  int nvector = 32;
+  bool set = false;
+};
+
+std::mutex g_cfg_mu;
+Config g_cfg;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* where) {
+  g_err = std::string(where) + ": " + hipGetErrorString(e);
+  return (int)e;
+}
+
+Config current() {
+  std::lock_guard<std::mutex> lk(g_cfg_mu);
+  return g_cfg;
+}
+
+// Device staging for the host-memory entry points.  The reference's unsplit
+// is not reentrant (its `save` scratch arrays, hydro/umuscl.f90:41-53); the
+// host-memory form is likewise serialised on one library-owned stream.
+struct Staging {
+  std::mutex mu;
+  hipStream_t stream = nullptr;
+  double* buf = nullptr;
+  size_t cap = 0;  // doubles
+  int ensure(size_t n) {
+    if (!stream) {
+      hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+      if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+    }
+    if (n <= cap) return 0;
+    if (buf) (void)hipFree(buf);
+    buf = nullptr;
+    cap = 0;
+    hipError_t e = hipMalloc(&buf, n * sizeof(double));
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    cap = n;
+    return 0;
+  }
+};
+Staging g_stage;
+
+}  // namespace
+
+extern "C" {
+
+int ramses_hydro_config(const ramses_hydro_params* p) {
+  if (!p) return fail(-1, "ramses_hydro_config: null parameters");
+  if (!(p->gamma > 1.0)) return fail(-2, "gamma must be > 1");
+  if (!(p->smallr > 0.0) || !(p->smallc > 0.0)) return fail(-3, "smallr and smallc must be > 0");
+  const int st = p->slope_type;
+  if (!(st == 0 || st == 1 || st == 2 || st == 3 || st == 7 || st == 8))
+    return fail(-4, "unsupported slope_type for NDIM=3 (supported: 0,1,2,3,7,8)");
+  if (p->riemann < RAMSES_RIEMANN_LLF || p->riemann > RAMSES_RIEMANN_HLL)
+    return fail(-5, "unsupported riemann solver (llf, hllc, hll)");
+  if (p->difmag != 0.0) return fail(-6, "difmag > 0 (artificial diffusion) is not supported");
+  if (p->precision != RAMSES_FP64 && p->precision != RAMSES_FP32_MIXED) return fail(-7, "bad precision");
+  if (p->nvector <= 0 || p->nvector > 4096) return fail(-8, "nvector out of range");
+  if (!(p->courant_factor > 0.0)) return fail(-9, "courant_factor must be > 0");
+  Config c;
+  c.hp.gamma = p->gamma;
+  c.hp.smallr = p->smallr;
+  c.hp.smallc = p->smallc;
+  c.hp.slope_theta = p->slope_theta;
+  c.hp.courant_factor = p->courant_factor;
+  c.hp.slope_type = p->slope_type;
+  c.hp.riemann = p->riemann;
+  c.precision = p->precision;
+  c.nvector = p->nvector;
+  c.set = true;
+  std::lock_guard<std::mutex> lk(g_cfg_mu);
+  g_cfg = c;
+  return 0;
+}
+
+const char* ramses_last_error(void) { return g_err.c_str(); }
+
+int ramses_unsplit_device(const double* uin, const double* gravin, double* flux, double* tmp, double dx,
+                          double dy, double dz, double dt, long ngroups, int ngrid_last, void* stream) {
+  const Config c = current();
+  if (ngroups < 0) return fail(-1, "ngroups < 0");
+  if (ngroups == 0) return 0;
+  if (ngrid_last < 0 || ngrid_last > c.nvector) return fail(-1, "ngrid_last out of [0, nvector]");
+  if (!uin || !flux || !tmp) return fail(-1, "null array");
+  if (!(dx > 0) || !(dy > 0) || !(dz > 0)) return fail(-1, "dx, dy, dz must be > 0");
+  if ((double)ngroups * 12.0 * c.nvector > 2147483647.0 * 256.0) return fail(-1, "batch too large");
+  ramses::UnsplitArgs a;
+  a.uin = uin;
+  a.grav = gravin;
+  a.flux = flux;
+  a.tmp = tmp;
+  a.dx[0] = dx;
+  a.dx[1] = dy;
+  a.dx[2] = dz;
+  a.dt = dt;
+  a.ngroups = ngroups;
+  a.ngrid_last = ngrid_last;
+  a.nv = c.nvector;
+  a.hp = c.hp;
+  hipError_t e = ramses::launch_unsplit(a, c.precision == RAMSES_FP32_MIXED, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "unsplit launch");
+  return 0;
+}
+
+int ramses_unsplit(const double* uin, const double* gravin, double* flux, double* tmp, double dx, double dy,
+                   double dz, double dt, int ngrid) {
+  const Config c = current();
+  if (!uin || !flux || !tmp) return fail(-1, "null array");
+  if (ngrid < 0 || ngrid > c.nvector) return fail(-1, "ngrid out of [0, nvector]");
+  if (ngrid == 0) return 0;
+  const size_t nv = (size_t)c.nvector;
+  const size_t nu = nv * 216 * 5, ng = nv * 216 * 3, nf = nv * 27 * 5 * 3, nt = nv * 27 * 2 * 3;
+  std::lock_guard<std::mutex> lk(g_stage.mu);
+  int rc = g_stage.ensure(nu + ng + nf + nt);
+  if (rc) return rc;
+  double* d_u = g_stage.buf;
+  double* d_g = d_u + nu;
+  double* d_f = d_g + ng;
+  double* d_t = d_f + nf;
+  hipStream_t s = g_stage.stream;
+  hipError_t e;
+  // flux/tmp travel both ways so that entries the reference never writes keep the caller's values
+  if ((e = hipMemcpyAsync(d_u, uin, nu * 8, hipMemcpyHostToDevice, s)) != hipSuccess) return hip_fail(e, "H2D uin");
+  if (gravin && (e = hipMemcpyAsync(d_g, gravin, ng * 8, hipMemcpyHostToDevice, s)) != hipSuccess)
+    return hip_fail(e, "H2D gravin");
+  if ((e = hipMemcpyAsync(d_f, flux, nf * 8, hipMemcpyHostToDevice, s)) != hipSuccess) return hip_fail(e, "H2D flux");
+  if ((e = hipMemcpyAsync(d_t, tmp, nt * 8, hipMemcpyHostToDevice, s)) != hipSuccess) return hip_fail(e, "H2D tmp");
+  rc = ramses_unsplit_device(d_u, gravin ? d_g : nullptr, d_f, d_t, dx, dy, dz, dt, 1, ngrid, s);
+  if (rc) return rc;
+  if ((e = hipMemcpyAsync(flux, d_f, nf * 8, hipMemcpyDeviceToHost, s)) != hipSuccess) return hip_fail(e, "D2H flux");
+  if ((e = hipMemcpyAsync(tmp, d_t, nt * 8, hipMemcpyDeviceToHost, s)) != hipSuccess) return hip_fail(e, "D2H tmp");
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "unsplit sync");
+  return 0;
+}
+
+void unsplit_(const double* uin, const double* gravin, const double* pin, double* flux, double* tmp,
+              const double* dx, const double* dy, const double* dz, const double* dt, const int* ngrid) {
+  (void)pin;  // declared but never read by the reference body (hydro/umuscl.f90:32)
+  const int rc = ramses_unsplit(uin, gravin, flux, tmp, *dx, *dy, *dz, *dt, *ngrid);
+  if (rc) {
+    std::fprintf(stderr, "unsplit: %s\n", ramses_last_error());
+    std::abort();
+  }
+}
+
+int ramses_cmpdt(const double* uu, const double* gg, double dx, int ncell, double* dt) {
+  const Config c = current();
+  if (!uu || !dt) return fail(-1, "null argument");
+  if (ncell < 0 || ncell > c.nvector) return fail(-1, "ncell out of [0, nvector]");
+  const size_t nv = (size_t)c.nvector;
+  std::lock_guard<std::mutex> lk(g_stage.mu);
+  int rc = g_stage.ensure(nv * 5 + nv * 3 + 1);
+  if (rc) return rc;
+  double* d_u = g_stage.buf;
+  double* d_g = d_u + nv * 5;
+  double* d_dt = d_g + nv * 3;
+  hipStream_t s = g_stage.stream;
+  hipError_t e;
+  if ((e = hipMemcpyAsync(d_u, uu, nv * 5 * 8, hipMemcpyHostToDevice, s)) != hipSuccess) return hip_fail(e, "H2D uu");
+  if (gg && (e = hipMemcpyAsync(d_g, gg, nv * 3 * 8, hipMemcpyHostToDevice, s)) != hipSuccess)
+    return hip_fail(e, "H2D gg");
+  if ((e = ramses::launch_cmpdt(d_u, gg ? d_g : nullptr, (int)nv, ncell, dx, c.hp, d_dt, s)) != hipSuccess)
+    return hip_fail(e, "cmpdt launch");
+  if ((e = hipMemcpyAsync(dt, d_dt, 8, hipMemcpyDeviceToHost, s)) != hipSuccess) return hip_fail(e, "D2H dt");
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "cmpdt sync");
+  return 0;
+}
+
+int ramses_uniform_dt_init(double* d_dt, double value, void* stream) {
+  if (!d_dt) return fail(-1, "null dt pointer");
+  hipError_t e = ramses::launch_set_double(d_dt, value, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "dt init launch");
+  return 0;
+}
+
+int ramses_uniform_courant(const double* u, int n, double dx, double dt_start, double* d_dt, void* stream) {
+  const Config c = current();
+  if (!u || !d_dt) return fail(-1, "null pointer");
+  if (n < 2) return fail(-1, "n must be >= 2");
+  if (!(dx > 0)) return fail(-1, "dx must be > 0");
+  // courant_fine starts from dtnew (dt_start); every cmpdt batch starts from
+  // courant_factor*dx/smallc (hydro/courant_fine.f90:35, godunov_utils.f90:114)
+  double start = c.hp.courant_factor * dx / c.hp.smallc;
+  if (dt_start < start) start = dt_start;
+  hipError_t e = ramses::launch_set_double(d_dt, start, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "dt init launch");
+  e = ramses::launch_courant(u, n, dx, c.hp, d_dt, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "courant launch");
+  return 0;
+}
+
+int ramses_uniform_godunov(const double* u, double* unew, int n, double dx, const double* d_dt, double* d_dt_next,
+                           void* stream) {
+  const Config c = current();
+  if (!u || !unew || !d_dt) return fail(-1, "null pointer");
+  if (u == unew) return fail(-1, "unew must not alias u");
+  if (n < 8 || n % 8 != 0) return fail(-1, "n must be a multiple of 8 (>= 8)");
+  if (!(dx > 0)) return fail(-1, "dx must be > 0");
+  if (c.hp.slope_type == 3) return fail(-4, "slope_type 3 is not supported on the resident level path");
+  hipError_t e = ramses::launch_uniform_godunov(u, unew, n, dx, d_dt, d_dt_next, c.hp,
+                                                c.precision == RAMSES_FP32_MIXED, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "uniform godunov launch");
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,443 @@
+// Device-side MUSCL-Hancock building blocks for gfx950, shared by every
+// kernel of the library.  Each routine follows one reference routine (cited
+// per function) and keeps its left-to-right evaluation order: the library is
+// compiled with -ffp-contract=off, so in FP64 every value rounds exactly as
+// the reference's own x86-64 build rounds it.
+//
+// Template parameter R is the arithmetic type of primitives, slopes, traced
+// states and Riemann fluxes: double (FP64 path) or float (FP32-mixed path).
+// Conservative state, E - kinetic recovery and the dt reduction are always
+// double.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ramses {
+
+enum Riemann : int { kLLF = 0, kHLLC = 1, kHLL = 2 };
+
+// hydro/hydro_parameters.f90:74-85 resolved once on the host.
+struct HydroParams {
+  double gamma;
+  double smallr;
+  double smallc;
+  double slope_theta;
+  double courant_factor;
+  int slope_type;  // 0,1,2,3,7,8
+  int riemann;     // Riemann enum
+};
+
+// Fortran MAX/MIN of two reals as the reference build evaluates them.
+template <typename R>
+__device__ __forceinline__ R fmax_f(R a, R b) { return (a > b) ? a : b; }
+template <typename R>
+__device__ __forceinline__ R fmin_f(R a, R b) { return (a < b) ? a : b; }
+
+// Division / square root in the working precision.  FP64 keeps IEEE
+// correctly rounded division and sqrt (the reference's semantics); the FP32
+// path uses the hardware reciprocal / sqrt (about 1 ulp), which is what makes
+// it cheaper than the FP64 path on the divide-heavy Riemann solve.
+__device__ __forceinline__ double div_w(double a, double b) { return a / b; }
+__device__ __forceinline__ float div_w(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double sqrt_w(double a) { return __builtin_sqrt(a); }
+__device__ __forceinline__ float sqrt_w(float a) { return __builtin_amdgcn_sqrtf(a); }
+__device__ __forceinline__ double abs_w(double a) { return __builtin_fabs(a); }
+__device__ __forceinline__ float abs_w(float a) { return __builtin_fabsf(a); }
+__device__ __forceinline__ double copysign_w(double a, double b) { return __builtin_copysign(a, b); }
+__device__ __forceinline__ float copysign_w(float a, float b) { return __builtin_copysignf(a, b); }
+
+// Primitive state of one cell: rho, vx, vy, vz, p.
+template <typename R>
+struct Prim {
+  R r, u, v, w, p;
+};
+
+// ctoprim for one cell (hydro/umuscl.f90:861-965, NENER=0, no sound speed:
+// the MUSCL trace does not read `c`).  Density floor, 1/rho, velocities and
+// E/rho - kinetic are evaluated in FP64; the result is rounded to R.
+template <typename R>
+__device__ __forceinline__ Prim<R> ctoprim_cell(double u0, double u1, double u2, double u3, double u4,
+                                                double gx, double gy, double gz, double dtxhalf,
+                                                double smalle, const HydroParams& hp) {
+  const double r = fmax_f(u0, hp.smallr);                  // :894
+  const double oneoverrho = 1.0 / r;                        // :897
+  const double vx = u1 * oneoverrho;                        // :898-904
+  const double vy = u2 * oneoverrho;
+  const double vz = u3 * oneoverrho;
+  double eken = 0.5 * vx * vx;                              // :907-913
+  eken = eken + 0.5 * vy * vy;
+  eken = eken + 0.5 * vz * vz;
+  const double erad = 0.0;
+  const double eint = fmax_f(u4 * oneoverrho - eken - erad, smalle);  // :923
+  const double p = (hp.gamma - 1.0) * r * eint;             // :924
+  Prim<R> q;
+  q.r = (R)r;
+  q.p = (R)p;
+  q.u = (R)(vx + gx * dtxhalf);                             // :936-942
+  q.v = (R)(vy + gy * dtxhalf);
+  q.w = (R)(vz + gz * dtxhalf);
+  return q;
+}
+
+// One-dimensional slope of the 3D uslope branches with axial stencils
+// (hydro/umuscl.f90:1241-1285 minmod, :1286-1327 moncen, :1393-1431 van
+// Leer, :1432-1473 generalised moncen).
+template <typename R>
+__device__ __forceinline__ R slope1(R qm1, R q0, R qp1, int st, R theta) {
+  const R zero = (R)0;
+  if (st == 1) {
+    const R dlft = q0 - qm1;
+    const R drgt = qp1 - q0;
+    if ((dlft * drgt) <= zero) return zero;
+    return (dlft > zero) ? fmin_f(dlft, drgt) : fmax_f(dlft, drgt);
+  } else if (st == 2) {
+    const R two = (R)2;
+    const R dlft = two * (q0 - qm1);
+    const R drgt = two * (qp1 - q0);
+    const R dcen = (R)0.5 * (dlft + drgt) / two;
+    const R dsgn = copysign_w((R)1, dcen);
+    R dlim = fmin_f(abs_w(dlft), abs_w(drgt));
+    if ((dlft * drgt) <= zero) dlim = zero;
+    return dsgn * fmin_f(dlim, abs_w(dcen));
+  } else if (st == 7) {
+    const R dlft = (q0 - qm1);
+    const R drgt = (qp1 - q0);
+    if ((dlft * drgt) <= zero) return zero;
+    return div_w((R)2 * dlft * drgt, (dlft + drgt));
+  } else if (st == 8) {
+    const R dlft = (q0 - qm1);
+    const R drgt = (qp1 - q0);
+    const R dcen = (R)0.5 * (dlft + drgt);
+    const R dsgn = copysign_w((R)1, dcen);
+    R dlim = fmin_f(theta * abs_w(dlft), theta * abs_w(drgt));
+    if ((dlft * drgt) <= zero) dlim = zero;
+    return dsgn * fmin_f(dlim, abs_w(dcen));
+  }
+  return zero;  // slope_type 0 (:1002-1005)
+}
+
+// Slopes of the five primitives in one direction.
+template <typename R>
+__device__ __forceinline__ Prim<R> slope_prim(const Prim<R>& a, const Prim<R>& b, const Prim<R>& c,
+                                              int st, R theta) {
+  Prim<R> d;
+  d.r = slope1(a.r, b.r, c.r, st, theta);
+  d.u = slope1(a.u, b.u, c.u, st, theta);
+  d.v = slope1(a.v, b.v, c.v, st, theta);
+  d.w = slope1(a.w, b.w, c.w, st, theta);
+  d.p = slope1(a.p, b.p, c.p, st, theta);
+  return d;
+}
+
+// Source terms of the Hancock predictor (hydro/umuscl.f90:576-580).
+template <typename R>
+struct Source {
+  R sr0, su0, sv0, sw0, sp0;
+};
+
+template <typename R>
+__device__ __forceinline__ Source<R> trace_sources(const Prim<R>& q, const Prim<R>& dx_,
+                                                   const Prim<R>& dy_, const Prim<R>& dz_, R gamma) {
+  const R r = q.r, u = q.u, v = q.v, w = q.w, p = q.p;
+  Source<R> s;
+  s.sr0 = -u * dx_.r - v * dy_.r - w * dz_.r - (dx_.u + dy_.v + dz_.w) * r;
+  s.sp0 = -u * dx_.p - v * dy_.p - w * dz_.p - (dx_.u + dy_.v + dz_.w) * gamma * p;
+  s.su0 = -u * dx_.u - v * dy_.u - w * dz_.u - div_w((dx_.p), r);
+  s.sv0 = -u * dx_.v - v * dy_.v - w * dz_.v - div_w((dy_.p), r);
+  s.sw0 = -u * dx_.w - v * dy_.w - w * dz_.w - div_w((dz_.p), r);
+  return s;
+}
+
+// Right (qp, minus side) and left (qm, plus side) traced states along one
+// direction with slope dd and dt/dx = dtdx (hydro/umuscl.f90:591-673).
+template <typename R>
+__device__ __forceinline__ void trace_dir(const Prim<R>& q, const Prim<R>& dd, const Source<R>& s, R dtdx,
+                                          R smallr, Prim<R>& qp, Prim<R>& qm) {
+  const R half = (R)0.5;
+  qp.r = q.r - half * dd.r + s.sr0 * dtdx * half;
+  qp.p = q.p - half * dd.p + s.sp0 * dtdx * half;
+  qp.u = q.u - half * dd.u + s.su0 * dtdx * half;
+  qp.v = q.v - half * dd.v + s.sv0 * dtdx * half;
+  qp.w = q.w - half * dd.w + s.sw0 * dtdx * half;
+  if (qp.r < smallr) qp.r = q.r;
+  qm.r = q.r + half * dd.r + s.sr0 * dtdx * half;
+  qm.p = q.p + half * dd.p + s.sp0 * dtdx * half;
+  qm.u = q.u + half * dd.u + s.su0 * dtdx * half;
+  qm.v = q.v + half * dd.v + s.sv0 * dtdx * half;
+  qm.w = q.w + half * dd.w + s.sw0 * dtdx * half;
+  if (qm.r < smallr) qm.r = q.r;
+}
+
+// Interface state in the rotated order of cmpflxm (hydro/umuscl.f90:749-780):
+// rho, normal velocity, pressure, tangential 1, tangential 2.
+template <typename R>
+struct Face {
+  R r, un, p, ut1, ut2;
+};
+
+template <typename R>
+__device__ __forceinline__ Face<R> rotate(const Prim<R>& q, int d) {
+  Face<R> f;
+  f.r = q.r;
+  f.p = q.p;
+  if (d == 0) { f.un = q.u; f.ut1 = q.v; f.ut2 = q.w; }
+  else if (d == 1) { f.un = q.v; f.ut1 = q.u; f.ut2 = q.w; }
+  else { f.un = q.w; f.ut1 = q.u; f.ut2 = q.v; }
+  return f;
+}
+
+// Godunov flux in rotated order plus the internal-energy flux (fgdnv(1:6)).
+template <typename R>
+struct Flux6 {
+  R f[6];
+};
+
+// riemann_llf: hydro/godunov_utils.f90:660-820
+template <typename R>
+__device__ __forceinline__ Flux6<R> riemann_llf(const Face<R>& L, const Face<R>& Rt, R gamma, R smallr,
+                                                R smallp, R entho) {
+  const R half = (R)0.5;
+  const R rl = fmax_f(L.r, smallr), ul = L.un, pl = fmax_f(L.p, rl * smallp);
+  R cl = gamma * pl;
+  cl = sqrt_w(div_w(cl, rl));
+  const R rr = fmax_f(Rt.r, smallr), ur = Rt.un, pr = fmax_f(Rt.p, rr * smallp);
+  R cr = gamma * pr;
+  cr = sqrt_w(div_w(cr, rr));
+  const R cmax = fmax_f(abs_w(ul) + cl, abs_w(ur) + cr);
+  R uL[6], uR[6], fL[6], fR[6];
+  uL[0] = L.r;
+  uR[0] = Rt.r;
+  uL[1] = L.r * L.un;
+  uR[1] = Rt.r * Rt.un;
+  uL[2] = L.p * entho + half * L.r * (L.un * L.un);
+  uR[2] = Rt.p * entho + half * Rt.r * (Rt.un * Rt.un);
+  uL[2] = uL[2] + half * L.r * (L.ut1 * L.ut1);
+  uR[2] = uR[2] + half * Rt.r * (Rt.ut1 * Rt.ut1);
+  uL[2] = uL[2] + half * L.r * (L.ut2 * L.ut2);
+  uR[2] = uR[2] + half * Rt.r * (Rt.ut2 * Rt.ut2);
+  uL[3] = L.r * L.ut1;
+  uR[3] = Rt.r * Rt.ut1;
+  uL[4] = L.r * L.ut2;
+  uR[4] = Rt.r * Rt.ut2;
+  uL[5] = L.p * entho;
+  uR[5] = Rt.p * entho;
+  fL[0] = L.un * uL[0];
+  fR[0] = Rt.un * uR[0];
+  fL[1] = L.un * uL[1] + L.p;
+  fR[1] = Rt.un * uR[1] + Rt.p;
+  fL[2] = L.un * (uL[2] + L.p);
+  fR[2] = Rt.un * (uR[2] + Rt.p);
+#pragma unroll
+  for (int n = 3; n < 6; ++n) {
+    fL[n] = L.un * uL[n];
+    fR[n] = Rt.un * uR[n];
+  }
+  Flux6<R> g;
+#pragma unroll
+  for (int n = 0; n < 6; ++n) g.f[n] = half * (fL[n] + fR[n] - cmax * (uR[n] - uL[n]));
+  return g;
+}
+
+// riemann_hll: hydro/godunov_utils.f90:825-983
+template <typename R>
+__device__ __forceinline__ Flux6<R> riemann_hll(const Face<R>& L, const Face<R>& Rt, R gamma, R smallr,
+                                                R smallp, R entho) {
+  const R half = (R)0.5, zero = (R)0;
+  const R rl = fmax_f(L.r, smallr), ul = L.un, pl = fmax_f(L.p, rl * smallp);
+  R cl = gamma * pl;
+  cl = sqrt_w(div_w(cl, rl));
+  const R rr = fmax_f(Rt.r, smallr), ur = Rt.un, pr = fmax_f(Rt.p, rr * smallp);
+  R cr = gamma * pr;
+  cr = sqrt_w(div_w(cr, rr));
+  const R SL = fmin_f(fmin_f(ul, ur) - fmax_f(cl, cr), zero);
+  const R SR = fmax_f(fmax_f(ul, ur) + fmax_f(cl, cr), zero);
+  R uL[6], uR[6], fL[6], fR[6];
+  uL[0] = L.r;
+  uR[0] = Rt.r;
+  uL[1] = L.r * L.un;
+  uR[1] = Rt.r * Rt.un;
+  uL[2] = L.p * entho + half * L.r * (L.un * L.un);
+  uR[2] = Rt.p * entho + half * Rt.r * (Rt.un * Rt.un);
+  uL[2] = uL[2] + half * L.r * (L.ut1 * L.ut1);
+  uR[2] = uR[2] + half * Rt.r * (Rt.ut1 * Rt.ut1);
+  uL[2] = uL[2] + half * L.r * (L.ut2 * L.ut2);
+  uR[2] = uR[2] + half * Rt.r * (Rt.ut2 * Rt.ut2);
+  uL[3] = L.r * L.ut1;
+  uR[3] = Rt.r * Rt.ut1;
+  uL[4] = L.r * L.ut2;
+  uR[4] = Rt.r * Rt.ut2;
+  uL[5] = L.p * entho;
+  uR[5] = Rt.p * entho;
+  fL[0] = uL[1];
+  fR[0] = uR[1];
+  fL[1] = L.p + uL[1] * L.un;
+  fR[1] = Rt.p + uR[1] * Rt.un;
+  fL[2] = L.un * (uL[2] + L.p);
+  fR[2] = Rt.un * (uR[2] + Rt.p);
+#pragma unroll
+  for (int n = 3; n < 6; ++n) {
+    fL[n] = L.un * uL[n];
+    fR[n] = Rt.un * uR[n];
+  }
+  Flux6<R> g;
+  const R den = SR - SL;
+#pragma unroll
+  for (int n = 0; n < 6; ++n) g.f[n] = div_w(SR * fL[n] - SL * fR[n] + SR * SL * (uR[n] - uL[n]), den);
+  return g;
+}
+
+// riemann_hllc: hydro/godunov_utils.f90:988-1209.  Only the star state of
+// the sampled branch feeds the flux, so the other branch's divisions are
+// skipped; the values of the sampled quantities are unchanged.
+template <typename R>
+__device__ __forceinline__ Flux6<R> riemann_hllc(const Face<R>& L, const Face<R>& Rt, R gamma, R smallr,
+                                                 R smallp, R entho, R smallc2) {
+  const R half = (R)0.5, zero = (R)0;
+  const R rl = fmax_f(L.r, smallr);
+  const R Pl = fmax_f(L.p, rl * smallp);
+  const R ul = L.un;
+  const R el = Pl * entho;
+  R ecinl = half * rl * ul * ul;
+  ecinl = ecinl + half * rl * (L.ut1 * L.ut1);
+  ecinl = ecinl + half * rl * (L.ut2 * L.ut2);
+  const R etotl = el + ecinl;
+  const R Ptotl = Pl;
+  const R rr = fmax_f(Rt.r, smallr);
+  const R Pr = fmax_f(Rt.p, rr * smallp);
+  const R ur = Rt.un;
+  const R er = Pr * entho;
+  R ecinr = half * rr * ur * ur;
+  ecinr = ecinr + half * rr * (Rt.ut1 * Rt.ut1);
+  ecinr = ecinr + half * rr * (Rt.ut2 * Rt.ut2);
+  const R etotr = er + ecinr;
+  const R Ptotr = Pr;
+  R cfastl = gamma * Pl;
+  cfastl = sqrt_w(fmax_f(div_w(cfastl, rl), smallc2));
+  R cfastr = gamma * Pr;
+  cfastr = sqrt_w(fmax_f(div_w(cfastr, rr), smallc2));
+  const R SL = fmin_f(ul, ur) - fmax_f(cfastl, cfastr);
+  const R SR = fmax_f(ul, ur) + fmax_f(cfastl, cfastr);
+  const R rcl = rl * (ul - SL);
+  const R rcr = rr * (SR - ur);
+  const R ustar = div_w(rcr * ur + rcl * ul + (Ptotl - Ptotr), (rcr + rcl));
+  R ro, uo, Ptoto, etoto, eo;
+  if (SL > zero) {
+    ro = rl; uo = ul; Ptoto = Ptotl; etoto = etotl; eo = el;
+  } else if (SR > zero || ustar > zero) {
+    const R Ptotstar = div_w(rcr * Ptotl + rcl * Ptotr + rcl * rcr * (ul - ur), (rcr + rcl));
+    // left star state when ustar>0, right star state otherwise (SR>0)
+    const bool left = ustar > zero;
+    const R S = left ? SL : SR;
+    const R us = left ? ul : ur;
+    const R rs = left ? rl : rr;
+    const R es = left ? el : er;
+    const R ets = left ? etotl : etotr;
+    const R Pts = left ? Ptotl : Ptotr;
+    const R den = S - ustar;
+    ro = div_w(rs * (S - us), den);
+    etoto = div_w((S - us) * ets - Pts * us + Ptotstar * ustar, den);
+    eo = div_w(es * (S - us), den);
+    uo = ustar;
+    Ptoto = Ptotstar;
+  } else {
+    ro = rr; uo = ur; Ptoto = Ptotr; etoto = etotr; eo = er;
+  }
+  Flux6<R> g;
+  g.f[0] = ro * uo;
+  g.f[1] = ro * uo * uo + Ptoto;
+  g.f[2] = (etoto + Ptoto) * uo;
+  if (ustar > zero) {
+    g.f[3] = ro * uo * L.ut1;
+    g.f[4] = ro * uo * L.ut2;
+  } else {
+    g.f[3] = ro * uo * Rt.ut1;
+    g.f[4] = ro * uo * Rt.ut2;
+  }
+  g.f[5] = uo * eo;
+  return g;
+}
+
+// Solver constants in working precision.
+template <typename R>
+struct SolverConsts {
+  R gamma, smallr, smallp, entho, smallc2, theta;
+  int riemann, slope_type;
+};
+
+template <typename R>
+__host__ __device__ inline SolverConsts<R> make_consts(const HydroParams& hp) {
+  SolverConsts<R> c;
+  c.gamma = (R)hp.gamma;
+  c.smallr = (R)hp.smallr;
+  c.smallp = (R)(hp.smallc * hp.smallc / hp.gamma);  // :683, :1017
+  c.entho = (R)(1.0 / (hp.gamma - 1.0));
+  c.smallc2 = (R)(hp.smallc * hp.smallc);
+  c.theta = (R)hp.slope_theta;
+  c.riemann = hp.riemann;
+  c.slope_type = hp.slope_type;
+  return c;
+}
+
+template <typename R>
+__device__ __forceinline__ Flux6<R> riemann_solve(const Face<R>& L, const Face<R>& Rt, const SolverConsts<R>& c) {
+  if (c.riemann == kHLLC) return riemann_hllc(L, Rt, c.gamma, c.smallr, c.smallp, c.entho, c.smallc2);
+  if (c.riemann == kHLL) return riemann_hll(L, Rt, c.gamma, c.smallr, c.smallp, c.entho);
+  return riemann_llf(L, Rt, c.gamma, c.smallr, c.smallp, c.entho);
+}
+
+template <typename R, int RS>
+__device__ __forceinline__ Flux6<R> riemann_fixed(const Face<R>& L, const Face<R>& Rt, const SolverConsts<R>& c) {
+  if constexpr (RS == kHLLC) return riemann_hllc(L, Rt, c.gamma, c.smallr, c.smallp, c.entho, c.smallc2);
+  else if constexpr (RS == kHLL) return riemann_hll(L, Rt, c.gamma, c.smallr, c.smallp, c.entho);
+  else return riemann_llf(L, Rt, c.gamma, c.smallr, c.smallp, c.entho);
+}
+
+// Conservative-variable flux of a face in (rho, mx, my, mz, E) order
+// (cmpflxm unpacking, hydro/umuscl.f90:809-842).
+template <typename R>
+__device__ __forceinline__ void unrotate(const Flux6<R>& g, int d, R out[5]) {
+  out[0] = g.f[0];
+  out[4] = g.f[2];
+  if (d == 0) { out[1] = g.f[1]; out[2] = g.f[3]; out[3] = g.f[4]; }
+  else if (d == 1) { out[2] = g.f[1]; out[1] = g.f[3]; out[3] = g.f[4]; }
+  else { out[3] = g.f[1]; out[1] = g.f[3]; out[2] = g.f[4]; }
+}
+
+// cmpdt for one cell (hydro/godunov_utils.f90:5-120): the per-cell dt limit
+// dx/w*(sqrt(1+2*C*r)-1)/r with w = 3*c + |vx|+|vy|+|vz| and
+// r = max(|g|_1*dx/w^2, 1e-4).  Always FP64.
+__device__ __forceinline__ double cmpdt_cell(double u0, double u1, double u2, double u3, double u4,
+                                             double gx, double gy, double gz, double dx,
+                                             const HydroParams& hp) {
+  const double gamma = hp.gamma;
+  const double smallp = hp.smallc * hp.smallc / gamma;
+  const double rho = fmax_f(u0, hp.smallr);
+  const double vx = u1 / rho, vy = u2 / rho, vz = u3 / rho;
+  double e = u4;
+  e = e - 0.5 * rho * (vx * vx);
+  e = e - 0.5 * rho * (vy * vy);
+  e = e - 0.5 * rho * (vz * vz);
+  double c = fmax_f((gamma - 1.0) * e, rho * smallp);
+  c = gamma * c;
+  c = __builtin_sqrt(c / rho);
+  double w = 3.0 * c;
+  w = w + __builtin_fabs(vx);
+  w = w + __builtin_fabs(vy);
+  w = w + __builtin_fabs(vz);
+  double r = 0.0;
+  r = r + __builtin_fabs(gx);
+  r = r + __builtin_fabs(gy);
+  r = r + __builtin_fabs(gz);
+  r = r * dx / (w * w);
+  r = fmax_f(r, 0.0001);
+  return dx / w * (__builtin_sqrt(1.0 + 2.0 * hp.courant_factor * r) - 1.0) / r;
+}
+
+// Positive doubles order like their bit patterns, so a min-reduction of dt
+// can use an integer atomic.
+__device__ __forceinline__ unsigned long long dt_bits(double x) {
+  return (unsigned long long)__double_as_longlong(x);
+}
+
+}  // namespace ramses

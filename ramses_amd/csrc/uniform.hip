@@ -1,0 +1,327 @@
+// Resident uniform periodic level: the godunov_fine -> godfine1 sweep
+// (hydro/godunov_fine.f90:5-35, 486-911) with set_unew/set_uold replaced by
+// ping-pong buffers, and the courant_fine dt reduction
+// (hydro/courant_fine.f90:1-159) fused into the update epilogue.
+//
+// State: u[var][z][y][x], n^3 doubles per variable (SoA, x fastest).
+//
+// Kernel structure (2.5D blocking).  A workgroup owns an interior tile of
+// TX x TY columns and marches along z through a chunk of TZ planes.  Its
+// threads cover the tile extended by one cell on each side in x and y, so
+// the traced states of the halo cells that face the tile are computed by a
+// thread like any other.  Per plane:
+//   1. primitives of the next plane (tile + 2 cells of halo) -> LDS,
+//   2. each thread computes slopes and traced states of its own cell from
+//      its column (registers) and its x/y neighbours (LDS),
+//   3. left states are exchanged through LDS and x/y face fluxes solved,
+//   4. the z face below the cell is solved from the previous plane's traced
+//      state kept in registers, completing the update of the plane below,
+//   5. the new state is written and its cmpdt limit min-reduced (wavefront
+//      shuffles, then LDS across waves, then one 64-bit atomic min on the
+//      bit pattern of the positive double).
+// Each primitive is computed by one ctoprim per cell per workgroup, each
+// face flux once, and every value keeps the reference's operation order, so
+// the FP64 path reproduces godfine1+unsplit on a uniform grid bit for bit.
+#include "hydro_device.hpp"
+
+namespace ramses {
+
+namespace {
+
+struct UniformArgs {
+  const double* u;
+  double* unew;
+  const double* d_dt;     // time step (device)
+  double* d_dt_next;      // fused courant output or null
+  int n;
+  double dx;
+  HydroParams hp;
+};
+
+__device__ __forceinline__ int wrapi(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+
+template <typename R, int TX, int TY, int RS>
+__global__ __launch_bounds__(((TX + 2) * (TY + 2) + 63) / 64 * 64) void uniform_godunov_kernel(UniformArgs a,
+                                                                                               int TZ) {
+  constexpr int EX = TX + 2, EY = TY + 2;      // extended tile (threads)
+  constexpr int QX = TX + 4, QY = TY + 4;      // primitive plane with 2-cell halo
+  constexpr int NQ = QX * QY;
+  constexpr int NE = EX * EY;
+  constexpr int NT = (NE + 63) / 64 * 64;
+
+  __shared__ R qs[2][5][NQ];   // primitive planes (ring of 2)
+  __shared__ __align__(16) R xch[10][NE];  // exchange: traced left states, then fluxes
+  __shared__ double red[NT / 64];
+
+  const int n = a.n;
+  const long plane = (long)n * n;
+  const long vstride = plane * n;
+  const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = blockIdx.z * TZ;
+  const int t = threadIdx.x;
+  const bool active = t < NE;
+  const int ex = active ? t % EX : 0, ey = active ? t / EX : 0;
+  const bool inx = ex >= 1 && ex <= TX, iny = ey >= 1 && ey <= TY;
+  const bool interior = active && inx && iny;
+  const int xg = wrapi(x0 - 1 + ex, n), yg = wrapi(y0 - 1 + ey, n);
+  const int qo = (ey + 1) * QX + (ex + 1);  // own cell in the primitive plane
+
+  const HydroParams hp = a.hp;
+  const SolverConsts<R> sc = make_consts<R>(hp);
+  const double dt = *a.d_dt;
+  const double dx = a.dx;
+  const double dtxhalf = dt * 0.5;
+  const double smalle = hp.smallc * hp.smallc / hp.gamma / (hp.gamma - 1.0);
+  const R dtdx = (R)(dt / dx);
+
+  // cooperative ctoprim of plane zp (periodic) into slot s
+  auto load_plane = [&](int zp, int s) {
+    const int zg = wrapi(zp, n);
+    for (int c = t; c < NQ; c += NT) {
+      const int qx = c % QX, qy = c / QX;
+      const int gx = wrapi(x0 - 2 + qx, n), gy = wrapi(y0 - 2 + qy, n);
+      const long o = (long)zg * plane + (long)gy * n + gx;
+      const Prim<R> q = ctoprim_cell<R>(a.u[o], a.u[o + vstride], a.u[o + 2 * vstride], a.u[o + 3 * vstride],
+                                        a.u[o + 4 * vstride], 0.0, 0.0, 0.0, dtxhalf, smalle, hp);
+      qs[s][0][c] = q.r;
+      qs[s][1][c] = q.u;
+      qs[s][2][c] = q.v;
+      qs[s][3][c] = q.w;
+      qs[s][4][c] = q.p;
+    }
+  };
+  auto read_q = [&](int s, int c) {
+    return Prim<R>{qs[s][0][c], qs[s][1][c], qs[s][2][c], qs[s][3][c], qs[s][4][c]};
+  };
+
+  // own-column primitives at p-1 and p
+  Prim<R> qprev{}, qcur{};
+  // values carried from the previous plane
+  Face<R> fz_left{};          // rotated qm_z of plane p-1
+  double fz_prev[5] = {0, 0, 0, 0, 0};  // flux through the z face below plane p-1 (scaled)
+  double dfx_prev[5] = {0, 0, 0, 0, 0}, dfy_prev[5] = {0, 0, 0, 0, 0};
+  double dtmin = 1e300;
+
+  // prologue: planes z0-2 and z0-1
+  load_plane(z0 - 2, 0);
+  __syncthreads();
+  if (active) qprev = read_q(0, qo);
+  __syncthreads();
+  load_plane(z0 - 1, 1);
+  __syncthreads();
+  if (active) qcur = read_q(1, qo);
+
+  for (int p = z0 - 1; p <= z0 + TZ; ++p) {
+    const int scur = (p - z0 + 2) & 1;  // slot of plane p (z0-2 -> 0, z0-1 -> 1)
+    const int snext = scur ^ 1;
+    __syncthreads();
+    load_plane(p + 1, snext);
+    __syncthreads();
+    const bool faces_xy = (p >= z0) && (p < z0 + TZ);
+    Prim<R> qnext{};
+    Prim<R> qp_x{}, qm_x{}, qp_y{}, qm_y{}, qp_z{}, qm_z{};
+    if (active) {
+      qnext = read_q(snext, qo);
+      const Prim<R> xm = read_q(scur, qo - 1), xp = read_q(scur, qo + 1);
+      const Prim<R> ym = read_q(scur, qo - QX), yp = read_q(scur, qo + QX);
+      const Prim<R> dqx = slope_prim(xm, qcur, xp, sc.slope_type, sc.theta);
+      const Prim<R> dqy = slope_prim(ym, qcur, yp, sc.slope_type, sc.theta);
+      const Prim<R> dqz = slope_prim(qprev, qcur, qnext, sc.slope_type, sc.theta);
+      const Source<R> src = trace_sources(qcur, dqx, dqy, dqz, sc.gamma);
+      trace_dir(qcur, dqx, src, dtdx, sc.smallr, qp_x, qm_x);
+      trace_dir(qcur, dqy, src, dtdx, sc.smallr, qp_y, qm_y);
+      trace_dir(qcur, dqz, src, dtdx, sc.smallr, qp_z, qm_z);
+    }
+    double dfx[5] = {0, 0, 0, 0, 0}, dfy[5] = {0, 0, 0, 0, 0};
+    if (faces_xy) {
+      if (active) {
+        xch[0][t] = qm_x.r; xch[1][t] = qm_x.u; xch[2][t] = qm_x.v; xch[3][t] = qm_x.w; xch[4][t] = qm_x.p;
+        xch[5][t] = qm_y.r; xch[6][t] = qm_y.u; xch[7][t] = qm_y.v; xch[8][t] = qm_y.w; xch[9][t] = qm_y.p;
+      }
+      __syncthreads();
+      double fxs[5] = {0, 0, 0, 0, 0}, fys[5] = {0, 0, 0, 0, 0};
+      if (active && iny && ex >= 1) {  // x face at x-1/2
+        const Prim<R> l{xch[0][t - 1], xch[1][t - 1], xch[2][t - 1], xch[3][t - 1], xch[4][t - 1]};
+        const Flux6<R> g = riemann_fixed<R, RS>(rotate(l, 0), rotate(qp_x, 0), sc);
+        R fc[5];
+        unrotate(g, 0, fc);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) fxs[v] = (double)fc[v] * dt / dx;  // hydro/umuscl.f90:106
+      }
+      if (active && inx && ey >= 1) {  // y face at y-1/2
+        const Prim<R> l{xch[5][t - EX], xch[6][t - EX], xch[7][t - EX], xch[8][t - EX], xch[9][t - EX]};
+        const Flux6<R> g = riemann_fixed<R, RS>(rotate(l, 1), rotate(qp_y, 1), sc);
+        R fc[5];
+        unrotate(g, 1, fc);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) fys[v] = (double)fc[v] * dt / dx;  // :129
+      }
+      __syncthreads();
+      // fluxes are stored in double in the exchange area reinterpreted per variable pair
+      double* fx_sh = reinterpret_cast<double*>(&xch[0][0]);
+      constexpr int FSTRIDE = NE;  // doubles per variable
+      static_assert(sizeof(R) * 10 * NE >= sizeof(double) * 5 * NE, "exchange area too small");
+      if (active) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) fx_sh[v * FSTRIDE + t] = fxs[v];
+      }
+      __syncthreads();
+      if (interior) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) dfx[v] = fx_sh[v * FSTRIDE + t] - fx_sh[v * FSTRIDE + t + 1];
+      }
+      __syncthreads();
+      if (active) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) fx_sh[v * FSTRIDE + t] = fys[v];
+      }
+      __syncthreads();
+      if (interior) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) dfy[v] = fx_sh[v * FSTRIDE + t] - fx_sh[v * FSTRIDE + t + EX];
+      }
+    }
+    // z face below plane p (between p-1 and p)
+    double fz[5] = {0, 0, 0, 0, 0};
+    if (interior && p >= z0) {
+      const Flux6<R> g = riemann_fixed<R, RS>(fz_left, rotate(qp_z, 2), sc);
+      R fc[5];
+      unrotate(g, 2, fc);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) fz[v] = (double)fc[v] * dt / dx;  // :153
+    }
+    // plane p-1 is complete: hydro/godunov_fine.f90:751-792 in idim order x, y, z
+    if (interior && p >= z0 + 1) {
+      const int zg = wrapi(p - 1, n);
+      const long o = (long)zg * plane + (long)yg * n + xg;
+      double un[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        double s = a.u[o + v * vstride];
+        s = s + dfx_prev[v];
+        s = s + dfy_prev[v];
+        s = s + (fz_prev[v] - fz[v]);
+        un[v] = s;
+        a.unew[o + v * vstride] = s;
+      }
+      if (a.d_dt_next) dtmin = fmin_f(dtmin, cmpdt_cell(un[0], un[1], un[2], un[3], un[4], 0.0, 0.0, 0.0, dx, hp));
+    }
+    // carry to the next plane
+    fz_left = rotate(qm_z, 2);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      fz_prev[v] = fz[v];
+      dfx_prev[v] = dfx[v];
+      dfy_prev[v] = dfy[v];
+    }
+    qprev = qcur;
+    qcur = qnext;
+  }
+
+  if (a.d_dt_next) {
+    // wavefront min with shuffles, then across the block's waves
+    for (int off = 32; off > 0; off >>= 1) dtmin = fmin_f(dtmin, __shfl_xor(dtmin, off, 64));
+    const int wave = t >> 6, lane = t & 63;
+    if (lane == 0) red[wave] = dtmin;
+    __syncthreads();
+    if (t == 0) {
+      double m = red[0];
+      for (int w = 1; w < NT / 64; ++w) m = fmin_f(m, red[w]);
+      atomicMin(reinterpret_cast<unsigned long long*>(a.d_dt_next), dt_bits(m));
+    }
+  }
+}
+
+__global__ void courant_kernel(const double* u, long ncell, double dx, HydroParams hp, double* d_dt) {
+  double dtmin = 1e300;
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += (long)gridDim.x * blockDim.x)
+    dtmin = fmin_f(dtmin, cmpdt_cell(u[c], u[c + ncell], u[c + 2 * ncell], u[c + 3 * ncell], u[c + 4 * ncell],
+                                     0.0, 0.0, 0.0, dx, hp));
+  for (int off = 32; off > 0; off >>= 1) dtmin = fmin_f(dtmin, __shfl_xor(dtmin, off, 64));
+  __shared__ double red[4];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) red[wave] = dtmin;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmin_f(m, red[w]);
+    atomicMin(reinterpret_cast<unsigned long long*>(d_dt), dt_bits(m));
+  }
+}
+
+__global__ void set_double(double* p, double v) { *p = v; }
+
+__global__ void cmpdt_group(const double* uu, const double* gg, int nv, int ncell, double dx, HydroParams hp,
+                            double* d_dt) {
+  // hydro/godunov_utils.f90:114-118: dt = min(C*dx/smallc, min over cells)
+  double dt = hp.courant_factor * dx / hp.smallc;
+  for (int k = threadIdx.x; k < ncell; k += blockDim.x) {
+    const double gx = gg ? gg[k] : 0.0, gy = gg ? gg[k + nv] : 0.0, gz = gg ? gg[k + 2 * nv] : 0.0;
+    dt = fmin_f(dt, cmpdt_cell(uu[k], uu[k + nv], uu[k + 2 * nv], uu[k + 3 * nv], uu[k + 4 * nv], gx, gy, gz, dx, hp));
+  }
+  for (int off = 32; off > 0; off >>= 1) dt = fmin_f(dt, __shfl_xor(dt, off, 64));
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmin_f(m, red[w]);
+    *d_dt = m;
+  }
+}
+
+template <typename R, int TX, int TY>
+hipError_t launch_tile(const UniformArgs& a, int TZ, hipStream_t s) {
+  constexpr int NT = ((TX + 2) * (TY + 2) + 63) / 64 * 64;
+  dim3 grid(a.n / TX, a.n / TY, a.n / TZ);
+  switch (a.hp.riemann) {
+    case kHLLC: hipLaunchKernelGGL((uniform_godunov_kernel<R, TX, TY, kHLLC>), grid, dim3(NT), 0, s, a, TZ); break;
+    case kHLL: hipLaunchKernelGGL((uniform_godunov_kernel<R, TX, TY, kHLL>), grid, dim3(NT), 0, s, a, TZ); break;
+    default: hipLaunchKernelGGL((uniform_godunov_kernel<R, TX, TY, kLLF>), grid, dim3(NT), 0, s, a, TZ); break;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_cmpdt(const double* uu, const double* gg, int nv, int ncell, double dx, const HydroParams& hp,
+                        double* d_dt, hipStream_t stream) {
+  hipLaunchKernelGGL(cmpdt_group, dim3(1), dim3(256), 0, stream, uu, gg, nv, ncell, dx, hp, d_dt);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_double(double* p, double v, hipStream_t s) {
+  hipLaunchKernelGGL(set_double, dim3(1), dim3(1), 0, s, p, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_courant(const double* u, int n, double dx, const HydroParams& hp, double* d_dt, hipStream_t s) {
+  const long ncell = (long)n * n * n;
+  long blocks = (ncell + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(courant_kernel, dim3((unsigned)blocks), dim3(256), 0, s, u, ncell, dx, hp, d_dt);
+  return hipGetLastError();
+}
+
+// Tile choice: the largest supported interior tile dividing n, and a z chunk
+// that leaves enough workgroups to fill the chip.
+hipError_t launch_uniform_godunov(const double* u, double* unew, int n, double dx, const double* d_dt,
+                                  double* d_dt_next, const HydroParams& hp, bool fp32, hipStream_t s) {
+  UniformArgs a{u, unew, d_dt, d_dt_next, n, dx, hp};
+  int TX, TY;
+  if (n % 32 == 0) { TX = 32; TY = 8; }
+  else if (n % 16 == 0) { TX = 16; TY = 8; }
+  else { TX = 8; TY = 8; }
+  const long tiles = (long)(n / TX) * (n / TY);
+  int TZ = n;
+  while (TZ > 4 && TZ % 2 == 0 && tiles * (n / TZ) < 2048) TZ /= 2;
+  if (fp32) {
+    if (TX == 32) return launch_tile<float, 32, 8>(a, TZ, s);
+    if (TX == 16) return launch_tile<float, 16, 8>(a, TZ, s);
+    return launch_tile<float, 8, 8>(a, TZ, s);
+  }
+  if (TX == 32) return launch_tile<double, 32, 8>(a, TZ, s);
+  if (TX == 16) return launch_tile<double, 16, 8>(a, TZ, s);
+  return launch_tile<double, 8, 8>(a, TZ, s);
+}
+
+}  // namespace ramses

@@ -1,0 +1,82 @@
+"""Resident uniform periodic level (levelmin == levelmax).
+
+Mirrors the reference's per-level step on a fully refined periodic grid
+(amr/amr_step.f90:327-424): ``newdt_fine -> courant_fine`` gives dt
+(pm/newdt_fine.f90:51,177; hydro/courant_fine.f90:1-159), ``set_unew``,
+``godunov_fine`` sweeps ``godfine1`` over every oct
+(hydro/godunov_fine.f90:5-35,486-911) and ``set_uold`` copies the result
+back.  Here the conserved state stays in device memory as
+``u[var][z][y][x]``; set_unew/set_uold become a swap of two buffers and the
+courant reduction of the new state is fused into the update kernel, so a
+step is one kernel launch and no host synchronisation.
+
+torch is used only to own device memory and streams.
+"""
+import ctypes
+
+from . import _check, _stream_handle, load_library
+
+
+class UniformLevel:
+    """Conserved state of an n^3 periodic level resident on the GPU."""
+
+    def __init__(self, u0, boxlen=1.0, smallc=1e-10, device="cuda", stream=None):
+        import torch
+        n = u0.shape[-1]
+        if tuple(u0.shape) != (5, n, n, n):
+            raise ValueError("u0 must have shape (5, n, n, n)")
+        self.n = n
+        self.dx = boxlen / n
+        self.boxlen = boxlen
+        self.smallc = smallc
+        self.stream = stream
+        t = torch.as_tensor(u0, dtype=torch.float64)
+        self.u = t.to(device).contiguous()
+        self.unew = torch.empty_like(self.u)
+        # dt slots: [0] current step, [1] fused reduction for the next step
+        self.dt = torch.zeros(2, dtype=torch.float64, device=device)
+        self.lib = load_library()
+        self._dt_ready = False
+        self.steps = 0
+        self.t = 0.0
+
+    def _ptr(self, t, off=0):
+        return ctypes.c_void_p(t.data_ptr() + 8 * off)
+
+    def courant(self, dt_start=None):
+        """courant_fine on the current state; returns dt (host float)."""
+        start = self.boxlen / self.smallc if dt_start is None else dt_start
+        s = _stream_handle(self.stream)
+        _check(self.lib.ramses_uniform_courant(self._ptr(self.u), self.n, self.dx, start,
+                                               self._ptr(self.dt, 0), s), "ramses_uniform_courant")
+        self._dt_ready = True
+        return self.dt[0]
+
+    def step(self, fused=True):
+        """One level step: dt = courant(u); u <- godunov(u, dt).  Returns the
+        device tensor holding the dt used (no host synchronisation)."""
+        s = _stream_handle(self.stream)
+        if not self._dt_ready:
+            self.courant()
+        d_dt = self._ptr(self.dt, 0)
+        d_next = self._ptr(self.dt, 1)
+        if fused:
+            # newdt_fine restarts every step from boxlen/smallc (pm/newdt_fine.f90:51);
+            # the C*dx/smallc cap of every cmpdt batch is applied at init
+            import torch  # noqa: F401
+            start = min(self.boxlen / self.smallc, self._courant_cap())
+            _check(self.lib.ramses_uniform_dt_init(d_next, start, s), "ramses_uniform_dt_init")
+        _check(self.lib.ramses_uniform_godunov(self._ptr(self.u), self._ptr(self.unew), self.n, self.dx,
+                                               d_dt, d_next if fused else None, s), "ramses_uniform_godunov")
+        used = self.dt[0].clone()
+        self.u, self.unew = self.unew, self.u
+        if fused:
+            self.dt[0:1].copy_(self.dt[1:2])
+        else:
+            self._dt_ready = False
+        self.steps += 1
+        return used
+
+    def _courant_cap(self):
+        from . import _config_cache
+        return _config_cache["courant_factor"] * self.dx / _config_cache["smallc"]

@@ -1,0 +1,109 @@
+"""GPU parity of the batched unsplit kernel through the C ABI against the
+reference's golden vectors and the CPU oracle (bitwise in FP64)."""
+import numpy as np
+import pytest
+
+import oracle
+import ramses_amd as ra
+from helpers import NV, defined_face_mask, random_stencils
+
+pytestmark = pytest.mark.gpu
+
+
+def _golden_inputs(meta, d, ci):
+    uin = np.zeros(ra.stencil_shape(NV))
+    uin[..., :2] = d[f"c{ci}_uin"]
+    g = None
+    if meta[ci]["grav"]:
+        g = np.zeros(ra.grav_shape(NV))
+        g[..., :2] = d[f"c{ci}_grav"]
+    return uin, g
+
+
+def test_unsplit_golden_host_abi(gpu, golden_unsplit):
+    meta, d = golden_unsplit
+    for ci, c in enumerate(meta):
+        ra.configure(gamma=c["gamma"], riemann=c["riemann"], slope_type=c["slope_type"])
+        uin, g = _golden_inputs(meta, d, ci)
+        flux = np.full(ra.flux_shape(NV), np.nan)
+        tmp = np.full(ra.tmp_shape(NV), np.nan)
+        ra.unsplit(uin, g, flux, tmp, c["dx"], c["dx"], c["dx"], c["dt"], 2)
+        assert np.array_equal(flux[..., :2], d[f"c{ci}_flux"], equal_nan=True), c
+        assert np.array_equal(tmp[..., :2], d[f"c{ci}_tmp"], equal_nan=True), c
+        assert np.all(np.isnan(flux[..., 2:])) and np.all(np.isnan(tmp[..., 2:]))
+
+
+def test_unsplit_golden_fortran_symbol(gpu, golden_unsplit):
+    meta, d = golden_unsplit
+    pin = np.zeros((6, 6, 6, NV))
+    for ci, c in enumerate(meta):
+        if c["kind"] != "random":
+            continue
+        ra.configure(gamma=c["gamma"], riemann=c["riemann"], slope_type=c["slope_type"])
+        uin, g = _golden_inputs(meta, d, ci)
+        gz = np.zeros(ra.grav_shape(NV)) if g is None else g
+        flux = np.full(ra.flux_shape(NV), np.nan)
+        tmp = np.full(ra.tmp_shape(NV), np.nan)
+        ra.unsplit_fortran(uin, gz, pin, flux, tmp, c["dx"], c["dx"], c["dx"], c["dt"], 2)
+        assert np.array_equal(flux[..., :2], d[f"c{ci}_flux"], equal_nan=True), c
+
+
+@pytest.mark.parametrize("riemann", ["llf", "hllc", "hll"])
+@pytest.mark.parametrize("slope", [0, 1, 2, 3, 7, 8])
+def test_unsplit_device_batched_vs_oracle(gpu, oracle_lib, riemann, slope):
+    import torch
+    rng = np.random.default_rng(1000 + 10 * slope + len(riemann))
+    G = 40
+    uin = random_stencils(rng, G)
+    grav = rng.normal(0, 1, (G,) + ra.grav_shape(NV)) if slope in (1, 2) else None
+    dx, dt, last = 1 / 128, 2e-4, 7
+    ra.configure(gamma=5 / 3, riemann=riemann, slope_type=slope)
+    oracle_lib.set_params(gamma=5 / 3, riemann=riemann, slope_type=slope)
+    f_ref = np.full((G,) + ra.flux_shape(NV), np.nan)
+    t_ref = np.full((G,) + ra.tmp_shape(NV), np.nan)
+    oracle_lib.unsplit_batched(uin, grav, f_ref, t_ref, dx, dt, ngrid_last=last)
+    d_u = torch.from_numpy(uin).cuda()
+    d_g = None if grav is None else torch.from_numpy(grav).cuda()
+    d_f = torch.full(f_ref.shape, float("nan"), dtype=torch.float64, device="cuda")
+    d_t = torch.full(t_ref.shape, float("nan"), dtype=torch.float64, device="cuda")
+    ra.unsplit_device(d_u, d_g, d_f, d_t, dx, dt, ngrid_last=last)
+    torch.cuda.synchronize()
+    f, t = d_f.cpu().numpy(), d_t.cpu().numpy()
+    assert np.array_equal(f, f_ref, equal_nan=True)
+    assert np.array_equal(t, t_ref, equal_nan=True)
+
+
+def test_unsplit_fp32_mixed_error(gpu, oracle_lib):
+    """FP32-mixed fluxes against the FP64 oracle: relative L1 per variable."""
+    import torch
+    rng = np.random.default_rng(55)
+    G = 64
+    uin = random_stencils(rng, G)
+    dx, dt = 1 / 128, 2e-4
+    m = defined_face_mask(NV)
+    for riemann in ("hllc", "llf"):
+        ra.configure(gamma=5 / 3, riemann=riemann, slope_type=1, precision="fp32")
+        oracle_lib.set_params(gamma=5 / 3, riemann=riemann, slope_type=1)
+        f_ref = np.zeros((G,) + ra.flux_shape(NV))
+        t_ref = np.zeros((G,) + ra.tmp_shape(NV))
+        oracle_lib.unsplit_batched(uin, None, f_ref, t_ref, dx, dt)
+        d_f = torch.zeros(f_ref.shape, dtype=torch.float64, device="cuda")
+        d_t = torch.zeros(t_ref.shape, dtype=torch.float64, device="cuda")
+        ra.unsplit_device(torch.from_numpy(uin).cuda(), None, d_f, d_t, dx, dt)
+        torch.cuda.synchronize()
+        f = d_f.cpu().numpy()
+        for v in range(5):
+            a, b = f[:, :, v][:, m[:, v]], f_ref[:, :, v][:, m[:, v]]
+            rel_l1 = np.abs(a - b).sum() / np.abs(b).sum()
+            assert rel_l1 < 1e-5, (riemann, v, rel_l1)
+    ra.configure()
+
+
+def test_cmpdt_golden(gpu, golden_cmpdt):
+    meta, d = golden_cmpdt
+    for ci, c in enumerate(meta):
+        ra.configure(gamma=c["gamma"], courant_factor=c["courant_factor"])
+        uu = d[f"c{ci}_uu_in"].copy()
+        dt = ra.cmpdt(uu, d[f"c{ci}_gg"].copy(), c["dx"], c["ncell"])
+        assert dt == float(d[f"c{ci}_dt"]), c
+    ra.configure()

@@ -1,0 +1,192 @@
+"""CPU tests of the oracle: pinned against the reference's own golden vectors,
+against the compiled reference where it is built, and against what the
+mathematics fixes independently of both (closed forms, invariants, symmetry)."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import NV, conserved, defined_face_mask, random_stencils
+
+
+def _unsplit_case(O, meta, d, ci, nv=NV):
+    c = meta[ci]
+    O.set_params(gamma=c["gamma"], riemann=c["riemann"], slope_type=c["slope_type"])
+    uin = np.zeros(oracle.stencil_shape(nv))
+    uin[..., :2] = d[f"c{ci}_uin"]
+    g = None
+    if c["grav"]:
+        g = np.zeros(oracle.grav_shape(nv))
+        g[..., :2] = d[f"c{ci}_grav"]
+    flux = np.full(oracle.flux_shape(nv), np.nan)
+    tmp = np.full(oracle.tmp_shape(nv), np.nan)
+    O.unsplit(uin, g, flux, tmp, c["dx"], c["dt"], 2)
+    return flux, tmp
+
+
+def test_oracle_unsplit_golden(oracle_lib, golden_unsplit):
+    meta, d = golden_unsplit
+    for ci in range(len(meta)):
+        flux, tmp = _unsplit_case(oracle_lib, meta, d, ci)
+        # bitwise, including the slots the reference never writes (NaN)
+        assert np.array_equal(flux[..., :2], d[f"c{ci}_flux"], equal_nan=True), meta[ci]
+        assert np.array_equal(tmp[..., :2], d[f"c{ci}_tmp"], equal_nan=True), meta[ci]
+        assert np.all(np.isnan(flux[..., 2:])), "lanes >= ngrid must stay untouched"
+
+
+def test_golden_defined_slots(golden_unsplit):
+    """The fixtures define exactly the 36 faces per oct of hydro/umuscl.f90:97-163."""
+    meta, d = golden_unsplit
+    m = defined_face_mask(2)
+    for ci in range(len(meta)):
+        f = d[f"c{ci}_flux"]
+        assert np.all(np.isfinite(f[m])) and np.all(np.isnan(f[~m]))
+
+
+def test_oracle_cmpdt_golden(oracle_lib, golden_cmpdt):
+    meta, d = golden_cmpdt
+    for ci, c in enumerate(meta):
+        oracle_lib.set_params(gamma=c["gamma"], courant_factor=c["courant_factor"])
+        uu = d[f"c{ci}_uu_in"].copy()
+        dt = oracle_lib.cmpdt(uu, d[f"c{ci}_gg"].copy(), c["dx"], c["ncell"])
+        assert dt == float(d[f"c{ci}_dt"])
+        assert np.array_equal(uu[:, :c["ncell"]], d[f"c{ci}_uu_out"][:, :c["ncell"]])
+
+
+def test_oracle_sedov16_golden(oracle_lib, golden_sedov16):
+    g = golden_sedov16
+    oracle_lib.set_params(gamma=1.4, riemann="hllc", slope_type=1, courant_factor=0.8)
+    u = g["u0"].copy()
+    for s in range(1, 11):
+        u, dt = oracle_lib.uniform_step(u, 1.0 / 16)
+        assert dt == g["dt"][s - 1]
+        if f"u_step{s}" in g:
+            assert np.array_equal(u, g[f"u_step{s}"])
+
+
+@pytest.mark.skipif(not oracle.reference_available(), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("riemann", ["llf", "hllc", "hll"])
+@pytest.mark.parametrize("slope", [0, 1, 2, 3, 7, 8])
+def test_oracle_vs_compiled_reference(oracle_lib, riemann, slope):
+    R = oracle.Reference()
+    rng = np.random.default_rng(hash((riemann, slope)) % 2**32)
+    for grav in (False, True):
+        for P in (oracle_lib, R):
+            P.set_params(gamma=5 / 3, riemann=riemann, slope_type=slope)
+        uin = np.ascontiguousarray(random_stencils(rng, 1)[0])
+        g = rng.normal(0, 1, oracle.grav_shape(NV)) if grav else None
+        out = []
+        for P in (oracle_lib, R):
+            f = np.full(oracle.flux_shape(NV), np.nan)
+            t = np.full(oracle.tmp_shape(NV), np.nan)
+            P.unsplit(uin, g, f, t, 1 / 128, 1e-3, 29)
+            out.append((f, t))
+        assert np.array_equal(out[0][0], out[1][0], equal_nan=True)
+        assert np.array_equal(out[0][1], out[1][1], equal_nan=True)
+
+
+# ---------------------------------------------------------------------------
+# Independent pins: what the mathematics fixes
+
+def euler_flux_x(rho, u, v, w, p, gamma):
+    E = p / (gamma - 1) + 0.5 * rho * (u * u + v * v + w * w)
+    return np.array([rho * u, rho * u * u + p, rho * u * v, rho * u * w, (E + p) * u])
+
+
+@pytest.mark.parametrize("riemann", ["llf", "hllc", "hll"])
+def test_uniform_state_gives_physical_flux(oracle_lib, riemann):
+    """Equal states on both sides of every face: the Godunov flux is the
+    physical Euler flux F(U), and the divergence of the update vanishes."""
+    gamma = 1.4
+    oracle_lib.set_params(gamma=gamma, riemann=riemann, slope_type=1)
+    rho, vel, p = 1.3, np.array([0.2, -0.1, 0.05]), 0.7
+    ones = np.ones((6, 6, 6, NV))
+    uin = conserved(ones * rho, np.zeros((3, 6, 6, 6, NV)) + vel[:, None, None, None, None], ones * p, gamma)
+    flux = np.zeros(oracle.flux_shape(NV))
+    tmp = np.zeros(oracle.tmp_shape(NV))
+    dx, dt = 1 / 64, 1e-3
+    oracle_lib.unsplit(np.ascontiguousarray(uin), None, flux, tmp, dx, dt, NV)
+    F = euler_flux_x(rho, *vel, p, gamma) * dt / dx
+    fx = flux[0, :, 0, 0, :, :]  # [v][i][l] at j=k=1
+    np.testing.assert_allclose(fx, np.broadcast_to(F[:, None, None], fx.shape), rtol=1e-13, atol=1e-18)
+    # flux(i) == flux(i+1) exactly -> zero update
+    assert np.array_equal(fx[:, 0], fx[:, 1]) and np.array_equal(fx[:, 1], fx[:, 2])
+
+
+def test_hllc_stationary_contact(oracle_lib):
+    """u=0, p_L=p_R, rho_L != rho_R: zero mass flux, momentum flux = p."""
+    gamma = 1.4
+    oracle_lib.set_params(gamma=gamma, riemann="hllc", slope_type=0)
+    xs = np.broadcast_to(np.arange(-1, 5)[None, None, :, None], (6, 6, 6, NV))
+    rho = np.where(xs < 2, 1.0, 0.125)
+    uin = conserved(rho, np.zeros((3, 6, 6, 6, NV)), np.ones((6, 6, 6, NV)), gamma)
+    flux = np.zeros(oracle.flux_shape(NV))
+    tmp = np.zeros(oracle.tmp_shape(NV))
+    dx, dt = 1 / 64, 1e-3
+    oracle_lib.unsplit(np.ascontiguousarray(uin), None, flux, tmp, dx, dt, NV)
+    face = flux[0, :, 0, 0, 1, :]  # x face i=2 (between cells 1 and 2: the contact)
+    assert np.all(face[0] == 0.0)
+    np.testing.assert_allclose(face[1], 1.0 * dt / dx, rtol=1e-15)
+
+
+@pytest.mark.parametrize("riemann", ["hllc", "hll"])
+def test_supersonic_upwind(oracle_lib, riemann):
+    """All waves moving right (SL > 0): the flux is the left state's F(U)."""
+    gamma = 1.4
+    oracle_lib.set_params(gamma=gamma, riemann=riemann, slope_type=0)
+    xs = np.broadcast_to(np.arange(-1, 5)[None, None, :, None], (6, 6, 6, NV))
+    rho = np.where(xs < 2, 1.0, 1.1)
+    vel = np.zeros((3, 6, 6, 6, NV))
+    vel[0] = 8.0
+    uin = conserved(rho, vel, np.ones((6, 6, 6, NV)), gamma)
+    flux = np.zeros(oracle.flux_shape(NV))
+    tmp = np.zeros(oracle.tmp_shape(NV))
+    dx, dt = 1 / 64, 1e-3
+    oracle_lib.unsplit(np.ascontiguousarray(uin), None, flux, tmp, dx, dt, NV)
+    F = euler_flux_x(1.0, 8.0, 0, 0, 1.0, gamma) * dt / dx
+    np.testing.assert_allclose(flux[0, :, 0, 0, 1, 0], F, rtol=1e-13)
+
+
+def test_llf_mirror_symmetry(oracle_lib):
+    """Reflecting x -> -x (and u -> -u) negates the x mass flux on the
+    mirrored face; LLF is sign-symmetric so this holds bitwise."""
+    oracle_lib.set_params(gamma=5 / 3, riemann="llf", slope_type=1)
+    rng = np.random.default_rng(7)
+    uin = np.ascontiguousarray(random_stencils(rng, 1)[0])
+    mir = uin[:, :, :, ::-1, :].copy()        # i -> 3 - i  (cells -1..4 mirror onto 4..-1)
+    mir[1] = -mir[1]
+    f1 = np.zeros(oracle.flux_shape(NV)); t1 = np.zeros(oracle.tmp_shape(NV))
+    f2 = np.zeros(oracle.flux_shape(NV)); t2 = np.zeros(oracle.tmp_shape(NV))
+    oracle_lib.unsplit(uin, None, f1, t1, 1 / 64, 1e-3, NV)
+    oracle_lib.unsplit(np.ascontiguousarray(mir), None, f2, t2, 1 / 64, 1e-3, NV)
+    # face i (between cells i-1, i) mirrors onto face 4-i
+    for i in range(3):
+        assert np.array_equal(f1[0, 0, 0:2, 0:2, i], -f2[0, 0, 0:2, 0:2, 2 - i])
+
+
+def test_periodic_conservation(oracle_lib):
+    """The flux-difference update conserves mass and energy on a periodic box."""
+    from helpers import random_grid
+    oracle_lib.set_params(gamma=5 / 3, riemann="hllc", slope_type=1, courant_factor=0.5)
+    rng = np.random.default_rng(3)
+    u = random_grid(rng, 16)
+    unew, dt = oracle_lib.uniform_step(u, 1 / 16)
+    for v in (0, 1, 2, 3, 4):
+        s0, s1 = u[v].sum(), unew[v].sum()
+        assert abs(s1 - s0) <= 1e-12 * np.abs(u[v]).sum(), v
+
+
+def test_cmpdt_closed_form(oracle_lib):
+    """With g=0 the gravity ratio is floored at 1e-4, so
+    dt = min(C dx/smallc, min_cell dx/w (sqrt(1+2C 1e-4)-1)/1e-4)."""
+    gamma, C, dx = 1.4, 0.8, 1 / 64
+    oracle_lib.set_params(gamma=gamma, courant_factor=C)
+    rng = np.random.default_rng(9)
+    rho = rng.lognormal(0, 1, NV)
+    v = rng.normal(0, 1, (3, NV))
+    p = rng.lognormal(0, 1, NV)
+    uu = conserved(rho, v, p, gamma)
+    dt = oracle_lib.cmpdt(uu.copy(), None, dx, NV)
+    c = np.sqrt(gamma * p / rho)
+    w = 3 * c + np.abs(v).sum(0)
+    expect = (dx / w * (np.sqrt(1 + 2 * C * 1e-4) - 1) / 1e-4).min()
+    np.testing.assert_allclose(dt, expect, rtol=1e-13)
