@@ -1,0 +1,288 @@
+"""Benchmark: the RAMSES `unsplit` hydro kernel on synthetic gathered oct
+batches, one MI355X.
+
+Workload (one "step" = one batched unsplit call over every oct):
+  2^20 octs (8.4M cells) as 6x6x6 stencils in nvector=32 batches, NDIM=3,
+  NVAR=5; rho ~ logN(0,1), velocities ~ N(0,1), P ~ logN(-0.5,0.5),
+  gamma=5/3, dx=1/128, dt = min cmpdt at CFL 0.5 over the octs' centre
+  cells; riemann=hllc, slope_type=1 (minmod).  Inputs are generated on the
+  device (torch Philox, seeded), resident in HBM before the timed region.
+
+Reported: cell-updates/s (8 per oct) of the FP64 path (value) and of the
+FP32-mixed path, per-call latency p50/p99, the kernel's roofline against
+MI355X HBM (8 TB/s) with algorithmic bytes of 10,656 B per oct (uin
+216*5*8 read, 36 faces of flux 5*8 and tmp 2*8 written), and the
+reference's own CPU unsplit (oracle/_ref, the Fortran routine compiled from
+the reference sources) timed on a bounded sample on one host core.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+NOCT = 1 << 20
+NV = 32
+GAMMA = 5.0 / 3.0
+DX = 1.0 / 128
+CFL = 0.5
+BYTES_PER_OCT = 216 * 5 * 8 + 36 * 5 * 8 + 36 * 2 * 8  # 10,656
+HBM_PEAK = 8.0e12
+
+
+def make_inputs(torch, noct, seed=1):
+    """uin[g][5][6][6][6][32] on the device and the CFL-0.5 dt."""
+    G = noct // NV
+    dev = "cuda"
+    uin = torch.empty((G, 5, 6, 6, 6, NV), dtype=torch.float64, device=dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    dtmin = float("inf")
+    chunk = 2048
+    for g0 in range(0, G, chunk):
+        g1 = min(G, g0 + chunk)
+        shp = (g1 - g0, 6, 6, 6, NV)
+        rho = torch.exp(torch.randn(shp, generator=gen, device=dev, dtype=torch.float64))
+        v = torch.randn((3,) + shp, generator=gen, device=dev, dtype=torch.float64)
+        p = torch.exp(-0.5 + 0.5 * torch.randn(shp, generator=gen, device=dev, dtype=torch.float64))
+        u = uin[g0:g1]
+        u[:, 0] = rho
+        u[:, 1:4] = (rho.unsqueeze(0) * v).permute(1, 0, 2, 3, 4, 5)
+        u[:, 4] = p / (GAMMA - 1.0) + 0.5 * rho * (v * v).sum(0)
+        # cmpdt (hydro/godunov_utils.f90:5-120, g=0) over the centre cells
+        c = torch.sqrt(GAMMA * p[:, 1:3, 1:3, 1:3] / rho[:, 1:3, 1:3, 1:3])
+        w = 3.0 * c + v[:, :, 1:3, 1:3, 1:3].abs().sum(0)
+        r = 1e-4
+        dtc = DX / w * ((1.0 + 2.0 * CFL * r) ** 0.5 - 1.0) / r
+        dtmin = min(dtmin, float(dtc.min()))
+        del rho, v, p, c, w, dtc
+    return uin, dtmin
+
+
+def time_variant(torch, ra, uin, flux, tmp, dt, riemann, precision, steps, warmup, tune=True):
+    from ramses_amd import autotune
+    ra.configure(gamma=GAMMA, riemann=riemann, slope_type=1, courant_factor=CFL, precision=precision)
+    stream = torch.cuda.current_stream()
+    tuned = None
+    if tune:  # untimed: pick the workgroup size on these inputs
+        tuned = autotune.tune_unsplit(lambda: ra.unsplit_device(uin, None, flux, tmp, DX, dt, stream=stream),
+                                      key=("unsplit", uin.shape[0], precision, riemann))
+    for _ in range(warmup):
+        ra.unsplit_device(uin, None, flux, tmp, DX, dt, stream=stream)
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        starts[k].record(stream)
+        ra.unsplit_device(uin, None, flux, tmp, DX, dt, stream=stream)
+        ends[k].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = sorted(s.elapsed_time(e) for s, e in zip(starts, ends))
+    avg = sum(ms) / len(ms)
+    pct = lambda q: ms[min(len(ms) - 1, int(round(q * (len(ms) - 1))))]  # noqa: E731
+    return dict(wall_s=wall, kernel_ms_avg=avg, p50_ms=pct(0.5), p99_ms=pct(0.99),
+                ms_per_step=1e3 * wall / steps, block=tuned[0] if tuned else None)
+
+
+def cpu_baseline(budget_s=12.0):
+    """The reference's own unsplit (oracle/_ref) on one core, bounded sample."""
+    import numpy as np
+    import oracle
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import random_stencils
+    rng = np.random.default_rng(1)
+    groups = random_stencils(rng, 32)
+    if oracle.reference_available():
+        impl, kind = oracle.Reference(), "reference"
+    else:
+        impl, kind = oracle.Oracle(), "port"
+    impl.set_params(gamma=GAMMA, riemann="hllc", slope_type=1, courant_factor=CFL)
+    flux = np.zeros((3, 5, 3, 3, 3, NV))
+    tmp = np.zeros((3, 2, 3, 3, 3, NV))
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        impl.unsplit(np.ascontiguousarray(groups[n % 32]), None, flux, tmp, DX, 1e-4, NV)
+        n += 1
+        el = time.perf_counter() - t0
+        if el > budget_s and n >= 32:
+            break
+    cells = 8 * NV * n
+    return {"value": cells / el, "unit": "cell-updates/s", "cores": 1, "kind": kind,
+            "sample": f"{n} unsplit calls of {NV} octs ({cells} cell-updates) in {el:.1f} s, hllc minmod, "
+                      f"same input distribution; {'flang -O2 build of hydro/umuscl.f90' if kind == 'reference' else 'C restatement'}"}
+
+
+LEVEL_WORKLOADS = {
+    # 3D Sedov, uniform 64^3 periodic, gamma=1.4, hllc, minmod, CFL 0.8
+    "sedov64": dict(n=64, gamma=1.4, courant=0.8, ic="sedov",
+                    name="3D Sedov blast, uniform 64^3 periodic, gamma=1.4, hllc, minmod, CFL 0.8"),
+    # 512^3 periodic Mach-5 k^-4 velocity field, lognormal rho (sigma 0.5), P=rho/gamma
+    "turb512": dict(n=512, gamma=5.0 / 3.0, courant=0.5, ic="turb", mach=5.0, rho_sigma=0.5, seed=4,
+                    name="512^3 periodic, Mach-5 k^-4 velocity field, rho lognormal(0.5), P=rho/gamma, "
+                         "gamma=5/3, hllc, minmod"),
+    "turb256": dict(n=256, gamma=5.0 / 3.0, courant=0.5, ic="turb", mach=5.0, rho_sigma=0.5, seed=4,
+                    name="256^3 periodic, Mach-5 k^-4 velocity field, rho lognormal(0.5), P=rho/gamma, "
+                         "gamma=5/3, hllc, minmod"),
+}
+
+
+def level_bench(torch, ra, args):
+    """Resident uniform level: K steps (courant fused into the update)."""
+    from ramses_amd import ics
+    w = LEVEL_WORKLOADS[args.workload]
+    n = w["n"]
+    if w["ic"] == "sedov":
+        u0 = ics.sedov(n, gamma=w["gamma"])
+    else:
+        u0 = ics.turbulent(n, w["mach"], gamma=w["gamma"], rho_sigma=w["rho_sigma"], seed=w["seed"])
+    out = {}
+    states = {}
+    for prec in ("fp64", "fp32"):
+        ra.configure(gamma=w["gamma"], riemann="hllc", slope_type=1, courant_factor=w["courant"],
+                     precision=prec)
+        lvl = ra.UniformLevel(u0)
+        from ramses_amd import autotune
+        tuned = autotune.tune_level(lvl.update_only, n, key=("level", n, prec))  # untimed
+        for _ in range(args.warmup):
+            lvl.step()
+        lvl = ra.UniformLevel(u0)  # restart from the IC for the timed run
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            lvl.step(events=ev[k])
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kms = sorted(a.elapsed_time(b) for a, b in ev)
+        kavg = sum(kms) / len(kms)
+        cells = n ** 3
+        ach = 80.0 * cells / (kavg * 1e-3)
+        out[prec] = {"cell_updates_per_s": cells * args.steps / wall, "ms_per_step": 1e3 * wall / args.steps,
+                     "tile_tz": list(tuned[0]),
+                     "kernel_ms_avg": kavg, "kernel_ms_p50": kms[len(kms) // 2], "kernel_ms_max": kms[-1],
+                     "roofline": {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                                  "frac": ach / HBM_PEAK, "traffic": None}}
+        states[prec] = lvl.u.clone()
+        del lvl
+    err = {}
+    names = ["rho", "mx", "my", "mz", "E"]
+    for v in range(5):
+        a, b = states["fp32"][v], states["fp64"][v]
+        err[names[v]] = float((a - b).abs().sum() / b.abs().sum())
+    ra.configure()
+    head = out["fp64"]
+    return {
+        "metric": "cell-updates/s FP64 vs FP32-mixed, % HBM roofline; p50/p99 solve latency",
+        "value": head["cell_updates_per_s"], "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": w["name"], "cells": n ** 3, "parallelism": "1 GPU"},
+        "roofline": head["roofline"], "variants": out, "fp32_rel_l1_vs_fp64": err,
+        "fp32_over_fp64": out["fp32"]["cell_updates_per_s"] / head["cell_updates_per_s"],
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--octs", type=int, default=NOCT)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--variants", default="fp64-hllc,fp32-hllc,fp64-llf,fp32-llf")
+    ap.add_argument("--workload", default="unsplit", choices=["unsplit", "sedov64", "turb512", "turb256"],
+                    help="unsplit (default, the headline) or a resident-level workload")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="run only a device copy of this many MiB (for HBM counter scaling)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only; multi-GPU sharding of the path is "
+              "design-only (see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import ramses_amd as ra
+    ra.load_library()
+    if args.calibrate_copy:
+        n = args.calibrate_copy * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": n * 8}))
+        return
+
+    if args.workload != "unsplit":
+        print(json.dumps(level_bench(torch, ra, args)), flush=True)
+        return
+
+    noct = args.octs
+    uin, dt = make_inputs(torch, noct)
+    G = noct // NV
+    flux = torch.zeros((G, 3, 5, 3, 3, 3, NV), dtype=torch.float64, device="cuda")
+    tmp = torch.zeros((G, 3, 2, 3, 3, 3, NV), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+
+    results = {}
+    for v in args.variants.split(","):
+        prec, riemann = v.split("-")
+        results[v] = time_variant(torch, ra, uin, flux, tmp, dt, riemann, prec, args.steps, args.warmup)
+    ra.configure()
+
+    cells = 8 * noct
+    head = results.get("fp64-hllc") or next(iter(results.values()))
+    alg_bytes = BYTES_PER_OCT * noct
+
+    def roof(r):
+        ach = alg_bytes / (r["kernel_ms_avg"] * 1e-3)
+        return {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": ach / HBM_PEAK, "traffic": None}
+
+    variants = {}
+    for k, r in results.items():
+        variants[k] = {"cell_updates_per_s": cells / (r["kernel_ms_avg"] * 1e-3),
+                       "kernel_ms_avg": r["kernel_ms_avg"], "p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"],
+                       "roofline_frac": roof(r)["frac"], "workgroup": r["block"]}
+    line = {
+        "metric": "cell-updates/s FP64 vs FP32-mixed, % HBM roofline; p50/p99 solve latency",
+        "value": cells / (head["ms_per_step"] * 1e-3),
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (device Philox, seed 1)",
+        "config": {"workload": "unsplit oct batches: 2^20 octs (8.4M cells) as 6x6x6 stencils, nvector=32, "
+                               "NVAR=5, hllc, minmod, gamma=5/3, dx=1/128, dt at CFL 0.5",
+                   "octs": noct, "nvector": NV, "riemann": "hllc", "slope_type": 1, "parallelism": "1 GPU"},
+        "latency_ms": {"p50": head["p50_ms"], "p99": head["p99_ms"]},
+        "roofline": roof(head),
+        "variants": variants,
+    }
+    if "fp32-hllc" in variants:
+        line["fp32_mixed_value"] = variants["fp32-hllc"]["cell_updates_per_s"]
+        line["fp32_over_fp64"] = variants["fp32-hllc"]["cell_updates_per_s"] / variants["fp64-hllc"]["cell_updates_per_s"] \
+            if "fp64-hllc" in variants else None
+    if not args.no_cpu:
+        cb = cpu_baseline()
+        line["cpu_baseline"] = cb
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -116,6 +116,23 @@ int ramses_uniform_godunov(const double *u, double *unew, int n, double dx, cons
  * newdt_fine (pm/newdt_fine.f90:51). */
 int ramses_uniform_dt_init(double *d_dt, double value, void *stream);
 
+/* ---------------------------------------------------------------------
+ * Launch-shape tuning.  The reference's only performance knob is the
+ * compile-time vector length nvector (amr/amr_parameters.f90:49-53, set by
+ * bin/Makefile:11); on the GPU the equivalent choices are the workgroup and
+ * tile shapes below, picked per device and workload by the autotuner
+ * (ramses_amd/autotune.py).  value 0 restores the built-in default.
+ * --------------------------------------------------------------------- */
+#define RAMSES_TUNE_UNSPLIT_BLOCK 1 /* threads per unsplit workgroup: multiple of 64, <= 768 */
+#define RAMSES_TUNE_LEVEL_TILE 2    /* level tile: 1 = 32x8, 2 = 16x8, 3 = 8x8 columns */
+#define RAMSES_TUNE_LEVEL_TZ 3      /* z planes per level workgroup: a divisor of n, >= 4 */
+
+/* Set one tuning knob (amr/amr_parameters.f90:49-53 is the reference's
+ * counterpart); returns 0, or < 0 for an unknown knob or invalid value.
+ * ramses_tune_get returns the current value (0 = default). */
+int ramses_tune(int knob, int value);
+int ramses_tune_get(int knob);
+
 #ifdef __cplusplus
 }
 #endif

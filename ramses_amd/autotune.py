@@ -1,0 +1,85 @@
+"""Autotuner for the launch shapes of the library's kernels.
+
+The reference tunes one thing, the compile-time vector length nvector
+(amr/amr_parameters.f90:49-53).  Here the equivalent choices are the unsplit
+workgroup size and the resident-level tile / z-chunk, and they depend on the
+device and on the problem size, so they are measured: every candidate is
+timed with HIP events on the caller's stream (median of a few launches) on
+the real inputs, the fastest is installed with ``ramses_tune`` and the
+choice is remembered per (kernel, size, precision, solver) for the process.
+"""
+import ctypes
+
+from . import _check, load_library
+
+UNSPLIT_BLOCK, LEVEL_TILE, LEVEL_TZ = 1, 2, 3
+_cache = {}
+
+
+def _set(knob, value):
+    lib = load_library()
+    lib.ramses_tune.argtypes = [ctypes.c_int, ctypes.c_int]
+    _check(lib.ramses_tune(knob, value), "ramses_tune")
+
+
+def _time(fn, reps=5):
+    import torch
+    ts = []
+    for _ in range(reps + 1):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        fn()
+        b.record()
+        torch.cuda.synchronize()
+        ts.append(a.elapsed_time(b))
+    ts = sorted(ts[1:])
+    return ts[len(ts) // 2]
+
+
+def tune_unsplit(run, key, blocks=(128, 192, 256, 320, 384, 512, 768), verbose=False):
+    """run(): one unsplit launch on the real inputs.  Returns (block, ms)."""
+    if key in _cache:
+        _set(UNSPLIT_BLOCK, _cache[key][0])
+        return _cache[key]
+    best = None
+    for b in blocks:
+        _set(UNSPLIT_BLOCK, b)
+        ms = _time(run)
+        if verbose:
+            print(f"autotune unsplit block={b}: {ms:.3f} ms")
+        if best is None or ms < best[1]:
+            best = (b, ms)
+    _set(UNSPLIT_BLOCK, best[0])
+    _cache[key] = best
+    return best
+
+
+def tune_level(run, n, key, verbose=False):
+    """run(): one level update on the real state.  Returns ((tile, tz), ms)."""
+    if key in _cache:
+        (tile, tz), _ = _cache[key]
+        _set(LEVEL_TILE, tile)
+        _set(LEVEL_TZ, tz)
+        return _cache[key]
+    tiles = [t for t, w in ((1, 32), (2, 16), (3, 8)) if n % w == 0]
+    tzs = [0] + [z for z in (8, 16, 32, 64, 128) if n % z == 0 and z < n]
+    best = None
+    for t in tiles:
+        for z in tzs:
+            _set(LEVEL_TILE, t)
+            _set(LEVEL_TZ, z)
+            ms = _time(run, reps=3)
+            if verbose:
+                print(f"autotune level tile={t} tz={z}: {ms:.3f} ms")
+            if best is None or ms < best[1]:
+                best = ((t, z), ms)
+    _set(LEVEL_TILE, best[0][0])
+    _set(LEVEL_TZ, best[0][1])
+    _cache[key] = best
+    return best
+
+
+def reset():
+    for k in (UNSPLIT_BLOCK, LEVEL_TILE, LEVEL_TZ):
+        _set(k, 0)

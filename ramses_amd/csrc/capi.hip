@@ -20,6 +20,10 @@ hipError_t launch_set_double(double* p, double v, hipStream_t s);
 hipError_t launch_courant(const double* u, int n, double dx, const HydroParams& hp, double* d_dt, hipStream_t s);
 hipError_t launch_uniform_godunov(const double* u, double* unew, int n, double dx, const double* d_dt,
                                   double* d_dt_next, const HydroParams& hp, bool fp32, hipStream_t s);
+Tuning& tuning() {
+  static Tuning t{0, 0, 0};
+  return t;
+}
 }  // namespace ramses
 
 #include "unsplit_args.hpp"
@@ -202,6 +206,36 @@ int ramses_cmpdt(const double* uu, const double* gg, double dx, int ncell, doubl
   if ((e = hipMemcpyAsync(dt, d_dt, 8, hipMemcpyDeviceToHost, s)) != hipSuccess) return hip_fail(e, "D2H dt");
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "cmpdt sync");
   return 0;
+}
+
+int ramses_tune(int knob, int value) {
+  ramses::Tuning& t = ramses::tuning();
+  switch (knob) {
+    case RAMSES_TUNE_UNSPLIT_BLOCK:
+      if (value < 0 || value > 768 || value % 64) return fail(-2, "unsplit block must be a multiple of 64 in [64, 768]");
+      t.unsplit_block = value;
+      return 0;
+    case RAMSES_TUNE_LEVEL_TILE:
+      if (value < 0 || value > 3) return fail(-2, "level tile must be 0..3");
+      t.level_tile = value;
+      return 0;
+    case RAMSES_TUNE_LEVEL_TZ:
+      if (value != 0 && value < 4) return fail(-2, "level tz must be 0 or >= 4");
+      t.level_tz = value;
+      return 0;
+    default:
+      return fail(-1, "unknown tuning knob");
+  }
+}
+
+int ramses_tune_get(int knob) {
+  const ramses::Tuning& t = ramses::tuning();
+  switch (knob) {
+    case RAMSES_TUNE_UNSPLIT_BLOCK: return t.unsplit_block;
+    case RAMSES_TUNE_LEVEL_TILE: return t.level_tile;
+    case RAMSES_TUNE_LEVEL_TZ: return t.level_tz;
+    default: return fail(-1, "unknown tuning knob");
+  }
 }
 
 int ramses_uniform_dt_init(double* d_dt, double value, void* stream) {

@@ -28,6 +28,14 @@ struct HydroParams {
   int riemann;     // Riemann enum
 };
 
+// Launch-shape knobs set through ramses_tune (0 = built-in default).
+struct Tuning {
+  int unsplit_block;
+  int level_tile;
+  int level_tz;
+};
+Tuning& tuning();
+
 // Fortran MAX/MIN of two reals as the reference build evaluates them.
 template <typename R>
 __device__ __forceinline__ R fmax_f(R a, R b) { return (a > b) ? a : b; }

@@ -41,8 +41,8 @@ struct UniformArgs {
 __device__ __forceinline__ int wrapi(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
 template <typename R, int TX, int TY, int RS>
-__global__ __launch_bounds__(((TX + 2) * (TY + 2) + 63) / 64 * 64) void uniform_godunov_kernel(UniformArgs a,
-                                                                                               int TZ) {
+__global__ __launch_bounds__(((TX + 2) * (TY + 2) + 63) / 64 * 64)
+__attribute__((amdgpu_waves_per_eu(3, 8))) void uniform_godunov_kernel(UniformArgs a, int TZ) {
   constexpr int EX = TX + 2, EY = TY + 2;      // extended tile (threads)
   constexpr int QX = TX + 4, QY = TY + 4;      // primitive plane with 2-cell halo
   constexpr int NQ = QX * QY;
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(((TX + 2) * (TY + 2) + 63) / 64 * 64) void uniform_
   // values carried from the previous plane
   Face<R> fz_left{};          // rotated qm_z of plane p-1
   double fz_prev[5] = {0, 0, 0, 0, 0};  // flux through the z face below plane p-1 (scaled)
-  double dfx_prev[5] = {0, 0, 0, 0, 0}, dfy_prev[5] = {0, 0, 0, 0, 0};
+  double part_prev[5] = {0, 0, 0, 0, 0};  // (u + dFx) + dFy of plane p-1
   double dtmin = 1e300;
 
   // prologue: planes z0-2 and z0-1
@@ -131,7 +131,7 @@ __global__ __launch_bounds__(((TX + 2) * (TY + 2) + 63) / 64 * 64) void uniform_
       trace_dir(qcur, dqy, src, dtdx, sc.smallr, qp_y, qm_y);
       trace_dir(qcur, dqz, src, dtdx, sc.smallr, qp_z, qm_z);
     }
-    double dfx[5] = {0, 0, 0, 0, 0}, dfy[5] = {0, 0, 0, 0, 0};
+    double part[5] = {0, 0, 0, 0, 0};
     if (faces_xy) {
       if (active) {
         xch[0][t] = qm_x.r; xch[1][t] = qm_x.u; xch[2][t] = qm_x.v; xch[3][t] = qm_x.w; xch[4][t] = qm_x.p;
@@ -166,8 +166,10 @@ __global__ __launch_bounds__(((TX + 2) * (TY + 2) + 63) / 64 * 64) void uniform_
       }
       __syncthreads();
       if (interior) {
+        // hydro/godunov_fine.f90:770-772, idim=1: unew = uold + (flux(i) - flux(i+1))
+        const long o = (long)wrapi(p, n) * plane + (long)yg * n + xg;
 #pragma unroll
-        for (int v = 0; v < 5; ++v) dfx[v] = fx_sh[v * FSTRIDE + t] - fx_sh[v * FSTRIDE + t + 1];
+        for (int v = 0; v < 5; ++v) part[v] = a.u[o + v * vstride] + (fx_sh[v * FSTRIDE + t] - fx_sh[v * FSTRIDE + t + 1]);
       }
       __syncthreads();
       if (active) {
@@ -177,7 +179,7 @@ __global__ __launch_bounds__(((TX + 2) * (TY + 2) + 63) / 64 * 64) void uniform_
       __syncthreads();
       if (interior) {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) dfy[v] = fx_sh[v * FSTRIDE + t] - fx_sh[v * FSTRIDE + t + EX];
+        for (int v = 0; v < 5; ++v) part[v] = part[v] + (fx_sh[v * FSTRIDE + t] - fx_sh[v * FSTRIDE + t + EX]);  // idim=2
       }
     }
     // z face below plane p (between p-1 and p)
@@ -196,10 +198,7 @@ __global__ __launch_bounds__(((TX + 2) * (TY + 2) + 63) / 64 * 64) void uniform_
       double un[5];
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
-        double s = a.u[o + v * vstride];
-        s = s + dfx_prev[v];
-        s = s + dfy_prev[v];
-        s = s + (fz_prev[v] - fz[v]);
+        const double s = part_prev[v] + (fz_prev[v] - fz[v]);  // idim=3
         un[v] = s;
         a.unew[o + v * vstride] = s;
       }
@@ -210,8 +209,7 @@ __global__ __launch_bounds__(((TX + 2) * (TY + 2) + 63) / 64 * 64) void uniform_
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
       fz_prev[v] = fz[v];
-      dfx_prev[v] = dfx[v];
-      dfy_prev[v] = dfy[v];
+      part_prev[v] = part[v];
     }
     qprev = qcur;
     qcur = qnext;
@@ -308,12 +306,18 @@ hipError_t launch_uniform_godunov(const double* u, double* unew, int n, double d
                                   double* d_dt_next, const HydroParams& hp, bool fp32, hipStream_t s) {
   UniformArgs a{u, unew, d_dt, d_dt_next, n, dx, hp};
   int TX, TY;
-  if (n % 32 == 0) { TX = 32; TY = 8; }
+  const int tile = tuning().level_tile;
+  if (tile == 1 && n % 32 == 0) { TX = 32; TY = 8; }
+  else if (tile == 2 && n % 16 == 0) { TX = 16; TY = 8; }
+  else if (tile == 3) { TX = 8; TY = 8; }
+  else if (n % 32 == 0) { TX = 32; TY = 8; }
   else if (n % 16 == 0) { TX = 16; TY = 8; }
   else { TX = 8; TY = 8; }
   const long tiles = (long)(n / TX) * (n / TY);
   int TZ = n;
   while (TZ > 4 && TZ % 2 == 0 && tiles * (n / TZ) < 2048) TZ /= 2;
+  const int tz = tuning().level_tz;
+  if (tz >= 4 && tz <= n && n % tz == 0) TZ = tz;
   if (fp32) {
     if (TX == 32) return launch_tile<float, 32, 8>(a, TZ, s);
     if (TX == 16) return launch_tile<float, 16, 8>(a, TZ, s);

@@ -182,7 +182,7 @@ __device__ __forceinline__ void row_faces(const RowCtx<R>& c, const SolverConsts
 }
 
 template <typename R, int RS, bool S27>
-__global__ __launch_bounds__(256) void unsplit_rows(UnsplitArgs a) {
+__global__ __launch_bounds__(768) void unsplit_rows(UnsplitArgs a) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int nv = a.nv;
   const long per_group = 12L * nv;
@@ -245,7 +245,11 @@ static void launch_r(const UnsplitArgs& a, long grid, int block, hipStream_t str
 hipError_t launch_unsplit(const UnsplitArgs& a, bool fp32, hipStream_t stream) {
   const long threads = a.ngroups * 12L * a.nv;
   if (threads <= 0) return hipSuccess;
-  const int block = 256;
+  // Any multiple of 64 works with the flat (batch, row, lane) mapping.  384 =
+  // 12 rows x 32 lanes puts exactly one nvector=32 batch in a workgroup, so
+  // re-reads of its stencil cells stay in one XCD's L2; 256 spreads a batch
+  // over two workgroups.  The autotuner picks per device.
+  const int block = tuning().unsplit_block ? tuning().unsplit_block : 256;
   const long grid = (threads + block - 1) / block;
   if (fp32)
     launch_r<float>(a, grid, block, stream);

@@ -32,6 +32,8 @@ class UniformLevel:
         self.stream = stream
         t = torch.as_tensor(u0, dtype=torch.float64)
         self.u = t.to(device).contiguous()
+        if self.u.data_ptr() == getattr(u0, "data_ptr", lambda: None)():
+            self.u = self.u.clone()
         self.unew = torch.empty_like(self.u)
         # dt slots: [0] current step, [1] fused reduction for the next step
         self.dt = torch.zeros(2, dtype=torch.float64, device=device)
@@ -52,9 +54,11 @@ class UniformLevel:
         self._dt_ready = True
         return self.dt[0]
 
-    def step(self, fused=True):
+    def step(self, fused=True, events=None):
         """One level step: dt = courant(u); u <- godunov(u, dt).  Returns the
-        device tensor holding the dt used (no host synchronisation)."""
+        device tensor holding the dt used (no host synchronisation).
+        ``events`` = (start, end) torch events recorded around the update
+        kernel alone."""
         s = _stream_handle(self.stream)
         if not self._dt_ready:
             self.courant()
@@ -66,8 +70,12 @@ class UniformLevel:
             import torch  # noqa: F401
             start = min(self.boxlen / self.smallc, self._courant_cap())
             _check(self.lib.ramses_uniform_dt_init(d_next, start, s), "ramses_uniform_dt_init")
+        if events is not None:
+            events[0].record(self._torch_stream())
         _check(self.lib.ramses_uniform_godunov(self._ptr(self.u), self._ptr(self.unew), self.n, self.dx,
                                                d_dt, d_next if fused else None, s), "ramses_uniform_godunov")
+        if events is not None:
+            events[1].record(self._torch_stream())
         used = self.dt[0].clone()
         self.u, self.unew = self.unew, self.u
         if fused:
@@ -76,6 +84,20 @@ class UniformLevel:
             self._dt_ready = False
         self.steps += 1
         return used
+
+    def update_only(self):
+        """Launch the update kernel into the spare buffer without advancing
+        the level (used by the autotuner to time launch shapes)."""
+        s = _stream_handle(self.stream)
+        if not self._dt_ready:
+            self.courant()
+        _check(self.lib.ramses_uniform_godunov(self._ptr(self.u), self._ptr(self.unew), self.n, self.dx,
+                                               self._ptr(self.dt, 0), self._ptr(self.dt, 1), s),
+               "ramses_uniform_godunov")
+
+    def _torch_stream(self):
+        import torch
+        return torch.cuda.current_stream() if self.stream is None else self.stream
 
     def _courant_cap(self):
         from . import _config_cache
