@@ -36,11 +36,16 @@ struct Tuning {
 };
 Tuning& tuning();
 
-// Fortran MAX/MIN of two reals as the reference build evaluates them.
-template <typename R>
-__device__ __forceinline__ R fmax_f(R a, R b) { return (a > b) ? a : b; }
-template <typename R>
-__device__ __forceinline__ R fmin_f(R a, R b) { return (a < b) ? a : b; }
+// Fortran MAX/MIN of two reals.  The hardware min/max (IEEE maxNum/minNum)
+// returns the same value as the reference for every non-NaN pair; the only
+// possible difference, the sign of a zero result when +0 and -0 tie, cannot
+// reach any output of the path: every tie of that kind feeds either a sum
+// with a nonzero term (SL/SR of hllc/hll) or is excluded by a dlft*drgt>0
+// test (minmod), and every other MAX/MIN of the path has a positive operand.
+__device__ __forceinline__ double fmax_f(double a, double b) { return __builtin_fmax(a, b); }
+__device__ __forceinline__ double fmin_f(double a, double b) { return __builtin_fmin(a, b); }
+__device__ __forceinline__ float fmax_f(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ float fmin_f(float a, float b) { return __builtin_fminf(a, b); }
 
 // Division / square root in the working precision.  FP64 keeps IEEE
 // correctly rounded division and sqrt (the reference's semantics); the FP32
@@ -64,12 +69,29 @@ struct Prim {
 // ctoprim for one cell (hydro/umuscl.f90:861-965, NENER=0, no sound speed:
 // the MUSCL trace does not read `c`).  Density floor, 1/rho, velocities and
 // E/rho - kinetic are evaluated in FP64; the result is rounded to R.
+// FP64 reciprocal to about 1 ulp: hardware estimate refined by two Newton
+// steps.  Used only by the FP32-mixed path, whose FP64 parts need FP64
+// accuracy but not the reference's correctly rounded division.
+__device__ __forceinline__ double rcp_fast(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
+template <typename R>
+__device__ __forceinline__ double recip(double x) {
+  if constexpr (sizeof(R) == 8) return 1.0 / x;  // hydro/umuscl.f90:897, correctly rounded
+  else return rcp_fast(x);
+}
+
 template <typename R>
 __device__ __forceinline__ Prim<R> ctoprim_cell(double u0, double u1, double u2, double u3, double u4,
                                                 double gx, double gy, double gz, double dtxhalf,
                                                 double smalle, const HydroParams& hp) {
   const double r = fmax_f(u0, hp.smallr);                  // :894
-  const double oneoverrho = 1.0 / r;                        // :897
+  const double oneoverrho = recip<R>(r);                    // :897
   const double vx = u1 * oneoverrho;                        // :898-904
   const double vy = u2 * oneoverrho;
   const double vz = u3 * oneoverrho;
@@ -366,6 +388,36 @@ __device__ __forceinline__ Flux6<R> riemann_hllc(const Face<R>& L, const Face<R>
   return g;
 }
 
+// Scaling of a face flux by dt/dx as the reference writes it, (f*dt)/dx
+// (hydro/umuscl.f90:106,129,153).  When dx is a power of two the division is
+// the exact multiplication by 1/dx (inv_dx != 0), which the FP64 path uses
+// to stay bitwise while avoiding a divide; the FP32-mixed path multiplies by
+// dt/dx in FP64.
+struct FluxScale {
+  double dt, dx, inv_dx, dtdx;
+};
+
+__host__ __device__ inline FluxScale make_scale(double dt, double dx) {
+  FluxScale s;
+  s.dt = dt;
+  s.dx = dx;
+  s.dtdx = dt / dx;
+  int e = 0;
+  const double m = __builtin_frexp(dx, &e);
+  s.inv_dx = (m == 0.5) ? 1.0 / dx : 0.0;
+  return s;
+}
+
+template <typename R>
+__device__ __forceinline__ double scale_flux(R f, const FluxScale& s) {
+  if constexpr (sizeof(R) == 8) {
+    const double fd = (double)f * s.dt;
+    return s.inv_dx != 0.0 ? fd * s.inv_dx : fd / s.dx;
+  } else {
+    return (double)f * s.dtdx;
+  }
+}
+
 // Solver constants in working precision.
 template <typename R>
 struct SolverConsts {
@@ -437,6 +489,39 @@ __device__ __forceinline__ double cmpdt_cell(double u0, double u1, double u2, do
   r = r + __builtin_fabs(gx);
   r = r + __builtin_fabs(gy);
   r = r + __builtin_fabs(gz);
+  r = r * dx / (w * w);
+  r = fmax_f(r, 0.0001);
+  return dx / w * (__builtin_sqrt(1.0 + 2.0 * hp.courant_factor * r) - 1.0) / r;
+}
+
+// Without gravity cmpdt's gravity ratio is max(0*dx/w^2, 1e-4) = 1e-4 for
+// every cell, so the per-cell limit dx/w*(sqrt(1+2C*1e-4)-1)/1e-4 is a
+// non-increasing function of the wave speed w alone (each IEEE operation is
+// monotone), and min over cells equals its value at max over cells of w.
+// cmpdt_w is the per-cell part (hydro/godunov_utils.f90:24-97), cmpdt_from_w
+// the rest (:108-118), bitwise the reference when combined.
+__device__ __forceinline__ double cmpdt_w(double u0, double u1, double u2, double u3, double u4,
+                                          const HydroParams& hp) {
+  const double gamma = hp.gamma;
+  const double smallp = hp.smallc * hp.smallc / gamma;
+  const double rho = fmax_f(u0, hp.smallr);
+  const double vx = u1 / rho, vy = u2 / rho, vz = u3 / rho;
+  double e = u4;
+  e = e - 0.5 * rho * (vx * vx);
+  e = e - 0.5 * rho * (vy * vy);
+  e = e - 0.5 * rho * (vz * vz);
+  double c = fmax_f((gamma - 1.0) * e, rho * smallp);
+  c = gamma * c;
+  c = __builtin_sqrt(c / rho);
+  double w = 3.0 * c;
+  w = w + __builtin_fabs(vx);
+  w = w + __builtin_fabs(vy);
+  w = w + __builtin_fabs(vz);
+  return w;
+}
+
+__device__ __forceinline__ double cmpdt_from_w(double w, double dx, const HydroParams& hp) {
+  double r = 0.0;
   r = r * dx / (w * w);
   r = fmax_f(r, 0.0001);
   return dx / w * (__builtin_sqrt(1.0 + 2.0 * hp.courant_factor * r) - 1.0) / r;

@@ -72,6 +72,7 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void uniform_godunov_kernel(UniformAr
   const double dtxhalf = dt * 0.5;
   const double smalle = hp.smallc * hp.smallc / hp.gamma / (hp.gamma - 1.0);
   const R dtdx = (R)(dt / dx);
+  const FluxScale fs = make_scale(dt, dx);
 
   // cooperative ctoprim of plane zp (periodic) into slot s
   auto load_plane = [&](int zp, int s) {
@@ -99,7 +100,7 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void uniform_godunov_kernel(UniformAr
   Face<R> fz_left{};          // rotated qm_z of plane p-1
   double fz_prev[5] = {0, 0, 0, 0, 0};  // flux through the z face below plane p-1 (scaled)
   double part_prev[5] = {0, 0, 0, 0, 0};  // (u + dFx) + dFy of plane p-1
-  double dtmin = 1e300;
+  double wmax = 0.0;  // largest cmpdt wave speed of the new state
 
   // prologue: planes z0-2 and z0-1
   load_plane(z0 - 2, 0);
@@ -145,7 +146,7 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void uniform_godunov_kernel(UniformAr
         R fc[5];
         unrotate(g, 0, fc);
 #pragma unroll
-        for (int v = 0; v < 5; ++v) fxs[v] = (double)fc[v] * dt / dx;  // hydro/umuscl.f90:106
+        for (int v = 0; v < 5; ++v) fxs[v] = scale_flux(fc[v], fs);  // hydro/umuscl.f90:106
       }
       if (active && inx && ey >= 1) {  // y face at y-1/2
         const Prim<R> l{xch[5][t - EX], xch[6][t - EX], xch[7][t - EX], xch[8][t - EX], xch[9][t - EX]};
@@ -153,7 +154,7 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void uniform_godunov_kernel(UniformAr
         R fc[5];
         unrotate(g, 1, fc);
 #pragma unroll
-        for (int v = 0; v < 5; ++v) fys[v] = (double)fc[v] * dt / dx;  // :129
+        for (int v = 0; v < 5; ++v) fys[v] = scale_flux(fc[v], fs);  // :129
       }
       __syncthreads();
       // fluxes are stored in double in the exchange area reinterpreted per variable pair
@@ -189,7 +190,7 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void uniform_godunov_kernel(UniformAr
       R fc[5];
       unrotate(g, 2, fc);
 #pragma unroll
-      for (int v = 0; v < 5; ++v) fz[v] = (double)fc[v] * dt / dx;  // :153
+      for (int v = 0; v < 5; ++v) fz[v] = scale_flux(fc[v], fs);  // :153
     }
     // plane p-1 is complete: hydro/godunov_fine.f90:751-792 in idim order x, y, z
     if (interior && p >= z0 + 1) {
@@ -202,7 +203,7 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void uniform_godunov_kernel(UniformAr
         un[v] = s;
         a.unew[o + v * vstride] = s;
       }
-      if (a.d_dt_next) dtmin = fmin_f(dtmin, cmpdt_cell(un[0], un[1], un[2], un[3], un[4], 0.0, 0.0, 0.0, dx, hp));
+      if (a.d_dt_next) wmax = fmax_f(wmax, cmpdt_w(un[0], un[1], un[2], un[3], un[4], hp));
     }
     // carry to the next plane
     fz_left = rotate(qm_z, 2);
@@ -216,33 +217,36 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void uniform_godunov_kernel(UniformAr
   }
 
   if (a.d_dt_next) {
-    // wavefront min with shuffles, then across the block's waves
-    for (int off = 32; off > 0; off >>= 1) dtmin = fmin_f(dtmin, __shfl_xor(dtmin, off, 64));
+    // wavefront max of w with shuffles, then across the block's waves; the
+    // block's dt limit is cmpdt_from_w(max w), min-combined across blocks
+    for (int off = 32; off > 0; off >>= 1) wmax = fmax_f(wmax, __shfl_xor(wmax, off, 64));
     const int wave = t >> 6, lane = t & 63;
-    if (lane == 0) red[wave] = dtmin;
+    if (lane == 0) red[wave] = wmax;
     __syncthreads();
     if (t == 0) {
       double m = red[0];
-      for (int w = 1; w < NT / 64; ++w) m = fmin_f(m, red[w]);
-      atomicMin(reinterpret_cast<unsigned long long*>(a.d_dt_next), dt_bits(m));
+      for (int w = 1; w < NT / 64; ++w) m = fmax_f(m, red[w]);
+      if (m > 0.0) atomicMin(reinterpret_cast<unsigned long long*>(a.d_dt_next), dt_bits(cmpdt_from_w(m, dx, hp)));
     }
   }
 }
 
+// courant_fine over a uniform level without gravity: max of the cmpdt wave
+// speed per block, then the block's dt limit min-combined across blocks
+// (exactly min over cells of cmpdt, see cmpdt_w).
 __global__ void courant_kernel(const double* u, long ncell, double dx, HydroParams hp, double* d_dt) {
-  double dtmin = 1e300;
+  double wmax = 0.0;
   for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += (long)gridDim.x * blockDim.x)
-    dtmin = fmin_f(dtmin, cmpdt_cell(u[c], u[c + ncell], u[c + 2 * ncell], u[c + 3 * ncell], u[c + 4 * ncell],
-                                     0.0, 0.0, 0.0, dx, hp));
-  for (int off = 32; off > 0; off >>= 1) dtmin = fmin_f(dtmin, __shfl_xor(dtmin, off, 64));
+    wmax = fmax_f(wmax, cmpdt_w(u[c], u[c + ncell], u[c + 2 * ncell], u[c + 3 * ncell], u[c + 4 * ncell], hp));
+  for (int off = 32; off > 0; off >>= 1) wmax = fmax_f(wmax, __shfl_xor(wmax, off, 64));
   __shared__ double red[4];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (lane == 0) red[wave] = dtmin;
+  if (lane == 0) red[wave] = wmax;
   __syncthreads();
   if (threadIdx.x == 0) {
     double m = red[0];
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmin_f(m, red[w]);
-    atomicMin(reinterpret_cast<unsigned long long*>(d_dt), dt_bits(m));
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmax_f(m, red[w]);
+    if (m > 0.0) atomicMin(reinterpret_cast<unsigned long long*>(d_dt), dt_bits(cmpdt_from_w(m, dx, hp)));
   }
 }
 

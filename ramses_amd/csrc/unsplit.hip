@@ -115,21 +115,20 @@ __device__ void slope_27(const RowCtx<R>& c, int i, int j, int k, Prim<R>& dqx, 
 template <typename R>
 __device__ __forceinline__ void store_face(double* flux, double* tmp, size_t fbase, size_t tbase, size_t fvs,
                                            int d, const Flux6<R>& g, const Face<R>& L, const Face<R>& Rt,
-                                           double dt, double dxd) {
+                                           const FluxScale& fs) {
   R fc[5];
   unrotate(g, d, fc);
 #pragma unroll
-  for (int v = 0; v < 5; ++v) flux[fbase + (size_t)v * fvs] = (double)fc[v] * dt / dxd;  // :106,129,153
-  const R t1 = (R)0.5 * (L.un + Rt.un);                                                   // :845
-  tmp[tbase] = (double)t1 * dt / dxd;                                                    // :111
-  tmp[tbase + fvs] = (double)g.f[5] * dt / dxd;
+  for (int v = 0; v < 5; ++v) flux[fbase + (size_t)v * fvs] = scale_flux(fc[v], fs);  // :106,129,153
+  const R t1 = (R)0.5 * (L.un + Rt.un);                                                // :845
+  tmp[tbase] = scale_flux(t1, fs);                                                    // :111
+  tmp[tbase + fvs] = scale_flux(g.f[5], fs);
 }
 
 // One row of faces along direction D of one oct (see the file comment).
 template <typename R, int D, int RS, bool S27>
 __device__ __forceinline__ void row_faces(const RowCtx<R>& c, const SolverConsts<R>& sc, int a1, int b1,
-                                          R dtdx, double* flux, double* tmp, size_t fvs, double dt,
-                                          double dxd) {
+                                          R dtdx, double* flux, double* tmp, size_t fvs, const FluxScale fs) {
   constexpr int T1 = (D == 0) ? 1 : 0;  // first transverse axis
   constexpr int T2 = (D == 2) ? 1 : 2;  // second transverse axis
   int ci, cj, ck;
@@ -172,8 +171,7 @@ __device__ __forceinline__ void row_faces(const RowCtx<R>& c, const SolverConsts
       const Face<R> fr = rotate(qp, D);
       const Flux6<R> g = riemann_fixed<R, RS>(fl_prev, fr, sc);
       const size_t fslot = (size_t)((i - 1) + 3 * ((j - 1) + 3 * (k - 1))) * c.nv;
-      store_face(flux, tmp, fslot + (size_t)D * 5 * fvs, fslot + (size_t)D * 2 * fvs, fvs, D, g, fl_prev, fr,
-                 dt, dxd);
+      store_face(flux, tmp, fslot + (size_t)D * 5 * fvs, fslot + (size_t)D * 2 * fvs, fvs, D, g, fl_prev, fr, fs);
     }
     fl_prev = rotate(qm, D);
     qa = qb;
@@ -218,11 +216,11 @@ __global__ __launch_bounds__(768) void unsplit_rows(UnsplitArgs a) {
   double* tmp = a.tmp + gidx * gsz_t + l;
   const size_t fvs = (size_t)nv * 27;
   if (d == 0)
-    row_faces<R, 0, RS, S27>(c, sc, a1, b1, (R)(a.dt / a.dx[0]), flux, tmp, fvs, a.dt, a.dx[0]);  // :516
+    row_faces<R, 0, RS, S27>(c, sc, a1, b1, (R)(a.dt / a.dx[0]), flux, tmp, fvs, make_scale(a.dt, a.dx[0]));  // :516
   else if (d == 1)
-    row_faces<R, 1, RS, S27>(c, sc, a1, b1, (R)(a.dt / a.dx[1]), flux, tmp, fvs, a.dt, a.dx[1]);
+    row_faces<R, 1, RS, S27>(c, sc, a1, b1, (R)(a.dt / a.dx[1]), flux, tmp, fvs, make_scale(a.dt, a.dx[1]));
   else
-    row_faces<R, 2, RS, S27>(c, sc, a1, b1, (R)(a.dt / a.dx[2]), flux, tmp, fvs, a.dt, a.dx[2]);
+    row_faces<R, 2, RS, S27>(c, sc, a1, b1, (R)(a.dt / a.dx[2]), flux, tmp, fvs, make_scale(a.dt, a.dx[2]));
 }
 
 }  // namespace
