@@ -64,12 +64,15 @@ def make_inputs(torch, noct, seed=1):
     return uin, dtmin
 
 
-def time_variant(torch, ra, uin, flux, tmp, dt, riemann, precision, steps, warmup, tune=True):
+def time_variant(torch, ra, uin, flux, tmp, dt, riemann, precision, steps, warmup, tune=True, block=0):
     from ramses_amd import autotune
     ra.configure(gamma=GAMMA, riemann=riemann, slope_type=1, courant_factor=CFL, precision=precision)
     stream = torch.cuda.current_stream()
     tuned = None
-    if tune:  # untimed: pick the workgroup size on these inputs
+    if not tune:
+        autotune._set(autotune.UNSPLIT_BLOCK, block)
+        tuned = (block or 256, None)
+    else:  # untimed: pick the workgroup size on these inputs
         tuned = autotune.tune_unsplit(lambda: ra.unsplit_device(uin, None, flux, tmp, DX, dt, stream=stream),
                                       key=("unsplit", uin.shape[0], precision, riemann))
     for _ in range(warmup):
@@ -151,7 +154,12 @@ def level_bench(torch, ra, args):
                      precision=prec)
         lvl = ra.UniformLevel(u0)
         from ramses_amd import autotune
-        tuned = autotune.tune_level(lvl.update_only, n, key=("level", n, prec))  # untimed
+        if args.no_tune:
+            autotune._set(autotune.LEVEL_TILE, args.level_tile)
+            autotune._set(autotune.LEVEL_TZ, args.level_tz)
+            tuned = ((args.level_tile, args.level_tz), None)
+        else:
+            tuned = autotune.tune_level(lvl.update_only, n, key=("level", n, prec))  # untimed
         for _ in range(args.warmup):
             lvl.step()
         lvl = ra.UniformLevel(u0)  # restart from the IC for the timed run
@@ -192,6 +200,70 @@ def level_bench(torch, ra, args):
     }
 
 
+def serve_bench(torch, ra, args):
+    """Continuous batching of independent periodic solves: sides log-uniform in
+    {16,32,64,128} (seed 3), request i's velocity field a k^-4 Gaussian field at
+    rms Mach 2 (seed 1000+i), rho=1, P=1/gamma, gamma=5/3, to t=0.1 sound
+    crossing times, FP32-mixed, hllc, minmod.  Capacity is measured first with
+    a closed batch; arrivals are then Poisson at `load` x capacity."""
+    import numpy as np
+    from ramses_amd import ics
+    from ramses_amd.serve import HydroServer, Request
+    ra.configure(gamma=5.0 / 3.0, riemann="hllc", slope_type=1, courant_factor=0.5, precision="fp32")
+    rng = np.random.default_rng(3)
+    sides = [int(16 * 2 ** rng.integers(0, 4)) for _ in range(args.requests)]
+
+    def ic(i):
+        return lambda n: ics.turbulent(n, 2.0, gamma=5.0 / 3.0, rho_sigma=0.0, seed=1000 + i)
+
+    budget = 64 << 30
+    # warm-up (untimed): FFT plans of the IC generator and the kernels
+    warm = [Request(-1 - j, n, 0.01, 0.0, ic(j)) for j, n in enumerate((16, 32, 64, 128) * 2)]
+    HydroServer(hbm_budget=budget, steps_per_tick=args.steps_per_tick).run(warm)
+    torch.cuda.synchronize()
+    # capacity: a closed batch of the first 128 solves, all present at t=0
+    cap_reqs = [Request(i, sides[i], 0.1, 0.0, ic(i)) for i in range(min(128, args.requests))]
+    srv = HydroServer(hbm_budget=budget, steps_per_tick=args.steps_per_tick)
+    t0 = time.perf_counter()
+    srv.run(cap_reqs)
+    cap_wall = time.perf_counter() - t0
+    capacity = len(cap_reqs) / cap_wall
+    cap_cells = sum(r.n ** 3 * r.steps for r in cap_reqs) / cap_wall
+    runs = {}
+    for load in [float(x) for x in str(args.load).split(",")]:
+        lam = load * capacity
+        arr = np.cumsum(np.random.default_rng(30).exponential(1.0 / lam, args.requests))
+        reqs = [Request(i, sides[i], 0.1, float(arr[i]), ic(i)) for i in range(args.requests)]
+        srv = HydroServer(hbm_budget=budget, steps_per_tick=args.steps_per_tick)
+        t0 = time.perf_counter()
+        srv.run(reqs)
+        wall = time.perf_counter() - t0
+        lat = np.array(sorted(r.latency for r in reqs))
+        cells = sum(r.n ** 3 * r.steps for r in reqs)
+        runs[load] = {"arrival_rate_per_s": lam, "wall_s": wall, "cell_updates_per_s": cells / wall,
+                      "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                                    "max": float(lat[-1])},
+                      "ticks": srv.ticks, "max_in_flight": srv.max_active, "max_hbm_bytes": srv.max_used,
+                      "total_steps": int(sum(r.steps for r in reqs))}
+    head = runs[max(runs)]
+    wall, cells = head["wall_s"], head["cell_updates_per_s"] * head["wall_s"]
+    ra.configure()
+    return {
+        "metric": "cell-updates/s FP64 vs FP32-mixed, % HBM roofline; p50/p99 solve latency",
+        "value": cells / wall, "unit": "cell-updates/s", "n_gpus": 1, "steps": head["ticks"], "warmup": 8,
+        "ms_per_step": 1e3 * wall / max(1, head["ticks"]), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32-mixed", "data": "synthetic",
+        "config": {"workload": f"serving: {args.requests} periodic solves, side log-uniform in {{16,32,64,128}}, "
+                               "Mach-2 k^-4 field, to t=0.1, FP32-mixed, continuous batching",
+                   "requests": args.requests, "parallelism": "1 GPU"},
+        "latency_s": head["latency_s"],
+        "capacity_solves_per_s": capacity, "capacity_cell_updates_per_s": cap_cells,
+        "offered_load": max(runs), "runs_by_load": {str(k): v for k, v in runs.items()},
+        "hbm_budget_bytes": budget,
+        "steps_per_tick": args.steps_per_tick,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,8 +272,15 @@ def main():
     ap.add_argument("--octs", type=int, default=NOCT)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--variants", default="fp64-hllc,fp32-hllc,fp64-llf,fp32-llf")
-    ap.add_argument("--workload", default="unsplit", choices=["unsplit", "sedov64", "turb512", "turb256"],
+    ap.add_argument("--workload", default="unsplit", choices=["unsplit", "sedov64", "turb512", "turb256", "serve"],
                     help="unsplit (default, the headline) or a resident-level workload")
+    ap.add_argument("--requests", type=int, default=2000, help="serve: number of solves")
+    ap.add_argument("--load", default="1.2,0.8", help="serve: offered loads / measured capacity (comma list)")
+    ap.add_argument("--steps-per-tick", type=int, default=8, help="serve: level steps per scheduler tick")
+    ap.add_argument("--no-tune", action="store_true", help="use the shapes given below instead of autotuning")
+    ap.add_argument("--unsplit-block", type=int, default=0, help="with --no-tune: unsplit workgroup size")
+    ap.add_argument("--level-tile", type=int, default=0, help="with --no-tune: 1=32x8, 2=16x8, 3=8x8")
+    ap.add_argument("--level-tz", type=int, default=0, help="with --no-tune: z planes per workgroup")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="run only a device copy of this many MiB (for HBM counter scaling)")
     args = ap.parse_args()
@@ -223,6 +302,9 @@ def main():
         print(json.dumps({"calibrate_copy_bytes": n * 8}))
         return
 
+    if args.workload == "serve":
+        print(json.dumps(serve_bench(torch, ra, args)), flush=True)
+        return
     if args.workload != "unsplit":
         print(json.dumps(level_bench(torch, ra, args)), flush=True)
         return
@@ -237,7 +319,8 @@ def main():
     results = {}
     for v in args.variants.split(","):
         prec, riemann = v.split("-")
-        results[v] = time_variant(torch, ra, uin, flux, tmp, dt, riemann, prec, args.steps, args.warmup)
+        results[v] = time_variant(torch, ra, uin, flux, tmp, dt, riemann, prec, args.steps, args.warmup,
+                                  tune=not args.no_tune, block=args.unsplit_block)
     ra.configure()
 
     cells = 8 * noct

@@ -117,6 +117,41 @@ int ramses_uniform_godunov(const double *u, double *unew, int n, double dx, cons
 int ramses_uniform_dt_init(double *d_dt, double value, void *stream);
 
 /* ---------------------------------------------------------------------
+ * Continuous batching of independent levels (serving).  Each slot is one
+ * periodic uniform level advanced by the per-level step of amr_step
+ * (amr/amr_step.f90:327-424: newdt_fine/courant_fine, godunov_fine,
+ * set_uold) with its own dt, capped at its end time as amr_step caps dt at
+ * the next output time.  Slots of different sizes share one launch.
+ * --------------------------------------------------------------------- */
+#define RAMSES_MAX_SLOTS 32
+/* Slot state: 8 device doubles, read by the host to follow progress. */
+#define RAMSES_SLOT_TIME 0    /* t */
+#define RAMSES_SLOT_TEND 1    /* end time */
+#define RAMSES_SLOT_DT 2      /* dt of the last step */
+#define RAMSES_SLOT_DTNEXT 3  /* courant dt of the current state */
+#define RAMSES_SLOT_DONE 4    /* 1 once t >= end time */
+#define RAMSES_SLOT_STEPS 5   /* steps taken */
+#define RAMSES_SLOT_PARITY 6  /* buf[parity] holds the current state */
+#define RAMSES_SLOT_STEPPED 7 /* internal */
+
+typedef struct ramses_level_slot {
+  double *buf[2]; /* two device buffers u[var][z][y][x], n^3 per variable */
+  double *state;  /* 8 device doubles (RAMSES_SLOT_*) */
+  int n;          /* cells per side, a multiple of 16 */
+  double dx;      /* cell size */
+} ramses_level_slot;
+
+/* Start a slot at t=0 from buf[0]: courant_fine of the initial state
+ * (hydro/courant_fine.f90:1-159) into its state. */
+int ramses_level_slot_init(const ramses_level_slot *slot, double t_end, void *stream);
+
+/* One step of every slot that is not done (the godunov_fine sweep,
+ * hydro/godunov_fine.f90:5-35, of each level, fused courant reduction of
+ * the new state).  Results depend only on each slot's own data: a slot
+ * advanced alone or in any mix gives the same bits. */
+int ramses_level_batch_step(const ramses_level_slot *slots, int nslots, void *stream);
+
+/* ---------------------------------------------------------------------
  * Launch-shape tuning.  The reference's only performance knob is the
  * compile-time vector length nvector (amr/amr_parameters.f90:49-53, set by
  * bin/Makefile:11); on the GPU the equivalent choices are the workgroup and

@@ -20,6 +20,9 @@ hipError_t launch_set_double(double* p, double v, hipStream_t s);
 hipError_t launch_courant(const double* u, int n, double dx, const HydroParams& hp, double* d_dt, hipStream_t s);
 hipError_t launch_uniform_godunov(const double* u, double* unew, int n, double dx, const double* d_dt,
                                   double* d_dt_next, const HydroParams& hp, bool fp32, hipStream_t s);
+hipError_t launch_level_batch(double* const* buf0, double* const* buf1, double* const* state, const int* n,
+                              const double* dx, int nslots, const HydroParams& hp, bool fp32, hipStream_t s);
+hipError_t launch_init_slot(double* state, double t_end, double start, hipStream_t s);
 Tuning& tuning() {
   static Tuning t{0, 0, 0};
   return t;
@@ -236,6 +239,53 @@ int ramses_tune_get(int knob) {
     case RAMSES_TUNE_LEVEL_TZ: return t.level_tz;
     default: return fail(-1, "unknown tuning knob");
   }
+}
+
+int ramses_level_slot_init(const ramses_level_slot* slot, double t_end, void* stream) {
+  const Config c = current();
+  if (!slot || !slot->buf[0] || !slot->buf[1] || !slot->state) return fail(-1, "null slot pointer");
+  if (slot->n < 16 || slot->n % 16 != 0) return fail(-1, "slot n must be a multiple of 16");
+  if (!(slot->dx > 0)) return fail(-1, "dx must be > 0");
+  const double boxlen = slot->n * slot->dx;
+  double start = boxlen / c.hp.smallc;
+  const double cap = c.hp.courant_factor * slot->dx / c.hp.smallc;
+  if (cap < start) start = cap;
+  hipError_t e = ramses::launch_init_slot(slot->state, t_end, start, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "slot init launch");
+  e = ramses::launch_courant(slot->buf[0], slot->n, slot->dx, c.hp, slot->state + 3, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "slot courant launch");
+  return 0;
+}
+
+int ramses_level_batch_step(const ramses_level_slot* slots, int nslots, void* stream) {
+  const Config c = current();
+  if (nslots < 0 || nslots > RAMSES_MAX_SLOTS) return fail(-1, "nslots out of [0, RAMSES_MAX_SLOTS]");
+  if (nslots == 0) return 0;
+  if (!slots) return fail(-1, "null slots");
+  if (c.hp.slope_type == 3) return fail(-4, "slope_type 3 is not supported on the resident level path");
+  double* b0[RAMSES_MAX_SLOTS];
+  double* b1[RAMSES_MAX_SLOTS];
+  double* st[RAMSES_MAX_SLOTS];
+  int n[RAMSES_MAX_SLOTS];
+  double dx[RAMSES_MAX_SLOTS];
+  long blocks = 0;
+  for (int i = 0; i < nslots; ++i) {
+    const ramses_level_slot& s = slots[i];
+    if (!s.buf[0] || !s.buf[1] || !s.state) return fail(-1, "null slot pointer");
+    if (s.n < 16 || s.n % 16 != 0) return fail(-1, "slot n must be a multiple of 16");
+    b0[i] = s.buf[0];
+    b1[i] = s.buf[1];
+    st[i] = s.state;
+    n[i] = s.n;
+    dx[i] = s.dx;
+    const int tz = ramses::batch_tz(s.n);
+    blocks += (long)(s.n / 16) * (s.n / 8) * (s.n / tz);
+  }
+  if (blocks > 2147483647L) return fail(-1, "batch too large");
+  hipError_t e = ramses::launch_level_batch(b0, b1, st, n, dx, nslots, c.hp, c.precision == RAMSES_FP32_MIXED,
+                                           (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "level batch launch");
+  return 0;
 }
 
 int ramses_uniform_dt_init(double* d_dt, double value, void* stream) {

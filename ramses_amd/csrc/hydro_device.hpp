@@ -527,6 +527,10 @@ __device__ __forceinline__ double cmpdt_from_w(double w, double dx, const HydroP
   return dx / w * (__builtin_sqrt(1.0 + 2.0 * hp.courant_factor * r) - 1.0) / r;
 }
 
+// z planes per workgroup of a batched level slot (n a multiple of 16): a
+// divisor of n, at most 32.
+__host__ __device__ inline int batch_tz(int n) { return n <= 32 ? n : (n % 32 == 0 ? 32 : 16); }
+
 // Positive doubles order like their bit patterns, so a min-reduction of dt
 // can use an integer atomic.
 __device__ __forceinline__ unsigned long long dt_bits(double x) {

@@ -38,11 +38,34 @@ struct UniformArgs {
   HydroParams hp;
 };
 
+// Slot state layout (device doubles, see ramses_level_slot in the header).
+enum SlotField : int { kTime = 0, kTend = 1, kDtUse = 2, kDtNext = 3, kDone = 4, kSteps = 5, kParity = 6, kStepped = 7 };
+
+constexpr int kMaxSlots = 32;
+
+struct BatchSlot {
+  double* buf[2];
+  double* state;
+  int n;
+  int tz;
+  double dx;
+};
+
+struct BatchArgs {
+  BatchSlot slot[kMaxSlots];
+  int blk0[kMaxSlots + 1];
+  int nslots;
+  HydroParams hp;
+};
+
 __device__ __forceinline__ int wrapi(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
+// One workgroup's tile of a level: columns [x0, x0+TX) x [y0, y0+TY),
+// planes [z0, z0+TZ).  Shared by the single-level and the batched kernels.
 template <typename R, int TX, int TY, int RS>
-__global__ __launch_bounds__(((TX + 2) * (TY + 2) + 63) / 64 * 64)
-__attribute__((amdgpu_waves_per_eu(3, 8))) void uniform_godunov_kernel(UniformArgs a, int TZ) {
+__device__ __forceinline__ void level_tile(const double* __restrict__ u, double* __restrict__ unew, int n, double dx,
+                                           double dt, double* d_dt_next, const HydroParams& hp, int x0, int y0,
+                                           int z0, int TZ) {
   constexpr int EX = TX + 2, EY = TY + 2;      // extended tile (threads)
   constexpr int QX = TX + 4, QY = TY + 4;      // primitive plane with 2-cell halo
   constexpr int NQ = QX * QY;
@@ -53,10 +76,8 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void uniform_godunov_kernel(UniformAr
   __shared__ __align__(16) R xch[10][NE];  // exchange: traced left states, then fluxes
   __shared__ double red[NT / 64];
 
-  const int n = a.n;
   const long plane = (long)n * n;
   const long vstride = plane * n;
-  const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = blockIdx.z * TZ;
   const int t = threadIdx.x;
   const bool active = t < NE;
   const int ex = active ? t % EX : 0, ey = active ? t / EX : 0;
@@ -65,10 +86,7 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void uniform_godunov_kernel(UniformAr
   const int xg = wrapi(x0 - 1 + ex, n), yg = wrapi(y0 - 1 + ey, n);
   const int qo = (ey + 1) * QX + (ex + 1);  // own cell in the primitive plane
 
-  const HydroParams hp = a.hp;
   const SolverConsts<R> sc = make_consts<R>(hp);
-  const double dt = *a.d_dt;
-  const double dx = a.dx;
   const double dtxhalf = dt * 0.5;
   const double smalle = hp.smallc * hp.smallc / hp.gamma / (hp.gamma - 1.0);
   const R dtdx = (R)(dt / dx);
@@ -81,8 +99,8 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void uniform_godunov_kernel(UniformAr
       const int qx = c % QX, qy = c / QX;
       const int gx = wrapi(x0 - 2 + qx, n), gy = wrapi(y0 - 2 + qy, n);
       const long o = (long)zg * plane + (long)gy * n + gx;
-      const Prim<R> q = ctoprim_cell<R>(a.u[o], a.u[o + vstride], a.u[o + 2 * vstride], a.u[o + 3 * vstride],
-                                        a.u[o + 4 * vstride], 0.0, 0.0, 0.0, dtxhalf, smalle, hp);
+      const Prim<R> q = ctoprim_cell<R>(u[o], u[o + vstride], u[o + 2 * vstride], u[o + 3 * vstride],
+                                        u[o + 4 * vstride], 0.0, 0.0, 0.0, dtxhalf, smalle, hp);
       qs[s][0][c] = q.r;
       qs[s][1][c] = q.u;
       qs[s][2][c] = q.v;
@@ -170,7 +188,7 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void uniform_godunov_kernel(UniformAr
         // hydro/godunov_fine.f90:770-772, idim=1: unew = uold + (flux(i) - flux(i+1))
         const long o = (long)wrapi(p, n) * plane + (long)yg * n + xg;
 #pragma unroll
-        for (int v = 0; v < 5; ++v) part[v] = a.u[o + v * vstride] + (fx_sh[v * FSTRIDE + t] - fx_sh[v * FSTRIDE + t + 1]);
+        for (int v = 0; v < 5; ++v) part[v] = u[o + v * vstride] + (fx_sh[v * FSTRIDE + t] - fx_sh[v * FSTRIDE + t + 1]);
       }
       __syncthreads();
       if (active) {
@@ -201,9 +219,9 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void uniform_godunov_kernel(UniformAr
       for (int v = 0; v < 5; ++v) {
         const double s = part_prev[v] + (fz_prev[v] - fz[v]);  // idim=3
         un[v] = s;
-        a.unew[o + v * vstride] = s;
+        unew[o + v * vstride] = s;
       }
-      if (a.d_dt_next) wmax = fmax_f(wmax, cmpdt_w(un[0], un[1], un[2], un[3], un[4], hp));
+      if (d_dt_next) wmax = fmax_f(wmax, cmpdt_w(un[0], un[1], un[2], un[3], un[4], hp));
     }
     // carry to the next plane
     fz_left = rotate(qm_z, 2);
@@ -216,7 +234,7 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void uniform_godunov_kernel(UniformAr
     qcur = qnext;
   }
 
-  if (a.d_dt_next) {
+  if (d_dt_next) {
     // wavefront max of w with shuffles, then across the block's waves; the
     // block's dt limit is cmpdt_from_w(max w), min-combined across blocks
     for (int off = 32; off > 0; off >>= 1) wmax = fmax_f(wmax, __shfl_xor(wmax, off, 64));
@@ -226,9 +244,78 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void uniform_godunov_kernel(UniformAr
     if (t == 0) {
       double m = red[0];
       for (int w = 1; w < NT / 64; ++w) m = fmax_f(m, red[w]);
-      if (m > 0.0) atomicMin(reinterpret_cast<unsigned long long*>(a.d_dt_next), dt_bits(cmpdt_from_w(m, dx, hp)));
+      if (m > 0.0) atomicMin(reinterpret_cast<unsigned long long*>(d_dt_next), dt_bits(cmpdt_from_w(m, dx, hp)));
     }
   }
+}
+
+
+template <typename R, int TX, int TY, int RS>
+__global__ __launch_bounds__(((TX + 2) * (TY + 2) + 63) / 64 * 64)
+__attribute__((amdgpu_waves_per_eu(3, 8))) void uniform_godunov_kernel(UniformArgs a, int TZ) {
+  // XCD-aware tile order: the hardware deals workgroups round-robin to the
+  // 8 XCDs, so workgroup b runs on XCD b%8.  Giving each XCD a contiguous
+  // range of tiles (x fastest, then y, then z chunk) keeps the tiles that run
+  // side by side on one XCD spatial neighbours, and their shared halo
+  // columns are served by that XCD's L2 instead of HBM.
+  const int ntx = gridDim.x, nty = gridDim.y;
+  const int nb = ntx * nty * gridDim.z;
+  const int b = blockIdx.x + ntx * (blockIdx.y + nty * blockIdx.z);
+  const int per_xcd = (nb + 7) / 8;
+  int logical = (b & 7) * per_xcd + (b >> 3);
+  if ((nb & 7) != 0) logical = b;  // ragged grids keep the plain order
+  const int tx = logical % ntx, ty = (logical / ntx) % nty, tzc = logical / (ntx * nty);
+  level_tile<R, TX, TY, RS>(a.u, a.unew, a.n, a.dx, *a.d_dt, a.d_dt_next, a.hp, tx * TX, ty * TY, tzc * TZ, TZ);
+}
+
+// Many independent levels advanced by one launch (continuous batching).
+// Workgroups [blk0[s], blk0[s+1]) belong to slot s; a slot whose state says
+// `done` is skipped.  The slot's buffer parity selects input and output.
+template <typename R, int TX, int TY, int RS>
+__global__ __launch_bounds__(((TX + 2) * (TY + 2) + 63) / 64 * 64)
+__attribute__((amdgpu_waves_per_eu(3, 8))) void batch_godunov_kernel(BatchArgs a) {
+  const int b = blockIdx.x;
+  int s = 0;
+  while (s + 1 < a.nslots && b >= a.blk0[s + 1]) ++s;
+  const BatchSlot& sl = a.slot[s];
+  double* st = sl.state;
+  if (st[kDone] != 0.0) return;
+  const int par = (int)st[kParity];
+  const double* u = sl.buf[par];
+  double* unew = sl.buf[par ^ 1];
+  const int n = sl.n;
+  const int ntx = n / TX, nty = n / TY;
+  const int local = b - a.blk0[s];
+  const int tx = local % ntx, ty = (local / ntx) % nty, tzc = local / (ntx * nty);
+  level_tile<R, TX, TY, RS>(u, unew, n, sl.dx, st[kDtUse], st + kDtNext, a.hp, tx * TX, ty * TY, tzc * sl.tz, sl.tz);
+}
+
+// Per-slot bookkeeping before each batched step (one thread per slot):
+// finish the previous step (t += dt, flip the buffer parity), stop at t_end,
+// and choose this step's dt = min(courant dt, t_end - t) as the reference
+// caps dt at the next output time; then reset the fused courant accumulator
+// to newdt_fine's start value (pm/newdt_fine.f90:51) capped by cmpdt's
+// C*dx/smallc (hydro/godunov_utils.f90:114).
+__global__ void batch_advance_kernel(BatchArgs a) {
+  const int s = threadIdx.x;
+  if (s >= a.nslots) return;
+  const BatchSlot& sl = a.slot[s];
+  double* st = sl.state;
+  if (st[kDone] != 0.0) return;
+  if (st[kStepped] != 0.0) {
+    st[kTime] = st[kTime] + st[kDtUse];
+    st[kSteps] = st[kSteps] + 1.0;
+    st[kParity] = 1.0 - st[kParity];
+    st[kStepped] = 0.0;
+  }
+  if (st[kTime] >= st[kTend]) {
+    st[kDone] = 1.0;
+    return;
+  }
+  st[kDtUse] = fmin_f(st[kDtNext], st[kTend] - st[kTime]);
+  const double boxlen = sl.n * sl.dx;
+  st[kDtNext] = fmin_f(boxlen / a.hp.smallc, a.hp.courant_factor * sl.dx / a.hp.smallc);
+  st[kStepped] = 1.0;
 }
 
 // courant_fine over a uniform level without gravity: max of the cmpdt wave
@@ -251,6 +338,17 @@ __global__ void courant_kernel(const double* u, long ncell, double dx, HydroPara
 }
 
 __global__ void set_double(double* p, double v) { *p = v; }
+
+__global__ void init_slot_kernel(double* st, double t_end, double start) {
+  st[kTime] = 0.0;
+  st[kTend] = t_end;
+  st[kDtUse] = 0.0;
+  st[kDtNext] = start;
+  st[kDone] = 0.0;
+  st[kSteps] = 0.0;
+  st[kParity] = 0.0;
+  st[kStepped] = 0.0;
+}
 
 __global__ void cmpdt_group(const double* uu, const double* gg, int nv, int ncell, double dx, HydroParams hp,
                             double* d_dt) {
@@ -283,7 +381,53 @@ hipError_t launch_tile(const UniformArgs& a, int TZ, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <typename R, int RS>
+void launch_batch_t(const BatchArgs& a, int nblocks, hipStream_t s) {
+  constexpr int TX = 16, TY = 8, NT = ((TX + 2) * (TY + 2) + 63) / 64 * 64;
+  hipLaunchKernelGGL((batch_godunov_kernel<R, TX, TY, RS>), dim3(nblocks), dim3(NT), 0, s, a);
+}
+
 }  // namespace
+
+// Continuous batching: one bookkeeping launch and one update launch advance
+// every slot of `slots` by one step of its own dt.
+hipError_t launch_level_batch(double* const* buf0, double* const* buf1, double* const* state, const int* n,
+                              const double* dx, int nslots, const HydroParams& hp, bool fp32, hipStream_t s) {
+  BatchArgs a{};
+  a.nslots = nslots;
+  a.hp = hp;
+  int blocks = 0;
+  for (int i = 0; i < nslots; ++i) {
+    BatchSlot& sl = a.slot[i];
+    sl.buf[0] = buf0[i];
+    sl.buf[1] = buf1[i];
+    sl.state = state[i];
+    sl.n = n[i];
+    sl.dx = dx[i];
+    sl.tz = batch_tz(n[i]);
+    a.blk0[i] = blocks;
+    blocks += (n[i] / 16) * (n[i] / 8) * (n[i] / sl.tz);
+  }
+  a.blk0[nslots] = blocks;
+  hipLaunchKernelGGL(batch_advance_kernel, dim3(1), dim3(64), 0, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || blocks == 0) return e;
+  if (fp32) {
+    if (hp.riemann == kHLLC) launch_batch_t<float, kHLLC>(a, blocks, s);
+    else if (hp.riemann == kHLL) launch_batch_t<float, kHLL>(a, blocks, s);
+    else launch_batch_t<float, kLLF>(a, blocks, s);
+  } else {
+    if (hp.riemann == kHLLC) launch_batch_t<double, kHLLC>(a, blocks, s);
+    else if (hp.riemann == kHLL) launch_batch_t<double, kHLL>(a, blocks, s);
+    else launch_batch_t<double, kLLF>(a, blocks, s);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_init_slot(double* state, double t_end, double start, hipStream_t s) {
+  hipLaunchKernelGGL(init_slot_kernel, dim3(1), dim3(1), 0, s, state, t_end, start);
+  return hipGetLastError();
+}
 
 hipError_t launch_cmpdt(const double* uu, const double* gg, int nv, int ncell, double dx, const HydroParams& hp,
                         double* d_dt, hipStream_t stream) {
