@@ -152,6 +152,62 @@ int ramses_level_slot_init(const ramses_level_slot *slot, double t_end, void *st
 int ramses_level_batch_step(const ramses_level_slot *slots, int nslots, void *stream);
 
 /* ---------------------------------------------------------------------
+ * Oct-tree (AMR) levels.  The tree and the state stay in device memory in
+ * the reference's own memory model (amr/amr_commons.f90:54-75,
+ * hydro/hydro_commons.f90:4): 1-based cell index
+ * icell = ncoarse + (ind-1)*ngridmax + igrid, arrays addressed so that
+ * element i of the reference is p[i] (element 0 unused), state
+ * u[var*(ncell+1) + icell].  nx*ny*nz coarse cells, periodic.
+ * --------------------------------------------------------------------- */
+typedef struct ramses_amr_tree {
+  const int *son;    /* son[icell], icell = 1..ncoarse+8*ngridmax */
+  const int *father; /* father[igrid], igrid = 1..ngridmax */
+  const int *nbor;   /* nbor[(j-1)*ngridmax + igrid], j = 1..6 (-x,+x,-y,+y,-z,+z) */
+  int ncoarse, ngridmax, nx, ny, nz;
+} ramses_amr_tree;
+
+typedef struct ramses_amr_level {
+  int ilevel;
+  double dx;                /* 0.5^ilevel * boxlen/nx */
+  const int *active;        /* active(ilevel)%igrid, device, noct entries */
+  int noct;
+  const int *pos;           /* pos[igrid] = index of igrid in `active` (read for this level's grids) */
+  const int *active_coarse; /* grids of level ilevel-1 (reflux targets); NULL at the coarsest level */
+  int noct_coarse;
+} ramses_amr_level;
+
+/* interpol_type (0-3) and interpol_var (0-2) of the buffer-oct
+ * prolongation, hydro/hydro_parameters.f90:88-89 (defaults 1 and 0). */
+int ramses_amr_config(int interpol_type, int interpol_var);
+
+/* Device workspace the godunov sweep of a level with `noct` octs needs
+ * (its nvector batches of stencils, flags and fluxes). */
+unsigned long ramses_amr_work_bytes(int noct);
+
+/* godunov_fine(ilevel) (hydro/godunov_fine.f90:5-35): godfine1 over every
+ * oct of the level -- 6x6x6 gather through son/nbor with interpolated
+ * buffer octs, unsplit, flux reset at refined faces, unew update at the
+ * level and conservative reflux into coarser leaf neighbours -- with dt
+ * read from device memory *d_dt.  *d_error becomes nonzero if an oct's
+ * 27 father neighbours do not all exist (the refinement rule is broken). */
+int ramses_amr_godunov(const ramses_amr_tree *t, const ramses_amr_level *lv, const double *uold, double *unew,
+                       const double *d_dt, void *work, unsigned long work_bytes, int *d_error, void *stream);
+
+/* set_unew / set_uold of a level (hydro/godunov_fine.f90:40,135): copy the
+ * state of the level's active cells from src to dst. */
+int ramses_amr_copy(const ramses_amr_tree *t, const ramses_amr_level *lv, const double *src, double *dst,
+                    void *stream);
+
+/* upload_fine(ilevel) (hydro/interpol_hydro.f90:5-263): refined cells of
+ * the level take the average of their children. */
+int ramses_amr_upload(const ramses_amr_tree *t, const ramses_amr_level *lv, double *u, void *stream);
+
+/* courant_fine(ilevel) (hydro/courant_fine.f90:1-159) over the level's leaf
+ * cells: *d_dt = min(dt_start, courant_factor*dx/smallc, min cmpdt). */
+int ramses_amr_courant(const ramses_amr_tree *t, const ramses_amr_level *lv, const double *u, double dt_start,
+                       double *d_dt, void *stream);
+
+/* ---------------------------------------------------------------------
  * Launch-shape tuning.  The reference's only performance knob is the
  * compile-time vector length nvector (amr/amr_parameters.f90:49-53, set by
  * bin/Makefile:11); on the GPU the equivalent choices are the workgroup and

@@ -147,7 +147,9 @@ class Reference:
     def __init__(self, path=REF_LIB):
         if not os.path.exists(path):
             raise FileNotFoundError(f"{path} is not built (make -C oracle ref)")
-        lib = ctypes.CDLL(path)
+        # lazy binding: the AMR routines reference clean_stop (amr/end.f90),
+        # called only on corrupt trees and not part of this build
+        lib = ctypes.CDLL(path, mode=os.RTLD_LAZY)
         lib.ref_set_hydro_params.argtypes = [ctypes.c_double] * 3 + [ctypes.c_int, ctypes.c_double,
                                                                        ctypes.c_int, ctypes.c_double,
                                                                        ctypes.c_double]
@@ -176,6 +178,67 @@ class Reference:
         g = np.zeros((NDIM, self.nvector)) if gg is None else gg
         self.lib.ref_cmpdt(_ptr(uu), _ptr(g), dx, ctypes.byref(dt), ncell)
         return dt.value
+
+
+class ReferenceAMR:
+    """The reference's own level routines (godunov_fine, set_unew, set_uold,
+    courant_fine, upload_fine) on an oct tree installed from numpy arrays
+    (ref_amr_glue.f90)."""
+
+    def __init__(self, ref):
+        self.ref = ref
+        lib = ref.lib
+        ip = ctypes.POINTER(ctypes.c_int)
+        lib.ref_amr_init.argtypes = [ctypes.c_int, ctypes.c_int]
+        lib.ref_amr_set_tree.argtypes = [ip, ip, ip]
+        lib.ref_amr_set_level.argtypes = [ctypes.c_int, ctypes.c_int, ip]
+        lib.ref_amr_set_interpol.argtypes = [ctypes.c_int, ctypes.c_int]
+        lib.ref_amr_put_u.argtypes = [ctypes.c_int, _dp]
+        lib.ref_amr_get_u.argtypes = [ctypes.c_int, _dp]
+        lib.ref_amr_set_unew.argtypes = [ctypes.c_int]
+        lib.ref_amr_set_uold.argtypes = [ctypes.c_int]
+        lib.ref_amr_upload.argtypes = [ctypes.c_int]
+        lib.ref_amr_godunov.argtypes = [ctypes.c_int, ctypes.c_double]
+        lib.ref_amr_courant.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, _dp]
+        self.lib = lib
+
+    def install(self, tree, nlevelmax=None):
+        ip = ctypes.POINTER(ctypes.c_int)
+        self.tree = tree
+        self.lib.ref_amr_init(nlevelmax or tree.nlevel, tree.ngridmax)
+        self._keep = [np.ascontiguousarray(a, dtype=np.int32) for a in (tree.son, tree.father, tree.nbor)]
+        self.lib.ref_amr_set_tree(*[a.ctypes.data_as(ip) for a in self._keep])
+        for l, ids in tree.active.items():
+            a = np.ascontiguousarray(ids, dtype=np.int32)
+            self.lib.ref_amr_set_level(l, len(a), a.ctypes.data_as(ip))
+
+    def set_interpol(self, interpol_type=1, interpol_var=0):
+        self.lib.ref_amr_set_interpol(interpol_type, interpol_var)
+
+    def put(self, which, u):
+        self.lib.ref_amr_put_u(which, _ptr(np.ascontiguousarray(u, dtype=np.float64)))
+
+    def get(self, which):
+        out = np.zeros((NVAR, self.tree.ncell))
+        self.lib.ref_amr_get_u(which, _ptr(out))
+        return out
+
+    def set_unew(self, l):
+        self.lib.ref_amr_set_unew(l)
+
+    def set_uold(self, l):
+        self.lib.ref_amr_set_uold(l)
+
+    def upload(self, l):
+        self.lib.ref_amr_upload(l)
+
+    def godunov(self, l, dt):
+        self.lib.ref_amr_godunov(l, dt)
+
+    def courant(self, l, dt_start):
+        v = [ctypes.c_double() for _ in range(4)]
+        self.lib.ref_amr_courant(l, dt_start, *[ctypes.byref(x) for x in v])
+        return tuple(x.value for x in v)
 
 
 def reference_available():

@@ -1,0 +1,468 @@
+// One level of an oct-tree (AMR) godunov sweep on the GPU:
+// godfine1 (hydro/godunov_fine.f90:486-911) split into four launches over
+// the level's active octs, all reading the tree (son / father / nbor) and
+// the state (uold / unew) resident in HBM in the reference's memory model:
+//   amr_gather   6x6x6 stencil of every oct in nvector-strided batches
+//                (get3cubefather/get3cubepos, amr/nbors_utils.f90:5-300; a
+//                missing neighbour oct is a buffer oct interpolated from its
+//                father's 7-point neighbourhood: getnborfather :404-525 and
+//                interpol_hydro, hydro/interpol_hydro.f90:268-637);
+//   unsplit      the batched kernel of unsplit.hip on those batches;
+//   amr_update   flux reset at refined faces and the level update (:720-792);
+//   amr_reflux   the conservative correction of coarser leaf neighbours
+//                (:798-908), gathered per coarse cell and summed in the
+//                reference's order (batch, dimension, side, face) so the
+//                result is bitwise and deterministic without atomics.
+// Index conventions are the reference's (1-based cells and grids; arrays
+// are passed so that element i of the reference is p[i]).
+#include "hydro_device.hpp"
+#include "unsplit_args.hpp"
+#include "amr_args.hpp"
+
+namespace ramses {
+
+namespace {
+
+__device__ __forceinline__ long cell_of(const AmrTreeView& t, int ind, int igrid) {
+  return (long)t.ncoarse + (long)(ind - 1) * t.ngridmax + igrid;
+}
+__device__ __forceinline__ int nbor_of(const AmrTreeView& t, int igrid, int j) {
+  return t.nbor[(long)(j - 1) * t.ngridmax + igrid];
+}
+
+// Neighbour father cell (i1,j1,k1 in 0..2) of father cell f of a level>1 oct:
+// the get3cubepos chain (z hop, then y, then x) over son(nbor(...)).
+__device__ int nbor_father_cell(const AmrTreeView& t, long f, int i1, int j1, int k1, int ilevel) {
+  if (ilevel == 1) {  // coarse grid: periodic index arithmetic (:28-140)
+    const int nxny = t.nx * t.ny;
+    const int iz = (int)((f - 1) / nxny), iy = (int)((f - 1 - (long)iz * nxny) / t.nx);
+    const int ix = (int)(f - 1 - (long)iy * t.nx - (long)iz * nxny);
+    int iix = ix + i1 - 1, iiy = iy + j1 - 1, iiz = iz + k1 - 1;
+    if (iix < 0) iix = t.nx - 1;
+    if (iix > t.nx - 1) iix = 0;
+    if (iiy < 0) iiy = t.ny - 1;
+    if (iiy > t.ny - 1) iiy = 0;
+    if (iiz < 0) iiz = t.nz - 1;
+    if (iiz > t.nz - 1) iiz = 0;
+    return 1 + iix + iiy * t.nx + iiz * nxny;
+  }
+  const int pos = (int)((f - t.ncoarse - 1) / t.ngridmax) + 1;
+  const int g = (int)(f - t.ncoarse - (long)(pos - 1) * t.ngridmax);
+  const int px = (pos - 1) & 1, py = ((pos - 1) >> 1) & 1, pz = (pos - 1) >> 2;
+  const int cx = px + i1 - 1, cy = py + j1 - 1, cz = pz + k1 - 1;
+  int gg = g;
+  if (cz != pz && (cz < 0 || cz > 1) && gg > 0) gg = t.son[nbor_of(t, gg, cz < 0 ? 5 : 6)];
+  if (cy != py && (cy < 0 || cy > 1) && gg > 0) gg = t.son[nbor_of(t, gg, cy < 0 ? 3 : 4)];
+  if (cx != px && (cx < 0 || cx > 1) && gg > 0) gg = t.son[nbor_of(t, gg, cx < 0 ? 1 : 2)];
+  if (gg <= 0) return 0;
+  const int ind = 1 + (cx & 1) + 2 * (cy & 1) + 4 * (cz & 1);
+  return (int)cell_of(t, ind, gg);
+}
+
+// The 2*ndim+1 cells used to interpolate a buffer oct in father cell f
+// (getnborfather, amr/nbors_utils.f90:404-525): f itself and its six
+// neighbours at its own level, or the coarser neighbour of f's grid where a
+// neighbour does not exist.
+__device__ void nbor_father7(const AmrTreeView& t, long f, int ilevel, long out[7]) {
+  out[0] = f;
+  if (ilevel == 1) {
+    const int nxny = t.nx * t.ny;
+    const int ix3 = (int)((f - 1) / nxny);
+    const int ix2 = (int)((f - 1 - (long)ix3 * nxny) / t.nx);
+    const int ix1 = (int)(f - 1 - (long)ix2 * t.nx - (long)ix3 * nxny);
+    const int ixs[3] = {ix1, ix2, ix3};
+    const int ib[3] = {t.nx - 1, t.ny - 1, t.nz - 1};
+    const int s1[3] = {1, t.nx, nxny};
+    const int s2[3] = {t.nx - 1, (t.ny - 1) * t.nx, (t.nz - 1) * nxny};
+    for (int d = 0; d < 3; ++d) {
+      out[2 * d + 1] = ixs[d] > 0 ? f - s1[d] : f + s2[d];
+      out[2 * d + 2] = ixs[d] < ib[d] ? f + s1[d] : f - s2[d];
+    }
+    return;
+  }
+  const int pos = (int)((f - t.ncoarse - 1) / t.ngridmax) + 1;
+  const int g = (int)(f - t.ncoarse - (long)(pos - 1) * t.ngridmax);
+  const int p[3] = {(pos - 1) & 1, ((pos - 1) >> 1) & 1, (pos - 1) >> 2};
+  for (int in = 1; in <= 6; ++in) {
+    const int a = (in - 1) >> 1, plus = (in - 1) & 1;
+    // getnborcells: the neighbour stays in the grid when it moves inward
+    const bool inside = plus ? (p[a] == 0) : (p[a] == 1);
+    int q[3] = {p[0], p[1], p[2]};
+    q[a] = 1 - p[a];
+    const int ind = 1 + q[0] + 2 * q[1] + 4 * q[2];
+    const int gn = inside ? g : t.son[nbor_of(t, g, in)];
+    out[in] = gn > 0 ? cell_of(t, ind, gn) : nbor_of(t, g, in);
+  }
+}
+
+// Slopes for the prolongation (compute_limiter_minmod / _central /
+// compute_central, hydro/interpol_hydro.f90:449-637), a[0..6] = centre, -x,
+// +x, -y, +y, -z, +z.
+__device__ void interp_slopes(const double a[7], int itype, double w[3]) {
+  if (itype == 1) {
+    for (int d = 0; d < 3; ++d) {
+      const double dl = 0.5 * (a[2 * d + 2] - a[0]);
+      const double dr = 0.5 * (a[0] - a[2 * d + 1]);
+      if (dl * dr <= 0.0)
+        w[d] = 0.0;
+      else
+        w[d] = fmin_f(__builtin_fabs(dl), __builtin_fabs(dr)) * dl / __builtin_fabs(dl);
+    }
+    return;
+  }
+  for (int d = 0; d < 3; ++d) w[d] = 0.25 * (a[2 * d + 2] - a[2 * d + 1]);
+  if (itype == 3) return;
+  if (itype != 2) {
+    w[0] = w[1] = w[2] = 0.0;
+    return;
+  }
+  // monotonized central: limit by the extrema of the kernel and the corners
+  double ac[8];
+  for (int ind = 0; ind < 8; ++ind) ac[ind] = a[0];
+  for (int d = 0; d < 3; ++d)
+    for (int ind = 0; ind < 8; ++ind) {
+      const double xxc = (double)((ind >> d) & 1) - 0.5;
+      ac[ind] = ac[ind] + 2.0 * w[d] * xxc;
+    }
+  double corner = ac[0], kernel = a[1];
+  for (int j = 1; j < 8; ++j) corner = fmax_f(corner, ac[j]);
+  for (int j = 2; j <= 6; ++j) kernel = fmax_f(kernel, a[j]);
+  double dk = a[0] - kernel, dc = a[0] - corner;
+  double maxl = 0.0;
+  if (dk * dc > 0.0) maxl = fmin_f(1.0, dk / dc);
+  corner = ac[0];
+  kernel = a[1];
+  for (int j = 1; j < 8; ++j) corner = fmin_f(corner, ac[j]);
+  for (int j = 2; j <= 6; ++j) kernel = fmin_f(kernel, a[j]);
+  dk = a[0] - kernel;
+  dc = a[0] - corner;
+  double minl = 0.0;
+  if (dk * dc > 0.0) minl = fmin_f(1.0, dk / dc);
+  const double lim = fmin_f(minl, maxl);
+  for (int d = 0; d < 3; ++d) w[d] = w[d] * lim;
+}
+
+// interpol_hydro for one buffer oct: u1[7][5] -> u2[8][5]
+// (hydro/interpol_hydro.f90:268-444; interpol_var 0, 1, 2; types 0-3)
+__device__ void interpol_oct(double u1[7][5], double u2[8][5], int itype, int ivar_mode, double smallr) {
+  if (ivar_mode == 1 || ivar_mode == 2) {
+    for (int j = 0; j < 7; ++j) {
+      double ekin = 0.0;
+      for (int d = 0; d < 3; ++d) ekin = ekin + 0.5 * (u1[j][d + 1] * u1[j][d + 1]) / fmax_f(u1[j][0], smallr);
+      const double erad = 0.0;
+      u1[j][4] = u1[j][4] - ekin - erad;
+      if (ivar_mode == 2)
+        for (int d = 0; d < 3; ++d) u1[j][d + 1] = u1[j][d + 1] / fmax_f(u1[j][0], smallr);
+    }
+  }
+  for (int v = 0; v < 5; ++v) {
+    double a[7], w[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < 7; ++j) a[j] = u1[j][v];
+    if (itype >= 1 && itype <= 3) interp_slopes(a, itype, w);
+    for (int ind = 0; ind < 8; ++ind) {
+      double s = a[0];
+      for (int d = 0; d < 3; ++d) s = s + w[d] * ((double)((ind >> d) & 1) - 0.5);
+      u2[ind][v] = s;
+    }
+  }
+  if (ivar_mode == 1 || ivar_mode == 2) {
+    if (ivar_mode == 2) {
+      for (int ind = 0; ind < 8; ++ind)
+        for (int d = 0; d < 3; ++d) u2[ind][d + 1] = u2[ind][d + 1] * u2[ind][0];
+      for (int d = 0; d < 3; ++d) {
+        double mom = 0.0;
+        for (int ind = 0; ind < 8; ++ind) mom = mom + u2[ind][d + 1] * (1.0 / 8.0);
+        mom = mom - u1[0][d + 1] * u1[0][0];
+        for (int ind = 0; ind < 8; ++ind) u2[ind][d + 1] = u2[ind][d + 1] - mom;
+      }
+    }
+    for (int ind = 0; ind < 8; ++ind) {
+      double ekin = 0.0;
+      for (int d = 0; d < 3; ++d)
+        ekin = ekin + 0.5 * (u2[ind][d + 1] * u2[ind][d + 1]) / fmax_f(u2[ind][0], smallr);
+      const double erad = 0.0;
+      u2[ind][4] = u2[ind][4] + ekin + erad;
+    }
+  }
+}
+
+// Thread = (batch, father neighbour j in 0..26, lane); workgroup = one batch.
+__global__ __launch_bounds__(1024) void amr_gather(AmrLevelArgs a) {
+  const int nv = a.nv;
+  const int lane = threadIdx.x % nv;
+  const int j = threadIdx.x / nv;
+  if (j >= 27) return;
+  const int b = blockIdx.x;
+  const int oct = b * nv + lane;
+  if (oct >= a.noct) return;
+  const AmrTreeView& t = a.t;
+  const int igrid = a.active[oct];
+  const long f = t.father[igrid];
+  const int i1 = j % 3, j1 = (j / 3) % 3, k1 = j / 9;
+  const long nf = nbor_father_cell(t, f, i1, j1, k1, a.ilevel);
+  if (nf <= 0) {
+    atomicOr(a.error, 1);
+    return;
+  }
+  const int igrid_n = t.son[nf];
+  const size_t us = (size_t)nv * 216;               // variable stride in uin
+  double* uin = a.uin + (size_t)b * us * 5 + lane;
+  unsigned char* ok = a.ok + (size_t)b * us + lane;
+  if (igrid_n > 0) {
+    for (int ind = 1; ind <= 8; ++ind) {
+      const int i2 = (ind - 1) & 1, j2 = ((ind - 1) >> 1) & 1, k2 = (ind - 1) >> 2;
+      const int i3 = 1 + 2 * (i1 - 1) + i2, j3 = 1 + 2 * (j1 - 1) + j2, k3 = 1 + 2 * (k1 - 1) + k2;
+      const size_t s = (size_t)((i3 + 1) + 6 * ((j3 + 1) + 6 * (k3 + 1))) * nv;
+      const long c = cell_of(t, ind, igrid_n);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) uin[s + (size_t)v * us] = a.uold[(long)v * a.ustride + c];
+      ok[s] = t.son[c] > 0 ? 1 : 0;
+    }
+  } else {
+    long nb7[7];
+    nbor_father7(t, nf, a.ilevel, nb7);
+    double u1[7][5], u2[8][5];
+    for (int q = 0; q < 7; ++q)
+#pragma unroll
+      for (int v = 0; v < 5; ++v) u1[q][v] = a.uold[(long)v * a.ustride + nb7[q]];
+    interpol_oct(u1, u2, a.interpol_type, a.interpol_var, a.hp.smallr);
+    for (int ind = 1; ind <= 8; ++ind) {
+      const int i2 = (ind - 1) & 1, j2 = ((ind - 1) >> 1) & 1, k2 = (ind - 1) >> 2;
+      const int i3 = 1 + 2 * (i1 - 1) + i2, j3 = 1 + 2 * (j1 - 1) + j2, k3 = 1 + 2 * (k1 - 1) + k2;
+      const size_t s = (size_t)((i3 + 1) + 6 * ((j3 + 1) + 6 * (k3 + 1))) * nv;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) uin[s + (size_t)v * us] = u2[ind - 1][v];
+      ok[s] = 0;
+    }
+  }
+}
+
+__device__ __forceinline__ size_t fslot(int i3, int j3, int k3) { return (size_t)((i3 - 1) + 3 * ((j3 - 1) + 3 * (k3 - 1))); }
+__device__ __forceinline__ size_t sslot(int i3, int j3, int k3) {
+  return (size_t)((i3 + 1) + 6 * ((j3 + 1) + 6 * (k3 + 1)));
+}
+
+// Flux reset at faces touching a refined cell, then the level update
+// unew += flux(i3) - flux(i3+1) per dimension.  Thread = one oct.
+__global__ void amr_update(AmrLevelArgs a) {
+  const int oct = blockIdx.x * blockDim.x + threadIdx.x;
+  if (oct >= a.noct) return;
+  const int nv = a.nv;
+  const int b = oct / nv, lane = oct % nv;
+  const AmrTreeView& t = a.t;
+  const int igrid = a.active[oct];
+  const size_t fvs = (size_t)nv * 27;
+  double* flux = a.flux + (size_t)b * fvs * 15 + lane;
+  const unsigned char* ok = a.ok + (size_t)b * nv * 216 + lane;
+  for (int d = 0; d < 3; ++d) {
+    const int i0 = d == 0, j0 = d == 1, k0 = d == 2;
+    for (int k3 = 1; k3 <= 2 + k0; ++k3)
+      for (int j3 = 1; j3 <= 2 + j0; ++j3)
+        for (int i3 = 1; i3 <= 2 + i0; ++i3) {
+          if (ok[sslot(i3 - i0, j3 - j0, k3 - k0) * nv] || ok[sslot(i3, j3, k3) * nv]) {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) flux[(fslot(i3, j3, k3) + (size_t)27 * (v + 5 * d)) * nv] = 0.0;
+          }
+        }
+  }
+  for (int d = 0; d < 3; ++d) {
+    const int i0 = d == 0, j0 = d == 1, k0 = d == 2;
+    for (int ind = 1; ind <= 8; ++ind) {
+      const int i2 = (ind - 1) & 1, j2 = ((ind - 1) >> 1) & 1, k2 = (ind - 1) >> 2;
+      const int i3 = 1 + i2, j3 = 1 + j2, k3 = 1 + k2;
+      const long c = cell_of(t, ind, igrid);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        const size_t base = (size_t)27 * (v + 5 * d);
+        const double fl = flux[(fslot(i3, j3, k3) + base) * nv];
+        const double fr = flux[(fslot(i3 + i0, j3 + j0, k3 + k0) + base) * nv];
+        a.unew[(long)v * a.ustride + c] = a.unew[(long)v * a.ustride + c] + (fl - fr);
+      }
+    }
+  }
+}
+
+// Conservative correction of the coarser leaf cells next to the level's
+// octs.  Thread = one cell of a coarse-level oct.  A coarse leaf c receives
+// from each fine oct o that sits across one of its faces the four (reset)
+// fluxes of o's boundary faces on c's side, times -1/8 (c left of o) or
+// +1/8 (c right of o).  The reference adds them in loop order: batch of o,
+// then dimension, then side, then face (k3, j3, i3); the same order is
+// rebuilt here from the position of o in the level's active list.
+__global__ void amr_reflux(AmrLevelArgs a) {
+  const long tix = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tix >= 8L * a.noct_coarse) return;
+  const AmrTreeView& t = a.t;
+  const int g = a.active_coarse[tix >> 3];
+  const int ind = (int)(tix & 7) + 1;
+  const long c = cell_of(t, ind, g);
+  if (t.son[c] != 0) return;  // only leaves receive the correction (:811,863)
+  const int p[3] = {(ind - 1) & 1, ((ind - 1) >> 1) & 1, (ind - 1) >> 2};
+  // up to 6 contributing octs: key = batch*6 + 2*d + side
+  int keys[6], octs[6], ne = 0;
+  for (int in = 1; in <= 6; ++in) {
+    const int ax = (in - 1) >> 1, plus = (in - 1) & 1;
+    const bool inside = plus ? (p[ax] == 0) : (p[ax] == 1);
+    int q[3] = {p[0], p[1], p[2]};
+    q[ax] = 1 - p[ax];
+    const int gn = inside ? g : t.son[nbor_of(t, g, in)];
+    if (gn <= 0) continue;
+    const long n = cell_of(t, 1 + q[0] + 2 * q[1] + 4 * q[2], gn);
+    const int o = t.son[n];
+    if (o <= 0) continue;
+    // c is on o's minus side when n is c's plus neighbour
+    const int side = plus ? 0 : 1;
+    const int position = a.pos[o];
+    keys[ne] = (position / a.nv) * 6 + 2 * ax + side;
+    octs[ne] = o;
+    ++ne;
+  }
+  if (ne == 0) return;
+  for (int i = 1; i < ne; ++i)  // insertion sort, at most 6 entries
+    for (int k = i; k > 0 && keys[k - 1] > keys[k]; --k) {
+      const int tk = keys[k]; keys[k] = keys[k - 1]; keys[k - 1] = tk;
+      const int to = octs[k]; octs[k] = octs[k - 1]; octs[k - 1] = to;
+    }
+  const int nv = a.nv;
+  const size_t fvs = (size_t)nv * 27;
+  const double oneontwotondim = 1.0 / 8.0;
+  for (int v = 0; v < 5; ++v) {
+    double s = a.unew[(long)v * a.ustride + c];
+    for (int e = 0; e < ne; ++e) {
+      const int d = (keys[e] % 6) >> 1, side = keys[e] & 1;
+      const int position = a.pos[octs[e]];
+      const int b = position / nv, lane = position % nv;
+      const double* flux = a.flux + (size_t)b * fvs * 15 + lane + (size_t)27 * (v + 5 * d) * nv;
+      const int i0 = d == 0, j0 = d == 1, k0 = d == 2;
+      if (side == 0) {
+        for (int k3 = 1; k3 <= 2 - k0; ++k3)
+          for (int j3 = 1; j3 <= 2 - j0; ++j3)
+            for (int i3 = 1; i3 <= 2 - i0; ++i3) s = s - flux[fslot(i3, j3, k3) * nv] * oneontwotondim;
+      } else {
+        for (int k3 = 1 + k0; k3 <= 2; ++k3)
+          for (int j3 = 1 + j0; j3 <= 2; ++j3)
+            for (int i3 = 1 + i0; i3 <= 2; ++i3) s = s + flux[fslot(i3 + i0, j3 + j0, k3 + k0) * nv] * oneontwotondim;
+      }
+    }
+    a.unew[(long)v * a.ustride + c] = s;
+  }
+}
+
+// set_unew / set_uold over the level's active cells (hydro/godunov_fine.f90:58-65, 193-197).
+__global__ void amr_copy_level(const int* active, int noct, AmrTreeView t, const double* src, double* dst,
+                               long ustride) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 8L * noct) return;
+  const long c = cell_of(t, (int)(i / noct) + 1, active[i % noct]);
+  for (int v = 0; v < 5; ++v) dst[(long)v * ustride + c] = src[(long)v * ustride + c];
+}
+
+// upload_fine: refined cells of the level get the average of their children
+// (hydro/interpol_hydro.f90:5-263: density with the smallr floor, sums over
+// the children in order, /8).
+__global__ void amr_upload(const int* active, int noct, AmrTreeView t, double* u, long ustride, double smallr) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 8L * noct) return;
+  const int igrid = active[i / 8];
+  const long c = cell_of(t, (int)(i % 8) + 1, igrid);
+  const int s = t.son[c];
+  if (s <= 0) return;
+  double getx = 0.0;
+  for (int ind = 1; ind <= 8; ++ind) getx = getx + fmax_f(u[cell_of(t, ind, s)], smallr);
+  double out[5];
+  out[0] = getx / 8.0;
+  for (int v = 1; v < 5; ++v) {
+    getx = 0.0;
+    for (int ind = 1; ind <= 8; ++ind) getx = getx + u[(long)v * ustride + cell_of(t, ind, s)];
+    out[v] = getx / 8.0;
+  }
+  for (int v = 0; v < 5; ++v) u[(long)v * ustride + c] = out[v];
+}
+
+// courant_fine over the level's leaf cells (hydro/courant_fine.f90:44-131):
+// block max of the cmpdt wave speed, block dt limit min-combined (exact,
+// see cmpdt_w).
+__global__ void amr_courant(const int* active, int noct, AmrTreeView t, const double* u, long ustride, double dx,
+                            HydroParams hp, double* d_dt) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double w = 0.0;
+  if (i < 8L * noct) {
+    const long c = cell_of(t, (int)(i % 8) + 1, active[i / 8]);
+    if (t.son[c] == 0)
+      w = cmpdt_w(u[c], u[ustride + c], u[2 * ustride + c], u[3 * ustride + c], u[4 * ustride + c], hp);
+  }
+  for (int off = 32; off > 0; off >>= 1) w = fmax_f(w, __shfl_xor(w, off, 64));
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) m = fmax_f(m, red[k]);
+    if (m > 0.0) atomicMin(reinterpret_cast<unsigned long long*>(d_dt), dt_bits(cmpdt_from_w(m, dx, hp)));
+  }
+}
+
+}  // namespace
+
+hipError_t launch_unsplit(const UnsplitArgs& a, bool fp32, hipStream_t stream);
+
+// gather + unsplit + update + reflux of one level; dt is read from d_dt.
+hipError_t launch_amr_godunov(const AmrLevelArgs& a, double dx, const double* d_dt, double* tmp, bool fp32,
+                              hipStream_t s) {
+  if (a.noct <= 0) return hipSuccess;
+  const int nbatch = (a.noct + a.nv - 1) / a.nv;
+  hipLaunchKernelGGL(amr_gather, dim3(nbatch), dim3(27 * a.nv), 0, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  UnsplitArgs u;
+  u.uin = a.uin;
+  u.grav = nullptr;
+  u.flux = a.flux;
+  u.tmp = tmp;
+  u.dx[0] = u.dx[1] = u.dx[2] = dx;
+  u.dt = 0.0;
+  u.d_dt = d_dt;
+  u.ngroups = nbatch;
+  u.ngrid_last = a.noct - (nbatch - 1) * a.nv;
+  u.nv = a.nv;
+  u.hp = a.hp;
+  e = launch_unsplit(u, fp32, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(amr_update, dim3((a.noct + 127) / 128), dim3(128), 0, s, a);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (a.noct_coarse > 0 && a.active_coarse) {
+    const long n = 8L * a.noct_coarse;
+    hipLaunchKernelGGL(amr_reflux, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_amr_copy(const int* active, int noct, const AmrTreeView& t, const double* src, double* dst,
+                           long ustride, hipStream_t s) {
+  if (noct <= 0) return hipSuccess;
+  const long n = 8L * noct;
+  hipLaunchKernelGGL(amr_copy_level, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, active, noct, t, src, dst,
+                     ustride);
+  return hipGetLastError();
+}
+
+hipError_t launch_amr_upload(const int* active, int noct, const AmrTreeView& t, double* u, long ustride,
+                             double smallr, hipStream_t s) {
+  if (noct <= 0) return hipSuccess;
+  const long n = 8L * noct;
+  hipLaunchKernelGGL(amr_upload, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, active, noct, t, u, ustride,
+                     smallr);
+  return hipGetLastError();
+}
+
+hipError_t launch_amr_courant(const int* active, int noct, const AmrTreeView& t, const double* u, long ustride,
+                              double dx, const HydroParams& hp, double* d_dt, hipStream_t s) {
+  if (noct <= 0) return hipSuccess;
+  const long n = 8L * noct;
+  hipLaunchKernelGGL(amr_courant, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, active, noct, t, u, ustride,
+                     dx, hp, d_dt);
+  return hipGetLastError();
+}
+
+}  // namespace ramses

@@ -7,9 +7,11 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <cstdint>
 
 #include "../../include/ramses_hip.h"
 #include "hydro_device.hpp"
+#include "amr_args.hpp"
 
 namespace ramses {
 struct UnsplitArgs;
@@ -23,6 +25,14 @@ hipError_t launch_uniform_godunov(const double* u, double* unew, int n, double d
 hipError_t launch_level_batch(double* const* buf0, double* const* buf1, double* const* state, const int* n,
                               const double* dx, int nslots, const HydroParams& hp, bool fp32, hipStream_t s);
 hipError_t launch_init_slot(double* state, double t_end, double start, hipStream_t s);
+hipError_t launch_amr_godunov(const AmrLevelArgs& a, double dx, const double* d_dt, double* tmp, bool fp32,
+                              hipStream_t s);
+hipError_t launch_amr_copy(const int* active, int noct, const AmrTreeView& t, const double* src, double* dst,
+                           long ustride, hipStream_t s);
+hipError_t launch_amr_upload(const int* active, int noct, const AmrTreeView& t, double* u, long ustride,
+                             double smallr, hipStream_t s);
+hipError_t launch_amr_courant(const int* active, int noct, const AmrTreeView& t, const double* u, long ustride,
+                              double dx, const HydroParams& hp, double* d_dt, hipStream_t s);
 Tuning& tuning() {
   static Tuning t{0, 0, 0};
   return t;
@@ -30,6 +40,7 @@ Tuning& tuning() {
 }  // namespace ramses
 
 #include "unsplit_args.hpp"
+#include "amr_args.hpp"
 
 namespace {
 
@@ -138,6 +149,7 @@ int ramses_unsplit_device(const double* uin, const double* gravin, double* flux,
   a.dx[1] = dy;
   a.dx[2] = dz;
   a.dt = dt;
+  a.d_dt = nullptr;
   a.ngroups = ngroups;
   a.ngrid_last = ngrid_last;
   a.nv = c.nvector;
@@ -208,6 +220,125 @@ int ramses_cmpdt(const double* uu, const double* gg, double dx, int ncell, doubl
     return hip_fail(e, "cmpdt launch");
   if ((e = hipMemcpyAsync(dt, d_dt, 8, hipMemcpyDeviceToHost, s)) != hipSuccess) return hip_fail(e, "D2H dt");
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "cmpdt sync");
+  return 0;
+}
+
+namespace {
+int g_interpol_type = 1, g_interpol_var = 0;
+
+int amr_view(const ramses_amr_tree* t, ramses::AmrTreeView& v) {
+  if (!t || !t->son || !t->father || !t->nbor) return fail(-1, "null tree array");
+  if (t->ncoarse < 1 || t->ngridmax < 1 || t->nx < 1 || t->ny < 1 || t->nz < 1) return fail(-1, "bad tree sizes");
+  if (t->ncoarse != t->nx * t->ny * t->nz) return fail(-1, "ncoarse must be nx*ny*nz");
+  v.son = t->son;
+  v.father = t->father;
+  v.nbor = t->nbor;
+  v.ncoarse = t->ncoarse;
+  v.ngridmax = t->ngridmax;
+  v.nx = t->nx;
+  v.ny = t->ny;
+  v.nz = t->nz;
+  v.ncell = (long)t->ncoarse + 8L * t->ngridmax;
+  return 0;
+}
+
+size_t amr_work(int noct, int nv) {
+  const size_t nb = (size_t)((noct + nv - 1) / nv);
+  const size_t per = (size_t)nv * (216 * 5 + 27 * 15 + 27 * 6) * sizeof(double) + (size_t)nv * 216;
+  return nb * per + 256;
+}
+}  // namespace
+
+int ramses_amr_config(int interpol_type, int interpol_var) {
+  if (interpol_type < 0 || interpol_type > 3) return fail(-2, "interpol_type must be 0..3");
+  if (interpol_var < 0 || interpol_var > 2) return fail(-2, "interpol_var must be 0..2");
+  g_interpol_type = interpol_type;
+  g_interpol_var = interpol_var;
+  return 0;
+}
+
+unsigned long ramses_amr_work_bytes(int noct) { return (unsigned long)amr_work(noct < 0 ? 0 : noct, current().nvector); }
+
+int ramses_amr_godunov(const ramses_amr_tree* t, const ramses_amr_level* lv, const double* uold, double* unew,
+                       const double* d_dt, void* work, unsigned long work_bytes, int* d_error, void* stream) {
+  const Config c = current();
+  ramses::AmrTreeView v;
+  int rc = amr_view(t, v);
+  if (rc) return rc;
+  if (!lv || !uold || !unew || !d_dt || !d_error) return fail(-1, "null argument");
+  if (lv->noct < 0 || lv->noct > t->ngridmax) return fail(-1, "noct out of range");
+  if (lv->noct > 0 && (!lv->active || !lv->pos)) return fail(-1, "null active/pos");
+  if (lv->ilevel < 1) return fail(-1, "ilevel must be >= 1");
+  if (!(lv->dx > 0)) return fail(-1, "dx must be > 0");
+  if (c.hp.slope_type == 3) return fail(-4, "slope_type 3 is not supported on the AMR path");
+  if (c.nvector * 27 > 1024) return fail(-8, "nvector too large for the AMR gather (max 37)");
+  const size_t need = amr_work(lv->noct, c.nvector);
+  if (!work || work_bytes < need) return fail(-1, "work buffer too small (see ramses_amr_work_bytes)");
+  ramses::AmrLevelArgs a;
+  a.t = v;
+  a.uold = uold;
+  a.unew = unew;
+  a.ustride = v.ncell + 1;
+  a.active = lv->active;
+  a.noct = lv->noct;
+  a.ilevel = lv->ilevel;
+  a.pos = lv->pos;
+  const int nv = c.nvector;
+  const size_t nb = (size_t)((lv->noct + nv - 1) / nv);
+  char* w = (char*)(((uintptr_t)work + 255) & ~(uintptr_t)255);
+  a.uin = (double*)w;
+  a.flux = a.uin + nb * nv * 216 * 5;
+  double* tmp = a.flux + nb * nv * 27 * 15;
+  a.ok = (unsigned char*)(tmp + nb * nv * 27 * 6);
+  a.nv = nv;
+  a.interpol_type = g_interpol_type;
+  a.interpol_var = g_interpol_var;
+  a.hp = c.hp;
+  a.active_coarse = lv->active_coarse;
+  a.noct_coarse = lv->active_coarse ? lv->noct_coarse : 0;
+  a.error = d_error;
+  hipError_t e = ramses::launch_amr_godunov(a, lv->dx, d_dt, tmp, c.precision == RAMSES_FP32_MIXED,
+                                            (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "amr godunov launch");
+  return 0;
+}
+
+int ramses_amr_copy(const ramses_amr_tree* t, const ramses_amr_level* lv, const double* src, double* dst,
+                    void* stream) {
+  ramses::AmrTreeView v;
+  int rc = amr_view(t, v);
+  if (rc) return rc;
+  if (!lv || !src || !dst || (lv->noct > 0 && !lv->active)) return fail(-1, "null argument");
+  hipError_t e = ramses::launch_amr_copy(lv->active, lv->noct, v, src, dst, v.ncell + 1, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "amr copy launch");
+  return 0;
+}
+
+int ramses_amr_upload(const ramses_amr_tree* t, const ramses_amr_level* lv, double* u, void* stream) {
+  const Config c = current();
+  ramses::AmrTreeView v;
+  int rc = amr_view(t, v);
+  if (rc) return rc;
+  if (!lv || !u || (lv->noct > 0 && !lv->active)) return fail(-1, "null argument");
+  hipError_t e = ramses::launch_amr_upload(lv->active, lv->noct, v, u, v.ncell + 1, c.hp.smallr, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "amr upload launch");
+  return 0;
+}
+
+int ramses_amr_courant(const ramses_amr_tree* t, const ramses_amr_level* lv, const double* u, double dt_start,
+                       double* d_dt, void* stream) {
+  const Config c = current();
+  ramses::AmrTreeView v;
+  int rc = amr_view(t, v);
+  if (rc) return rc;
+  if (!lv || !u || !d_dt || (lv->noct > 0 && !lv->active)) return fail(-1, "null argument");
+  if (!(lv->dx > 0)) return fail(-1, "dx must be > 0");
+  double start = c.hp.courant_factor * lv->dx / c.hp.smallc;
+  if (dt_start < start) start = dt_start;
+  hipError_t e = ramses::launch_set_double(d_dt, start, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "dt init launch");
+  e = ramses::launch_amr_courant(lv->active, lv->noct, v, u, v.ncell + 1, lv->dx, c.hp, d_dt, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "amr courant launch");
   return 0;
 }
 

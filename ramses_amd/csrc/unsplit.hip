@@ -202,12 +202,13 @@ __global__ __launch_bounds__(768) void unsplit_rows(UnsplitArgs a) {
   const size_t gsz_t = (size_t)nv * 27 * 2 * 3;
 
   const HydroParams hp = a.hp;
+  const double dt = a.d_dt ? *a.d_dt : a.dt;
   RowCtx<R> c;
   c.u = a.uin + gidx * gsz_u + l;
   c.g = a.grav ? a.grav + gidx * gsz_g + l : nullptr;
   c.vs = (size_t)nv * 216;
   c.nv = nv;
-  c.dtxhalf = a.dt * 0.5;
+  c.dtxhalf = dt * 0.5;
   c.smalle = hp.smallc * hp.smallc / hp.gamma / (hp.gamma - 1.0);  // :884
   c.hp = &hp;
   const SolverConsts<R> sc = make_consts<R>(hp);
@@ -216,11 +217,11 @@ __global__ __launch_bounds__(768) void unsplit_rows(UnsplitArgs a) {
   double* tmp = a.tmp + gidx * gsz_t + l;
   const size_t fvs = (size_t)nv * 27;
   if (d == 0)
-    row_faces<R, 0, RS, S27>(c, sc, a1, b1, (R)(a.dt / a.dx[0]), flux, tmp, fvs, make_scale(a.dt, a.dx[0]));  // :516
+    row_faces<R, 0, RS, S27>(c, sc, a1, b1, (R)(dt / a.dx[0]), flux, tmp, fvs, make_scale(dt, a.dx[0]));  // :516
   else if (d == 1)
-    row_faces<R, 1, RS, S27>(c, sc, a1, b1, (R)(a.dt / a.dx[1]), flux, tmp, fvs, make_scale(a.dt, a.dx[1]));
+    row_faces<R, 1, RS, S27>(c, sc, a1, b1, (R)(dt / a.dx[1]), flux, tmp, fvs, make_scale(dt, a.dx[1]));
   else
-    row_faces<R, 2, RS, S27>(c, sc, a1, b1, (R)(a.dt / a.dx[2]), flux, tmp, fvs, make_scale(a.dt, a.dx[2]));
+    row_faces<R, 2, RS, S27>(c, sc, a1, b1, (R)(dt / a.dx[2]), flux, tmp, fvs, make_scale(dt, a.dx[2]));
 }
 
 }  // namespace

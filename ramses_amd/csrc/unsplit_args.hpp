@@ -13,6 +13,7 @@ struct UnsplitArgs {
   double* tmp;
   double dx[3];
   double dt;
+  const double* d_dt;  // if not null, dt is read from device memory
   long ngroups;
   int ngrid_last;
   int nv;

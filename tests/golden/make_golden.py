@@ -18,6 +18,11 @@ Families
                       CFL 0.8): conserved state after steps 1, 2 and 10 and
                       the dt sequence, composed from the reference unsplit
                       and cmpdt by the uniform-level driver below.
+  amr_sphere.npz      F4: an oct tree of levels 3-5 (tests/amr_cases.py)
+                      driven through the reference's own courant_fine,
+                      set_unew, godunov_fine and upload_fine
+                      (oracle/ref_amr_glue.f90): dt per level and the cells
+                      each routine changed.
 """
 import json
 import os
@@ -235,11 +240,50 @@ def sedov16(R):
     print("sedov16.npz: dt =", dts[:3], "...")
 
 
+def amr_sphere(R):
+    """F4: one oct tree (levels 3-5, refined ball + blob), smooth state; for
+    each level: the reference's courant_fine dt, then set_unew(l-1),
+    set_unew(l), godunov_fine(l) (changed cells only) and upload_fine(l)."""
+    sys.path.insert(0, os.path.dirname(HERE))
+    from amr_cases import smooth_state, sphere_tree
+    tree = sphere_tree()
+    u = smooth_state(tree)
+    A = oracle.ReferenceAMR(R)
+    A.install(tree)
+    A.set_interpol(1, 0)
+    R.set_params(gamma=5.0 / 3.0, riemann="hllc", slope_type=1, courant_factor=0.5)
+    out = {"u0": u}
+    for l in (3, 4, 5):
+        A.put(0, u)
+        A.put(1, u)
+        dt = A.courant(l, 1e10)[0]
+        if l > 3:
+            A.set_unew(l - 1)
+        A.set_unew(l)
+        A.godunov(l, dt)
+        un = A.get(1)
+        idx = np.nonzero((un != u).any(0))[0]
+        out[f"dt{l}"] = np.array(dt)
+        out[f"idx{l}"] = idx
+        out[f"val{l}"] = un[:, idx]
+        A.put(0, u)
+        A.upload(l)
+        uo = A.get(0)
+        uidx = np.nonzero((uo != u).any(0))[0]
+        out[f"upidx{l}"] = uidx
+        out[f"upval{l}"] = uo[:, uidx]
+    np.savez_compressed(os.path.join(HERE, "amr_sphere.npz"),
+                        meta=json.dumps(dict(gamma=5.0 / 3.0, riemann="hllc", levels=[3, 4, 5],
+                                             interpol_type=1, interpol_var=0)), **out)
+    print("amr_sphere.npz:", {l: len(out[f"idx{l}"]) for l in (3, 4, 5)})
+
+
 def main():
     R = oracle.Reference()
     unsplit_cases(R)
     cmpdt_cases(R)
     sedov16(R)
+    amr_sphere(R)
 
 
 if __name__ == "__main__":
