@@ -151,3 +151,74 @@ class AmrLevels:
     def check(self):
         if int(self.error.item()) != 0:
             raise RuntimeError("oct tree violates the refinement rule (missing father neighbours)")
+
+
+class AmrSolver(AmrLevels):
+    """amr_step (amr/amr_step.f90) on a static oct tree: per coarse step the
+    levels are advanced recursively with nsubcycle=2 time steps per level,
+    the fine levels first, exactly in the reference's order of operations:
+
+      newdt_fine(l) -> dt_l = min(courant_fine(l), dt_{l-1}/nsubcycle)  :327-330
+      set_unew(l)                                                       :334
+      amr_step(l+1, 1); amr_step(l+1, 2)                                :345-352
+      godunov_fine(l) with dt_l                                         :389
+      set_uold(l); upload_fine(l)                                       :424-442
+      dt_{l-1} = dtold_l + dt_l after the second substep               :573-576
+
+    The time steps stay on the device (min/sum of device scalars), so a
+    coarse step needs no host synchronisation.  Refinement is out of scope:
+    the tree is the one it was built with.
+    """
+
+    def __init__(self, tree, u, nsubcycle=2, **kw):
+        super().__init__(tree, u, **kw)
+        import torch
+        self.levelmin = tree.levelmin
+        self.nlevelmax = tree.nlevel
+        self.nsubcycle = nsubcycle
+        self.dtold = torch.zeros_like(self.dt)
+        self.time = torch.zeros(1, dtype=torch.float64, device=self.device)
+        self.fine_steps = 0
+
+    def noct(self, l):
+        ids = self.active.get(l)
+        return 0 if ids is None else len(ids)
+
+    def coarse_step(self):
+        self.amr_step(self.levelmin, 1)
+
+    def amr_step(self, l, icount):
+        import torch
+        if self.noct(l) == 0:
+            return
+        # newdt_fine (hydro/courant_fine.f90 via pm/newdt_fine.f90):
+        # dtold = dtnew, then dtnew = courant_fine starting from boxlen/smallc
+        from . import _config_cache
+        self.dtold[l] = self.dt[l]
+        self.courant(l, self.boxlen / _config_cache.get("smallc", 1e-10))
+        if l > self.levelmin:
+            self.dt[l] = torch.minimum(self.dt[l - 1] / float(self.nsubcycle), self.dt[l])
+        self.set_unew(l)
+        if l < self.nlevelmax and self.noct(l + 1) > 0:
+            self.amr_step(l + 1, 1)
+            if self.nsubcycle == 2:
+                self.amr_step(l + 1, 2)
+        else:
+            if l < self.nlevelmax:
+                self.dtold[l + 1] = self.dt[l] / float(self.nsubcycle)
+                self.dt[l + 1] = self.dt[l] / float(self.nsubcycle)
+            self.time += self.dt[l]        # update_time (amr/update_time.f90:217,331)
+            self.fine_steps += 1
+        self.godunov(l)
+        self.set_uold(l)
+        self.upload(l)
+        if l > self.levelmin:
+            if self.nsubcycle == 1:
+                self.dt[l - 1] = self.dt[l]
+            if icount == 2:
+                self.dt[l - 1] = self.dtold[l] + self.dt[l]
+
+    def cell_updates_per_coarse_step(self):
+        """sum over levels of 8*noct_l*nsubcycle^(l-levelmin)."""
+        return sum(8 * self.noct(l) * self.nsubcycle ** (l - self.levelmin)
+                   for l in range(self.levelmin, self.nlevelmax + 1))

@@ -29,3 +29,30 @@ def smooth_state(tree, gamma=5.0 / 3.0, seed=0, amp=0.3):
         p = 1.0 + amp * s[4] + 2.0 * np.exp(-((x - 0.5) ** 2).sum(1) / 0.01)
         return conserved(rho, vel, p, gamma)
     return tree.fill(fn)
+
+
+def reference_coarse_step(A, tree, levelmin, dt, dtold, nsub=2):
+    """The same amr_step sequence driving the reference's compiled routines
+    (host floats for the level time steps)."""
+    nlevelmax = tree.nlevel
+
+    def step(l, icount):
+        if len(tree.active.get(l, [])) == 0:
+            return
+        dtold[l] = dt[l]
+        dt[l] = A.courant(l, 1.0 / 1e-10)[0]
+        if l > levelmin:
+            dt[l] = min(dt[l - 1] / float(nsub), dt[l])
+        A.set_unew(l)
+        if l < nlevelmax and len(tree.active.get(l + 1, [])) > 0:
+            step(l + 1, 1)
+            step(l + 1, 2)
+        elif l < nlevelmax:
+            dtold[l + 1] = dt[l] / float(nsub)
+            dt[l + 1] = dt[l] / float(nsub)
+        A.godunov(l, dt[l])
+        A.set_uold(l)
+        A.upload(l)
+        if l > levelmin and icount == 2:
+            dt[l - 1] = dtold[l] + dt[l]
+    step(levelmin, 1)
