@@ -160,11 +160,11 @@ int ramses_level_batch_step(const ramses_level_slot *slots, int nslots, void *st
  * u[var*(ncell+1) + icell].  nx*ny*nz coarse cells, periodic.
  * --------------------------------------------------------------------- */
 typedef struct ramses_amr_tree {
-  const int *son;    /* son[icell], icell = 1..ncoarse+8*ngridmax */
-  const int *father; /* father[igrid], igrid = 1..ngridmax */
-  const int *nbor;   /* nbor[(j-1)*ngridmax + igrid], j = 1..6 (-x,+x,-y,+y,-z,+z) */
+  int *son;    /* son[icell], icell = 1..ncoarse+8*ngridmax */
+  int *father; /* father[igrid], igrid = 1..ngridmax */
+  int *nbor;   /* nbor[(j-1)*ngridmax + igrid], j = 1..6 (-x,+x,-y,+y,-z,+z) */
   int ncoarse, ngridmax, nx, ny, nz;
-} ramses_amr_tree;
+} ramses_amr_tree; /* only ramses_amr_refine_fine writes the arrays */
 
 typedef struct ramses_amr_level {
   int ilevel;
@@ -206,6 +206,62 @@ int ramses_amr_upload(const ramses_amr_tree *t, const ramses_amr_level *lv, doub
  * cells: *d_dt = min(dt_start, courant_factor*dx/smallc, min cmpdt). */
 int ramses_amr_courant(const ramses_amr_tree *t, const ramses_amr_level *lv, const double *u, double dt_start,
                        double *d_dt, void *stream);
+
+/* Refinement criteria: err_grad_d/u/p and floor_d/u/p
+ * (hydro/hydro_parameters.f90:47-52; a criterion is off when its err_grad
+ * is negative, the default -1) and nexpand (amr/amr_parameters.f90:85,
+ * default 1, the same for every level). */
+typedef struct ramses_refine_params {
+  double err_grad_d, err_grad_u, err_grad_p;
+  double floor_d, floor_u, floor_p;
+  int nexpand;
+} ramses_refine_params;
+
+/* Set the refinement criteria (hydro/hydro_parameters.f90:47-52,
+ * amr/amr_parameters.f90:85); NULL restores the defaults. */
+int ramses_refine_config(const ramses_refine_params *p);
+
+/* flag_fine(ilevel, icount) (amr/flag_utils.f90:57-106) for one level: the
+ * refinement map flag1 of the level's cells -- refined cells with refined
+ * or flagged children (test_flag), one smooth_fine buffer, the gradient
+ * criteria (hydro_flag, hydro/hydro_flag.f90:1-180; hydro_refine,
+ * hydro/godunov_utils.f90:125-263), nexpand more buffers, and for
+ * ilevel > levelmin with icount < 2 the refinement rule (ensure_ref_rules).
+ * flag1 and flag2 have ncoarse+8*ngridmax+1 entries, element 0 zero;
+ * flag2 is workspace.  No-op at ilevel == nlevelmax (nsubcycle is 2). */
+int ramses_amr_flag_fine(const ramses_amr_tree *t, const ramses_amr_level *lv, int icount, int levelmin,
+                         int nlevelmax, const double *uold, int *flag1, int *flag2, void *stream);
+
+/* The free-grid list (headf/tailf/numbf of amr/amr_commons.f90): a device
+ * ring of ngridmax grid indices holding the numbf free grids from index
+ * `head` on, in the reference's free-list order. */
+typedef struct ramses_amr_free {
+  int *ring;
+  int head;
+  int numbf;
+} ramses_amr_free;
+
+/* Device workspace of ramses_amr_refine_fine for a level of noct octs whose
+ * next level has noct_fine octs. */
+unsigned long ramses_amr_refine_work_bytes(int noct, int noct_fine);
+
+/* refine_fine(ilevel) (amr/refine_utils.f90:332-585): every leaf cell of the
+ * level with flag1 == 1 gets a new oct (make_grid_fine, :590-948: the next
+ * free grid, its son/father/nbor links, its cells prolongated with
+ * interpol_hydro from the father cell and its six neighbours), then every
+ * refined cell with flag1 == 0 loses its oct (kill_grid, :953-1158: links,
+ * flags and state cleared, grid returned to the tail of the free list).
+ * active_fine[0..*noct_fine) is the grid list of level ilevel+1 (capacity
+ * ngridmax); it is rebuilt as build_comm rebuilds it from the level's
+ * linked list (amr/virtual_boundaries.f90:1286-1420) -- survivors in order,
+ * then the new octs in creation order -- and pos[] follows.  Writes the tree
+ * arrays of t, the state, flag1/flag2 and *fl.  Synchronises `stream` once
+ * (to learn how many octs are made and killed).  Returns -9 and changes
+ * nothing if the free list cannot hold the new octs ("No more free memory",
+ * :415-423); *d_error gets bit 2 if a new oct lacks a neighbour cell. */
+int ramses_amr_refine_fine(const ramses_amr_tree *t, const ramses_amr_level *lv, int *active_fine, int *noct_fine,
+                           int *pos, ramses_amr_free *fl, int *flag1, int *flag2, double *uold, double *unew,
+                           void *work, unsigned long work_bytes, int *d_error, void *stream);
 
 /* ---------------------------------------------------------------------
  * Launch-shape tuning.  The reference's only performance knob is the

@@ -182,8 +182,8 @@ class Reference:
 
 class ReferenceAMR:
     """The reference's own level routines (godunov_fine, set_unew, set_uold,
-    courant_fine, upload_fine) on an oct tree installed from numpy arrays
-    (ref_amr_glue.f90)."""
+    courant_fine, upload_fine, flag_fine, refine_fine) on an oct tree
+    installed from numpy arrays (ref_amr_glue.f90)."""
 
     def __init__(self, ref):
         self.ref = ref
@@ -200,6 +200,16 @@ class ReferenceAMR:
         lib.ref_amr_upload.argtypes = [ctypes.c_int]
         lib.ref_amr_godunov.argtypes = [ctypes.c_int, ctypes.c_double]
         lib.ref_amr_courant.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, _dp]
+        lib.ref_amr_set_free.argtypes = [ctypes.c_int, ip]
+        lib.ref_amr_set_refine.argtypes = [ctypes.c_double] * 6 + [ctypes.c_int, ctypes.c_int]
+        lib.ref_amr_flag_fine.argtypes = [ctypes.c_int, ctypes.c_int]
+        lib.ref_amr_refine_fine.argtypes = [ctypes.c_int]
+        lib.ref_amr_get_active.argtypes = [ctypes.c_int, ip]
+        lib.ref_amr_get_active.restype = ctypes.c_int
+        lib.ref_amr_get_free.argtypes = [ip]
+        lib.ref_amr_get_free.restype = ctypes.c_int
+        lib.ref_amr_get_tree.argtypes = [ip, ip, ip, ip]
+        lib.ref_amr_set_flag1.argtypes = [ip]
         self.lib = lib
 
     def install(self, tree, nlevelmax=None):
@@ -211,6 +221,13 @@ class ReferenceAMR:
         for l, ids in tree.active.items():
             a = np.ascontiguousarray(ids, dtype=np.int32)
             self.lib.ref_amr_set_level(l, len(a), a.ctypes.data_as(ip))
+        # free list: the unused grids in increasing order
+        used = np.zeros(tree.ngridmax + 1, dtype=bool)
+        for ids in tree.active.values():
+            used[np.asarray(ids)] = True
+        free = np.ascontiguousarray(np.nonzero(~used[1:])[0] + 1, dtype=np.int32)
+        self.lib.ref_amr_set_free(len(free), free.ctypes.data_as(ip))
+        self.nlevelmax = nlevelmax or tree.nlevel
 
     def set_interpol(self, interpol_type=1, interpol_var=0):
         self.lib.ref_amr_set_interpol(interpol_type, interpol_var)
@@ -239,6 +256,43 @@ class ReferenceAMR:
         v = [ctypes.c_double() for _ in range(4)]
         self.lib.ref_amr_courant(l, dt_start, *[ctypes.byref(x) for x in v])
         return tuple(x.value for x in v)
+
+    def set_refine(self, levelmin, err_grad_d=-1.0, err_grad_u=-1.0, err_grad_p=-1.0, floor_d=1e-10,
+                   floor_u=1e-10, floor_p=1e-10, nexpand=1):
+        self.lib.ref_amr_set_refine(err_grad_d, err_grad_u, err_grad_p, floor_d, floor_u, floor_p, nexpand,
+                                    levelmin)
+
+    def flag_fine(self, l, icount):
+        self.lib.ref_amr_flag_fine(l, icount)
+
+    def refine_fine(self, l):
+        self.lib.ref_amr_refine_fine(l)
+
+    def set_flag1(self, flag1):
+        a = np.ascontiguousarray(flag1, dtype=np.int32)
+        self.lib.ref_amr_set_flag1(a.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+
+    def grids(self, l):
+        out = np.zeros(self.tree.ngridmax, dtype=np.int32)
+        n = self.lib.ref_amr_get_active(l, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        return out[:n].copy()
+
+    def free_list(self):
+        out = np.zeros(self.tree.ngridmax, dtype=np.int32)
+        n = self.lib.ref_amr_get_free(out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        return out[:n].copy()
+
+    def tree_arrays(self):
+        """(son, father, nbor, flag1) as 0-based copies of the 1-based arrays."""
+        t = self.tree
+        ip = ctypes.POINTER(ctypes.c_int)
+        son = np.zeros(t.ncell, dtype=np.int32)
+        father = np.zeros(t.ngridmax, dtype=np.int32)
+        nbor = np.zeros(6 * t.ngridmax, dtype=np.int32)
+        flag1 = np.zeros(t.ncell, dtype=np.int32)
+        self.lib.ref_amr_get_tree(son.ctypes.data_as(ip), father.ctypes.data_as(ip), nbor.ctypes.data_as(ip),
+                                  flag1.ctypes.data_as(ip))
+        return son, father, nbor, flag1
 
 
 def reference_available():

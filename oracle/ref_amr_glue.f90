@@ -5,8 +5,11 @@
 ! the reference's own AMR module globals (amr/amr_commons.f90,
 ! hydro/hydro_commons.f90) and runs the reference's compiled level routines
 ! on it: set_unew / godunov_fine / set_uold (hydro/godunov_fine.f90),
-! courant_fine (hydro/courant_fine.f90) and upload_fine
-! (hydro/interpol_hydro.f90).  Single domain (ncpu=1), nx=ny=nz=1.
+! courant_fine (hydro/courant_fine.f90), upload_fine
+! (hydro/interpol_hydro.f90), and the mesh refinement flag_fine
+! (amr/flag_utils.f90), refine_fine (amr/refine_utils.f90) and build_comm
+! (amr/virtual_boundaries.f90).  Single domain (ncpu=1), nx=ny=nz=1; the
+! tree arrays are allocated as amr/init_amr.f90 allocates them.
 ! ---------------------------------------------------------------------------
 subroutine ref_amr_init(nlevelmax_in, ngridmax_in) bind(C, name="ref_amr_init")
   use iso_c_binding
@@ -22,19 +25,38 @@ subroutine ref_amr_init(nlevelmax_in, ngridmax_in) bind(C, name="ref_amr_init")
   boxlen = 1.0d0
   ncoarse = nx*ny*nz
   ncpu = 1; myid = 1
+  hydro = .true.       ! set by read_params for the hydro solver
   nlevelmax = nlevelmax_in
   ngridmax = ngridmax_in
   ncell = ncoarse + twotondim*ngridmax
-  if (allocated(son)) deallocate(son, father, nbor, uold, unew, numbtot, active, reception)
+  if (allocated(son)) then
+     deallocate(son, father, nbor, uold, unew, numbtot, active, reception)
+     deallocate(flag1, flag2, cpu_map, cpu_map2, hilbert_key, xg, next, prev)
+     deallocate(headl, taill, numbl, headb, tailb, numbb, boundary, bound_key, bound_key2)
+  end if
   allocate(son(1:ncell), father(1:ngridmax), nbor(1:ngridmax,1:twondim))
   allocate(uold(1:ncell,1:nvar), unew(1:ncell,1:nvar))
   allocate(numbtot(1:10,1:nlevelmax))
   allocate(active(1:nlevelmax), reception(1:ncpu,1:nlevelmax))
+  allocate(flag1(0:ncell), flag2(0:ncell), cpu_map(1:ncell), cpu_map2(1:ncell), hilbert_key(1:ncell))
+  allocate(xg(1:ngridmax,1:ndim), next(1:ngridmax), prev(1:ngridmax))
+  allocate(headl(1:ncpu,1:nlevelmax), taill(1:ncpu,1:nlevelmax), numbl(1:ncpu,1:nlevelmax))
+  allocate(headb(1:MAXBOUND,1:nlevelmax), tailb(1:MAXBOUND,1:nlevelmax), numbb(1:MAXBOUND,1:nlevelmax))
+  allocate(boundary(1:MAXBOUND,1:nlevelmax))
+  ndomain = ncpu
+  allocate(bound_key(0:ndomain), bound_key2(0:ndomain))
+  ordering = 'planar'
+  bound_key(0) = -1; bound_key(1) = 1d30
+  bound_key2 = bound_key
   son = 0; father = 0; nbor = 0; uold = 0; unew = 0; numbtot = 0
+  flag1 = 0; flag2 = 0; cpu_map = 1; cpu_map2 = 0; hilbert_key = 0; xg = 0; next = 0; prev = 0
+  headl = 0; taill = 0; numbl = 0; headb = 0; tailb = 0; numbb = 0
+  headf = 0; tailf = 0; numbf = 0; used_mem = 0
   do il = 1, nlevelmax
      active(il)%ngrid = 0
      reception(1,il)%ngrid = 0
      nullify(active(il)%igrid)
+     boundary(1:MAXBOUND,il)%ngrid = 0
   end do
   mass_tot = 0; ekin_tot = 0; eint_tot = 0
 end subroutine ref_amr_init
@@ -73,7 +95,160 @@ subroutine ref_amr_set_level(ilevel, ngrid, igrid_in) bind(C, name="ref_amr_set_
   end do
   active(ilevel)%ngrid = ngrid
   numbtot(1,ilevel) = ngrid
+  ! the level's linked list in the same order (amr/amr_commons.f90 headl/taill)
+  numbl(1,ilevel) = ngrid
+  headl(1,ilevel) = 0; taill(1,ilevel) = 0
+  if (ngrid > 0) then
+     headl(1,ilevel) = igrid_in(1)
+     taill(1,ilevel) = igrid_in(ngrid)
+     do i = 1, ngrid
+        if (i > 1) then
+           prev(igrid_in(i)) = igrid_in(i-1)
+        else
+           prev(igrid_in(i)) = 0
+        end if
+        if (i < ngrid) then
+           next(igrid_in(i)) = igrid_in(i+1)
+        else
+           next(igrid_in(i)) = 0
+        end if
+     end do
+  end if
 end subroutine ref_amr_set_level
+
+! The free-grid list headf..tailf in the given order.
+subroutine ref_amr_set_free(nfree, igrid_in) bind(C, name="ref_amr_set_free")
+  use iso_c_binding
+  use amr_commons
+  implicit none
+  integer(c_int), value :: nfree
+  integer(c_int) :: igrid_in(*)
+  integer :: i
+  numbf = nfree
+  used_mem = ngridmax - numbf
+  headf = 0; tailf = 0
+  if (nfree > 0) then
+     headf = igrid_in(1)
+     tailf = igrid_in(nfree)
+     do i = 1, nfree
+        if (i > 1) then
+           prev(igrid_in(i)) = igrid_in(i-1)
+        else
+           prev(igrid_in(i)) = 0
+        end if
+        if (i < nfree) then
+           next(igrid_in(i)) = igrid_in(i+1)
+        else
+           next(igrid_in(i)) = 0
+        end if
+     end do
+  end if
+end subroutine ref_amr_set_free
+
+! Refinement criteria (hydro/hydro_parameters.f90:47-52, amr/amr_parameters.f90:85)
+subroutine ref_amr_set_refine(egd, egu, egp, fd, fu, fp, nexp, lmin) bind(C, name="ref_amr_set_refine")
+  use iso_c_binding
+  use amr_commons
+  use hydro_commons
+  implicit none
+  real(c_double), value :: egd, egu, egp, fd, fu, fp
+  integer(c_int), value :: nexp, lmin
+  err_grad_d = egd; err_grad_u = egu; err_grad_p = egp
+  floor_d = fd; floor_u = fu; floor_p = fp
+  nexpand = nexp
+  levelmin = lmin
+end subroutine ref_amr_set_refine
+
+subroutine ref_amr_flag_fine(ilevel, icount) bind(C, name="ref_amr_flag_fine")
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: ilevel, icount
+  integer :: il, ic
+  il = ilevel; ic = icount
+  call flag_fine(il, ic)
+end subroutine ref_amr_flag_fine
+
+! refine_fine(ilevel), then build_comm(ilevel+1) as amr_step does before it
+! touches the next level (amr/amr_step.f90:40-93).
+subroutine ref_amr_refine_fine(ilevel) bind(C, name="ref_amr_refine_fine")
+  use iso_c_binding
+  use amr_commons
+  implicit none
+  integer(c_int), value :: ilevel
+  integer :: il
+  il = ilevel
+  call refine_fine(il)
+  if (il + 1 <= nlevelmax) then
+     il = ilevel + 1
+     call build_comm(il)
+  end if
+end subroutine ref_amr_refine_fine
+
+! Current grid list of a level; returns its length (out must hold ngridmax).
+function ref_amr_get_active(ilevel, out) bind(C, name="ref_amr_get_active") result(n)
+  use iso_c_binding
+  use amr_commons
+  implicit none
+  integer(c_int), value :: ilevel
+  integer(c_int) :: out(*)
+  integer(c_int) :: n
+  integer :: i
+  n = active(ilevel)%ngrid
+  do i = 1, n
+     out(i) = active(ilevel)%igrid(i)
+  end do
+end function ref_amr_get_active
+
+! The free list from headf; returns numbf.
+function ref_amr_get_free(out) bind(C, name="ref_amr_get_free") result(n)
+  use iso_c_binding
+  use amr_commons
+  implicit none
+  integer(c_int) :: out(*)
+  integer(c_int) :: n
+  integer :: i, ig
+  n = numbf
+  ig = headf
+  do i = 1, numbf
+     out(i) = ig
+     ig = next(ig)
+  end do
+end function ref_amr_get_free
+
+subroutine ref_amr_get_tree(son_out, father_out, nbor_out, flag1_out) bind(C, name="ref_amr_get_tree")
+  use iso_c_binding
+  use amr_commons
+  implicit none
+  integer(c_int) :: son_out(*), father_out(*), nbor_out(*), flag1_out(*)
+  integer :: i, j, ncell
+  ncell = ncoarse + twotondim*ngridmax
+  do i = 1, ncell
+     son_out(i) = son(i)
+     flag1_out(i) = flag1(i)
+  end do
+  do i = 1, ngridmax
+     father_out(i) = father(i)
+  end do
+  do j = 1, twondim
+     do i = 1, ngridmax
+        nbor_out(i + (j-1)*ngridmax) = nbor(i,j)
+     end do
+  end do
+end subroutine ref_amr_get_tree
+
+! flag1 of every cell from the caller (the refinement map to refine from)
+subroutine ref_amr_set_flag1(flag1_in) bind(C, name="ref_amr_set_flag1")
+  use iso_c_binding
+  use amr_commons
+  implicit none
+  integer(c_int) :: flag1_in(*)
+  integer :: i, ncell
+  ncell = ncoarse + twotondim*ngridmax
+  flag1(0) = 0
+  do i = 1, ncell
+     flag1(i) = flag1_in(i)
+  end do
+end subroutine ref_amr_set_flag1
 
 subroutine ref_amr_set_interpol(itype, ivar) bind(C, name="ref_amr_set_interpol")
   use iso_c_binding

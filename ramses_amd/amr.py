@@ -1,7 +1,7 @@
 """Oct-tree (AMR) levels resident on the GPU.
 
 ``AmrLevels`` keeps an oct tree (``tree.OctTree``: son / father / nbor and
-the per-level active lists of amr/amr_commons.f90) and the conserved state
+the per-level grid lists of amr/amr_commons.f90) and the conserved state
 (uold / unew of hydro/hydro_commons.f90) in device memory, in the
 reference's own index conventions, and exposes the reference's level
 routines on it:
@@ -11,15 +11,20 @@ routines on it:
   set_uold(l)      set_uold(l)       hydro/godunov_fine.f90:135-232
   upload(l)        upload_fine(l)    hydro/interpol_hydro.f90:5-263
   courant(l, dt0)  courant_fine(l)   hydro/courant_fine.f90:1-159
+  flag(l, icount)  flag_fine(l)      amr/flag_utils.f90:57-106
+  refine(l)        refine_fine(l)    amr/refine_utils.f90:332-585
 
-All work is enqueued on one stream without host synchronisation; ``dt`` may
-be a host float or a device scalar produced by ``courant``.
+The grid list of every level (active(l)%igrid) has room for ngridmax grids
+so that refinement can grow it in place; the free grids sit in a device
+ring in the reference's free-list order (headf .. tailf).  All work is
+enqueued on one stream; only ``refine`` synchronises (to learn how many
+octs it made and killed).
 """
 import ctypes
 
 import numpy as np
 
-from . import _check, _stream_handle, load_library
+from . import _check, _config_cache, _stream_handle, load_library
 
 
 class ramses_amr_tree(ctypes.Structure):
@@ -34,24 +39,39 @@ class ramses_amr_level(ctypes.Structure):
                 ("noct_coarse", ctypes.c_int)]
 
 
+class ramses_refine_params(ctypes.Structure):
+    _fields_ = [("err_grad_d", ctypes.c_double), ("err_grad_u", ctypes.c_double), ("err_grad_p", ctypes.c_double),
+                ("floor_d", ctypes.c_double), ("floor_u", ctypes.c_double), ("floor_p", ctypes.c_double),
+                ("nexpand", ctypes.c_int)]
+
+
+class ramses_amr_free(ctypes.Structure):
+    _fields_ = [("ring", ctypes.c_void_p), ("head", ctypes.c_int), ("numbf", ctypes.c_int)]
+
+
 def _bind(lib):
+    if getattr(lib, "_amr_bound", False):
+        return
     P = ctypes.POINTER
-    lib.ramses_amr_config.argtypes = [ctypes.c_int, ctypes.c_int]
-    lib.ramses_amr_config.restype = ctypes.c_int
-    lib.ramses_amr_work_bytes.argtypes = [ctypes.c_int]
-    lib.ramses_amr_work_bytes.restype = ctypes.c_ulong
-    lib.ramses_amr_godunov.argtypes = [P(ramses_amr_tree), P(ramses_amr_level), ctypes.c_void_p, ctypes.c_void_p,
-                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_ulong, ctypes.c_void_p,
-                                       ctypes.c_void_p]
-    lib.ramses_amr_godunov.restype = ctypes.c_int
-    lib.ramses_amr_copy.argtypes = [P(ramses_amr_tree), P(ramses_amr_level), ctypes.c_void_p, ctypes.c_void_p,
-                                    ctypes.c_void_p]
-    lib.ramses_amr_copy.restype = ctypes.c_int
-    lib.ramses_amr_upload.argtypes = [P(ramses_amr_tree), P(ramses_amr_level), ctypes.c_void_p, ctypes.c_void_p]
-    lib.ramses_amr_upload.restype = ctypes.c_int
-    lib.ramses_amr_courant.argtypes = [P(ramses_amr_tree), P(ramses_amr_level), ctypes.c_void_p, ctypes.c_double,
-                                       ctypes.c_void_p, ctypes.c_void_p]
-    lib.ramses_amr_courant.restype = ctypes.c_int
+    vp, ci, cd, ul = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_ulong
+    T, L = P(ramses_amr_tree), P(ramses_amr_level)
+    sig = {
+        "ramses_amr_config": [ci, ci],
+        "ramses_amr_work_bytes": [ci],
+        "ramses_amr_godunov": [T, L, vp, vp, vp, vp, ul, vp, vp],
+        "ramses_amr_copy": [T, L, vp, vp, vp],
+        "ramses_amr_upload": [T, L, vp, vp],
+        "ramses_amr_courant": [T, L, vp, cd, vp, vp],
+        "ramses_refine_config": [P(ramses_refine_params)],
+        "ramses_amr_flag_fine": [T, L, ci, ci, ci, vp, vp, vp, vp],
+        "ramses_amr_refine_work_bytes": [ci, ci],
+        "ramses_amr_refine_fine": [T, L, vp, P(ci), vp, P(ramses_amr_free), vp, vp, vp, vp, vp, ul, vp, vp],
+    }
+    for name, args in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = ul if name.endswith("work_bytes") else ci
+    lib._amr_bound = True
 
 
 def configure_interpolation(interpol_type=1, interpol_var=0):
@@ -61,10 +81,21 @@ def configure_interpolation(interpol_type=1, interpol_var=0):
     _check(lib.ramses_amr_config(interpol_type, interpol_var), "ramses_amr_config")
 
 
+def configure_refinement(err_grad_d=-1.0, err_grad_u=-1.0, err_grad_p=-1.0, floor_d=1e-10, floor_u=1e-10,
+                         floor_p=1e-10, nexpand=1):
+    """Refinement criteria, hydro/hydro_parameters.f90:47-52 and
+    amr/amr_parameters.f90:85 (defaults: every criterion off, nexpand 1)."""
+    lib = load_library()
+    _bind(lib)
+    p = ramses_refine_params(err_grad_d, err_grad_u, err_grad_p, floor_d, floor_u, floor_p, nexpand)
+    _check(lib.ramses_refine_config(ctypes.byref(p)), "ramses_refine_config")
+
+
 class AmrLevels:
-    def __init__(self, tree, u, device="cuda", stream=None, boxlen=1.0):
+    def __init__(self, tree, u, device="cuda", stream=None, boxlen=1.0, nlevelmax=None):
         """tree: tree.OctTree; u: conserved state (5, ncell) over every cell
-        of every level, icell-1 indexed (OctTree.fill)."""
+        of every level, icell-1 indexed (OctTree.fill).  nlevelmax: deepest
+        level refinement may create (default: the tree's levelmax)."""
         import torch
         self.lib = load_library()
         _bind(self.lib)
@@ -72,6 +103,8 @@ class AmrLevels:
         self.stream = stream
         self.boxlen = boxlen
         self.device = device
+        self.nlevelmax = max(tree.nlevel, tree.levelmax if nlevelmax is None else nlevelmax)
+        ngm = tree.ngridmax
 
         def dev_int(a):  # element 0 unused so that p[i] is the reference's element i
             return torch.as_tensor(np.concatenate([[0], np.asarray(a, dtype=np.int32)]), dtype=torch.int32,
@@ -79,38 +112,72 @@ class AmrLevels:
 
         self.son = dev_int(tree.son)
         self.father = dev_int(tree.father)
-        # nbor[(j-1)*ngridmax + igrid]: shift by one so that igrid indexes directly
-        self.nbor = torch.as_tensor(np.concatenate([[0], tree.nbor]).astype(np.int32), device=device)
-        self.t = ramses_amr_tree(self.son.data_ptr(), self.father.data_ptr(), self.nbor.data_ptr(), 1,
-                                 tree.ngridmax, 1, 1, 1)
-        self.active = {l: torch.as_tensor(ids, dtype=torch.int32, device=device) for l, ids in tree.active.items()}
-        pos = np.zeros(tree.ngridmax + 1, dtype=np.int32)
-        for l, ids in tree.active.items():
+        self.nbor = dev_int(tree.nbor)
+        self.t = ramses_amr_tree(self.son.data_ptr(), self.father.data_ptr(), self.nbor.data_ptr(), 1, ngm, 1, 1, 1)
+        # grid lists with room for ngridmax grids each, and their lengths
+        self.lists = {}
+        self.nocts = {}
+        pos = np.zeros(ngm + 1, dtype=np.int32)
+        for l in range(1, self.nlevelmax + 1):
+            ids = np.asarray(tree.active.get(l, []), dtype=np.int32)
+            buf = np.zeros(ngm, dtype=np.int32)
+            buf[:len(ids)] = ids
             pos[ids] = np.arange(len(ids), dtype=np.int32)
+            self.lists[l] = torch.as_tensor(buf, device=device)
+            self.nocts[l] = len(ids)
         self.pos = torch.as_tensor(pos, device=device)
+        # free grids in increasing order, as the reference's free list after
+        # allocating grids 1..n in order (amr/init_amr.f90:245-256)
+        used = np.zeros(ngm + 1, dtype=bool)
+        for ids in tree.active.values():
+            used[np.asarray(ids)] = True
+        free = np.nonzero(~used[1:])[0].astype(np.int32) + 1
+        ring = np.zeros(ngm, dtype=np.int32)
+        ring[:len(free)] = free
+        self.ring = torch.as_tensor(ring, device=device)
+        self.free = ramses_amr_free(self.ring.data_ptr(), 0, len(free))
         u = np.asarray(u, dtype=np.float64)
         full = np.zeros((5, tree.ncell + 1))
         full[:, 1:] = u
         self.uold = torch.as_tensor(full, device=device).contiguous()
         self.unew = self.uold.clone()
-        maxoct = max(len(v) for v in tree.active.values())
-        self.work = torch.empty(int(self.lib.ramses_amr_work_bytes(maxoct)), dtype=torch.uint8, device=device)
+        self.flag1 = torch.zeros(tree.ncell + 1, dtype=torch.int32, device=device)
+        self.flag2 = torch.zeros_like(self.flag1)
+        self.work = torch.empty(0, dtype=torch.uint8, device=device)
         self.error = torch.zeros(1, dtype=torch.int32, device=device)
-        self.dt = torch.zeros(tree.nlevel + 2, dtype=torch.float64, device=device)
+        self.dt = torch.zeros(self.nlevelmax + 2, dtype=torch.float64, device=device)
+
+    # -- bookkeeping ---------------------------------------------------------
+    def noct(self, l):
+        return self.nocts.get(l, 0)
+
+    @property
+    def active(self):
+        """Current grid list of every non-empty level (device views)."""
+        return {l: self.lists[l][:n] for l, n in self.nocts.items() if n > 0}
+
+    def grids(self, l):
+        return self.lists[l][:self.noct(l)].cpu().numpy()
 
     def dx(self, l):
         return 0.5 ** l * self.boxlen
 
     def _level(self, l):
-        ids = self.active.get(l)
-        coarse = self.active.get(l - 1)
-        return ramses_amr_level(l, self.dx(l), 0 if ids is None else ids.data_ptr(),
-                                0 if ids is None else len(ids), self.pos.data_ptr(),
-                                0 if coarse is None else coarse.data_ptr(), 0 if coarse is None else len(coarse))
+        lst, n = self.lists.get(l), self.noct(l)
+        coarse, nc = self.lists.get(l - 1), self.noct(l - 1)
+        return ramses_amr_level(l, self.dx(l), 0 if lst is None else lst.data_ptr(), n, self.pos.data_ptr(),
+                                0 if (coarse is None or nc == 0) else coarse.data_ptr(), nc)
 
     def _s(self):
         return _stream_handle(self.stream)
 
+    def _work(self, nbytes):
+        import torch
+        if self.work.numel() < nbytes:
+            self.work = torch.empty(int(nbytes * 1.25) + 4096, dtype=torch.uint8, device=self.device)
+        return self.work
+
+    # -- level routines ------------------------------------------------------
     def set_unew(self, l):
         _check(self.lib.ramses_amr_copy(ctypes.byref(self.t), ctypes.byref(self._level(l)),
                                         ctypes.c_void_p(self.uold.data_ptr()), ctypes.c_void_p(self.unew.data_ptr()),
@@ -138,51 +205,98 @@ class AmrLevels:
         """godunov_fine(l) with dt (host float, or None for self.dt[l])."""
         if dt is not None:
             self.dt[l] = float(dt)
+        w = self._work(self.lib.ramses_amr_work_bytes(self.noct(l)))
         _check(self.lib.ramses_amr_godunov(ctypes.byref(self.t), ctypes.byref(self._level(l)),
                                            ctypes.c_void_p(self.uold.data_ptr()), ctypes.c_void_p(self.unew.data_ptr()),
                                            ctypes.c_void_p(self.dt.data_ptr() + 8 * l),
-                                           ctypes.c_void_p(self.work.data_ptr()), self.work.numel(),
+                                           ctypes.c_void_p(w.data_ptr()), w.numel(),
                                            ctypes.c_void_p(self.error.data_ptr()), self._s()), "ramses_amr_godunov")
 
+    def flag(self, l, icount, levelmin):
+        """flag_fine(l, icount): the refinement map of level l in flag1."""
+        _check(self.lib.ramses_amr_flag_fine(ctypes.byref(self.t), ctypes.byref(self._level(l)), icount, levelmin,
+                                             self.nlevelmax, ctypes.c_void_p(self.uold.data_ptr()),
+                                             ctypes.c_void_p(self.flag1.data_ptr()),
+                                             ctypes.c_void_p(self.flag2.data_ptr()), self._s()),
+               "ramses_amr_flag_fine")
+
+    def refine(self, l):
+        """refine_fine(l): make and kill the octs of level l+1 from flag1."""
+        if l >= self.nlevelmax or self.noct(l) == 0:
+            return
+        nf = ctypes.c_int(self.noct(l + 1))
+        w = self._work(self.lib.ramses_amr_refine_work_bytes(self.noct(l), nf.value))
+        _check(self.lib.ramses_amr_refine_fine(ctypes.byref(self.t), ctypes.byref(self._level(l)),
+                                               ctypes.c_void_p(self.lists[l + 1].data_ptr()), ctypes.byref(nf),
+                                               ctypes.c_void_p(self.pos.data_ptr()), ctypes.byref(self.free),
+                                               ctypes.c_void_p(self.flag1.data_ptr()),
+                                               ctypes.c_void_p(self.flag2.data_ptr()),
+                                               ctypes.c_void_p(self.uold.data_ptr()),
+                                               ctypes.c_void_p(self.unew.data_ptr()), ctypes.c_void_p(w.data_ptr()),
+                                               w.numel(), ctypes.c_void_p(self.error.data_ptr()), self._s()),
+               "ramses_amr_refine_fine")
+        self.nocts[l + 1] = nf.value
+
+    # -- host views ------------------------------------------------------------
     def state(self, which="new"):
         t = self.unew if which == "new" else self.uold
         return t[:, 1:].cpu().numpy()
 
+    def tree_arrays(self):
+        """(son, father, nbor, flag1) as the reference's 1-based arrays, 0-based here."""
+        return (self.son[1:].cpu().numpy(), self.father[1:].cpu().numpy(), self.nbor[1:].cpu().numpy(),
+                self.flag1[1:].cpu().numpy())
+
+    def free_list(self):
+        r = self.ring.cpu().numpy()
+        ngm = self.tree.ngridmax
+        return r[(self.free.head + np.arange(self.free.numbf)) % ngm]
+
     def check(self):
-        if int(self.error.item()) != 0:
+        e = int(self.error.item())
+        if e & 1:
             raise RuntimeError("oct tree violates the refinement rule (missing father neighbours)")
+        if e & 2:
+            raise RuntimeError("a new oct has no neighbour cell (refinement rule broken)")
 
 
 class AmrSolver(AmrLevels):
-    """amr_step (amr/amr_step.f90) on a static oct tree: per coarse step the
-    levels are advanced recursively with nsubcycle=2 time steps per level,
-    the fine levels first, exactly in the reference's order of operations:
+    """amr_step (amr/amr_step.f90) with nsubcycle=2: per coarse step the
+    levels are advanced recursively, the fine levels first, in the
+    reference's order of operations:
 
+      refine_fine(i), i = l..nlevelmax, at l == levelmin or icount > 1  :40-93
       newdt_fine(l) -> dt_l = min(courant_fine(l), dt_{l-1}/nsubcycle)  :327-330
       set_unew(l)                                                       :334
       amr_step(l+1, 1); amr_step(l+1, 2)                                :345-352
       godunov_fine(l) with dt_l                                         :389
       set_uold(l); upload_fine(l)                                       :424-442
+      flag_fine(l, icount)                                              :532
       dt_{l-1} = dtold_l + dt_l after the second substep               :573-576
 
-    The time steps stay on the device (min/sum of device scalars), so a
-    coarse step needs no host synchronisation.  Refinement is out of scope:
-    the tree is the one it was built with.
+    The time steps stay on the device (min/sum of device scalars).  With
+    refine=False the tree is static (the reference's `static` mode).
     """
 
-    def __init__(self, tree, u, nsubcycle=2, **kw):
+    def __init__(self, tree, u, nsubcycle=2, refine=False, **kw):
         super().__init__(tree, u, **kw)
         import torch
+        if nsubcycle != 2:
+            raise ValueError("nsubcycle must be 2")
         self.levelmin = tree.levelmin
-        self.nlevelmax = tree.nlevel
         self.nsubcycle = nsubcycle
+        self.adaptive = refine
         self.dtold = torch.zeros_like(self.dt)
         self.time = torch.zeros(1, dtype=torch.float64, device=self.device)
         self.fine_steps = 0
+        self.cell_updates = 0
 
-    def noct(self, l):
-        ids = self.active.get(l)
-        return 0 if ids is None else len(ids)
+    def flag_all(self):
+        """The initial refinement map: flag_fine(l, 2) from the finest level
+        to the coarsest (amr/adaptive_loop.f90:187-190)."""
+        for l in range(self.nlevelmax, 0, -1):
+            if self.noct(l):
+                self.flag(l, 2, self.levelmin)
 
     def coarse_step(self):
         self.amr_step(self.levelmin, 1)
@@ -191,9 +305,11 @@ class AmrSolver(AmrLevels):
         import torch
         if self.noct(l) == 0:
             return
-        # newdt_fine (hydro/courant_fine.f90 via pm/newdt_fine.f90):
+        if self.adaptive and self.levelmin < self.nlevelmax and (l == self.levelmin or icount > 1):
+            for i in range(l, self.nlevelmax + 1):
+                self.refine(i)
+        # newdt_fine (pm/newdt_fine.f90:51, hydro/courant_fine.f90):
         # dtold = dtnew, then dtnew = courant_fine starting from boxlen/smallc
-        from . import _config_cache
         self.dtold[l] = self.dt[l]
         self.courant(l, self.boxlen / _config_cache.get("smallc", 1e-10))
         if l > self.levelmin:
@@ -201,8 +317,7 @@ class AmrSolver(AmrLevels):
         self.set_unew(l)
         if l < self.nlevelmax and self.noct(l + 1) > 0:
             self.amr_step(l + 1, 1)
-            if self.nsubcycle == 2:
-                self.amr_step(l + 1, 2)
+            self.amr_step(l + 1, 2)
         else:
             if l < self.nlevelmax:
                 self.dtold[l + 1] = self.dt[l] / float(self.nsubcycle)
@@ -210,15 +325,15 @@ class AmrSolver(AmrLevels):
             self.time += self.dt[l]        # update_time (amr/update_time.f90:217,331)
             self.fine_steps += 1
         self.godunov(l)
+        self.cell_updates += 8 * self.noct(l)
         self.set_uold(l)
         self.upload(l)
-        if l > self.levelmin:
-            if self.nsubcycle == 1:
-                self.dt[l - 1] = self.dt[l]
-            if icount == 2:
-                self.dt[l - 1] = self.dtold[l] + self.dt[l]
+        if self.adaptive:
+            self.flag(l, icount, self.levelmin)
+        if l > self.levelmin and icount == 2:
+            self.dt[l - 1] = self.dtold[l] + self.dt[l]
 
     def cell_updates_per_coarse_step(self):
-        """sum over levels of 8*noct_l*nsubcycle^(l-levelmin)."""
+        """sum over levels of 8*noct_l*nsubcycle^(l-levelmin) on the current tree."""
         return sum(8 * self.noct(l) * self.nsubcycle ** (l - self.levelmin)
                    for l in range(self.levelmin, self.nlevelmax + 1))

@@ -33,6 +33,24 @@ hipError_t launch_amr_upload(const int* active, int noct, const AmrTreeView& t, 
                              double smallr, hipStream_t s);
 hipError_t launch_amr_courant(const int* active, int noct, const AmrTreeView& t, const double* u, long ustride,
                               double dx, const HydroParams& hp, double* d_dt, hipStream_t s);
+struct RefineParams {
+  double err_grad_d, err_grad_u, err_grad_p;
+  double floor_d, floor_u, floor_p;
+  int nexpand;
+};
+struct TreeMut {
+  int* son;
+  int* father;
+  int* nbor;
+};
+hipError_t launch_flag_fine(const int* active, int noct, const AmrTreeView& t, int ilevel, int icount, int levelmin,
+                            int nlevelmax, const double* u, long us, int* flag1, int* flag2, const RefineParams& rp,
+                            const HydroParams& hp, hipStream_t s);
+size_t refine_work_bytes(int noct, int noct_fine);
+hipError_t launch_refine_fine(const int* active, int noct, const AmrTreeView& t, const TreeMut& m, int* active_fine,
+                              int* noct_fine, int* pos, int* ring, int* head, int* numbf, int* flag1, int* flag2,
+                              double* uold, double* unew, int interpol_type, int interpol_var, double smallr,
+                              void* work, int* d_error, int* nofree, hipStream_t s);
 Tuning& tuning() {
   static Tuning t{0, 0, 0};
   return t;
@@ -224,7 +242,7 @@ int ramses_cmpdt(const double* uu, const double* gg, double dx, int ncell, doubl
 }
 
 namespace {
-int g_interpol_type = 1, g_interpol_var = 0;
+static int g_interpol_type = 1, g_interpol_var = 0;
 
 int amr_view(const ramses_amr_tree* t, ramses::AmrTreeView& v) {
   if (!t || !t->son || !t->father || !t->nbor) return fail(-1, "null tree array");
@@ -339,6 +357,68 @@ int ramses_amr_courant(const ramses_amr_tree* t, const ramses_amr_level* lv, con
   if (e != hipSuccess) return hip_fail(e, "dt init launch");
   e = ramses::launch_amr_courant(lv->active, lv->noct, v, u, v.ncell + 1, lv->dx, c.hp, d_dt, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "amr courant launch");
+  return 0;
+}
+
+namespace {
+static const ramses::RefineParams kRefineDefaults{-1.0, -1.0, -1.0, 1e-10, 1e-10, 1e-10, 1};
+static ramses::RefineParams g_refine = kRefineDefaults;
+}  // namespace
+
+int ramses_refine_config(const ramses_refine_params* p) {
+  if (!p) {
+    g_refine = kRefineDefaults;
+    return 0;
+  }
+  if (p->nexpand < 0 || p->nexpand > 16) return fail(-2, "nexpand must be 0..16");
+  if (!(p->floor_d >= 0) || !(p->floor_u >= 0) || !(p->floor_p >= 0)) return fail(-2, "floors must be >= 0");
+  g_refine = ramses::RefineParams{p->err_grad_d, p->err_grad_u, p->err_grad_p,
+                                  p->floor_d,    p->floor_u,    p->floor_p,    p->nexpand};
+  return 0;
+}
+
+int ramses_amr_flag_fine(const ramses_amr_tree* t, const ramses_amr_level* lv, int icount, int levelmin,
+                         int nlevelmax, const double* uold, int* flag1, int* flag2, void* stream) {
+  const Config c = current();
+  ramses::AmrTreeView v;
+  int rc = amr_view(t, v);
+  if (rc) return rc;
+  if (!lv || !uold || !flag1 || !flag2 || (lv->noct > 0 && !lv->active)) return fail(-1, "null argument");
+  if (lv->noct < 0 || lv->noct > t->ngridmax) return fail(-1, "noct out of range");
+  if (lv->ilevel < 1 || levelmin < 1 || nlevelmax < levelmin) return fail(-1, "bad levels");
+  hipError_t e = ramses::launch_flag_fine(lv->active, lv->noct, v, lv->ilevel, icount, levelmin, nlevelmax, uold,
+                                          v.ncell + 1, flag1, flag2, g_refine, c.hp, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "flag_fine launch");
+  return 0;
+}
+
+unsigned long ramses_amr_refine_work_bytes(int noct, int noct_fine) {
+  return (unsigned long)ramses::refine_work_bytes(noct < 0 ? 0 : noct, noct_fine < 0 ? 0 : noct_fine);
+}
+
+int ramses_amr_refine_fine(const ramses_amr_tree* t, const ramses_amr_level* lv, int* active_fine, int* noct_fine,
+                           int* pos, ramses_amr_free* fl, int* flag1, int* flag2, double* uold, double* unew,
+                           void* work, unsigned long work_bytes, int* d_error, void* stream) {
+  const Config c = current();
+  ramses::AmrTreeView v;
+  int rc = amr_view(t, v);
+  if (rc) return rc;
+  if (!lv || !active_fine || !noct_fine || !pos || !fl || !fl->ring || !flag1 || !flag2 || !uold || !unew ||
+      !d_error || (lv->noct > 0 && !lv->active))
+    return fail(-1, "null argument");
+  if (lv->noct < 0 || lv->noct > t->ngridmax || *noct_fine < 0 || *noct_fine > t->ngridmax)
+    return fail(-1, "noct out of range");
+  if (fl->head < 0 || fl->head >= t->ngridmax || fl->numbf < 0 || fl->numbf > t->ngridmax)
+    return fail(-1, "bad free list");
+  if ((unsigned long)ramses::refine_work_bytes(lv->noct, *noct_fine) > work_bytes || !work)
+    return fail(-1, "work buffer too small (see ramses_amr_refine_work_bytes)");
+  ramses::TreeMut m{t->son, t->father, t->nbor};
+  int nofree = 0;
+  hipError_t e = ramses::launch_refine_fine(lv->active, lv->noct, v, m, active_fine, noct_fine, pos, fl->ring,
+                                            &fl->head, &fl->numbf, flag1, flag2, uold, unew, g_interpol_type,
+                                            g_interpol_var, c.hp.smallr, work, d_error, &nofree, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "refine_fine");
+  if (nofree) return fail(-9, "No more free memory: increase ngridmax");
   return 0;
 }
 

@@ -1,0 +1,484 @@
+// Mesh refinement on the GPU: the refinement map of a level (flag_fine,
+// amr/flag_utils.f90:57-106) and the creation / destruction of the octs of
+// the next level (refine_fine, amr/refine_utils.f90:332-585), with the tree
+// (son / father / nbor), the flags and the free-grid list resident in HBM.
+//
+// flag_fine      flag_init   test_flag: a refined cell stays flagged while a
+//                            child is refined or flagged (:168-208)
+//                flag_count  smooth_fine's neighbour count into flag2
+//                flag_merge  ... and the merge back into flag1 (:555-677)
+//                flag_hydro  hydro_flag / hydro_refine gradient criteria
+//                            (hydro/hydro_flag.f90:1-180,
+//                            hydro/godunov_utils.f90:125-263)
+//                flag_ensure ensure_ref_rules (:213-276)
+// refine_fine    refine_mark one thread per (batch, ind, lane) slot, the
+//                            order in which the reference visits cells, so
+//                            an exclusive scan of the marks gives every new
+//                            or killed oct its rank in the reference's order
+//                refine_make make_grid_fine (:590-948): the k-th new oct takes
+//                            the k-th free grid from the head of the free
+//                            list, links to its father and neighbour cells,
+//                            and its cells are prolongated (interpol_hydro)
+//                refine_kill kill_grid (:953-1158): the k-th killed oct is
+//                            cleared and appended to the tail of the free list
+//                list_*      the level's grid list rebuilt as build_comm does
+//                            from the linked list (amr/virtual_boundaries.f90:
+//                            1286-1420): survivors in order, then new octs in
+//                            creation order.
+// The free list of the reference is a FIFO (taken at headf, returned at
+// tailf); here it is a ring of ngridmax grid indices.
+#include "hydro_device.hpp"
+#include "amr_args.hpp"
+#include "amr_device.hpp"
+
+namespace ramses {
+
+struct RefineParams {
+  double err_grad_d, err_grad_u, err_grad_p;
+  double floor_d, floor_u, floor_p;
+  int nexpand;
+};
+
+struct TreeMut {
+  int* son;
+  int* father;
+  int* nbor;
+};
+
+namespace {
+
+// --- flag_fine ----------------------------------------------------------
+
+// Cells of the level in cell-major order: i -> (ind = i / noct + 1, a = i % noct).
+__device__ __forceinline__ bool level_cell(const int* active, int noct, const AmrTreeView& t, long i, int& g,
+                                           int& ind, long& c) {
+  if (i >= 8L * noct) return false;
+  ind = (int)(i / noct) + 1;
+  g = active[i % noct];
+  c = cell_of(t, ind, g);
+  return true;
+}
+
+__global__ void flag_init(const int* active, int noct, AmrTreeView t, int* flag1, int all) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int g, ind;
+  long c;
+  if (!level_cell(active, noct, t, i, g, ind, c)) return;
+  if (all) {  // below levelmin every cell is flagged (init_flag :150-157)
+    flag1[c] = 1;
+    return;
+  }
+  const int s = t.son[c];
+  int f = 0;
+  if (s > 0)
+    for (int k = 1; k <= 8; ++k) {
+      const long cc = cell_of(t, k, s);
+      if (t.son[cc] > 0 || flag1[cc] == 1) f = 1;
+    }
+  flag1[c] = f;
+}
+
+__global__ void flag_count(const int* active, int noct, AmrTreeView t, const int* flag1, int* flag2, int nthr) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int g, ind;
+  long c;
+  if (!level_cell(active, noct, t, i, g, ind, c)) return;
+  int n = 0;
+  for (int in = 1; in <= 6; ++in) n += flag1[nbor_cell(t, g, ind, in)];  // flag1[0] == 0
+  flag2[c] = n >= nthr ? 1 : 0;
+}
+
+__global__ void flag_merge(const int* active, int noct, AmrTreeView t, int* flag1, int* flag2) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int g, ind;
+  long c;
+  if (!level_cell(active, noct, t, i, g, ind, c)) return;
+  if (flag1[c] == 1) flag2[c] = 0;
+  if (flag2[c] == 1) flag1[c] = 1;
+}
+
+struct Prim3 {
+  double d, v[3], p;
+};
+
+// hydro_refine's conversion to primitives (godunov_utils.f90:149-180).
+__device__ __forceinline__ Prim3 refine_prim(const double* u, long us, long c, const HydroParams& hp) {
+  Prim3 q;
+  q.d = fmax_f(u[c], hp.smallr);
+  for (int d = 0; d < 3; ++d) q.v[d] = u[(long)(d + 1) * us + c] / q.d;
+  double ek = 0.0;
+  for (int d = 0; d < 3; ++d) ek = ek + (0.5 * q.d) * (q.v[d] * q.v[d]);
+  q.p = (hp.gamma - 1.0) * (u[4 * us + c] - ek);
+  return q;
+}
+
+__device__ __forceinline__ double grad_err(double g, double m, double d, double floor_x) {
+  return 2.0 * fmax_f(__builtin_fabs((d - m) / ((d + m) + floor_x)), __builtin_fabs((m - g) / ((m + g) + floor_x)));
+}
+
+__global__ void flag_hydro(const int* active, int noct, AmrTreeView t, const double* u, long us, int* flag1,
+                           RefineParams rp, HydroParams hp) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int g, ind;
+  long c;
+  if (!level_cell(active, noct, t, i, g, ind, c)) return;
+  bool ok = false;
+  const Prim3 m = refine_prim(u, us, c, hp);
+  for (int idim = 0; idim < 3; ++idim) {
+    long cl = nbor_cell(t, g, ind, 2 * idim + 1);
+    long cr = nbor_cell(t, g, ind, 2 * idim + 2);
+    if (cl == 0) cl = nbor_of(t, g, 2 * idim + 1);  // missing neighbour: the father's neighbour
+    if (cr == 0) cr = nbor_of(t, g, 2 * idim + 2);
+    const Prim3 l = refine_prim(u, us, cl, hp), r = refine_prim(u, us, cr, hp);
+    if (rp.err_grad_d >= 0.0) ok = ok || grad_err(l.d, m.d, r.d, rp.floor_d) > rp.err_grad_d;
+    if (rp.err_grad_p >= 0.0) ok = ok || grad_err(l.p, m.p, r.p, rp.floor_p) > rp.err_grad_p;
+    if (rp.err_grad_u >= 0.0) {
+      const double fu2 = rp.floor_u * rp.floor_u;
+      const double cg = __builtin_sqrt(fmax_f(hp.gamma * l.p / l.d, fu2));
+      const double cm = __builtin_sqrt(fmax_f(hp.gamma * m.p / m.d, fu2));
+      const double cd = __builtin_sqrt(fmax_f(hp.gamma * r.p / r.d, fu2));
+      for (int d = 0; d < 3; ++d) {
+        const double vg = l.v[d], vm = m.v[d], vd = r.v[d];
+        const double e1 = __builtin_fabs((vd - vm) / ((((cd + cm) + __builtin_fabs(vd)) + __builtin_fabs(vm)) + rp.floor_u));
+        const double e2 = __builtin_fabs((vm - vg) / ((((cm + cg) + __builtin_fabs(vm)) + __builtin_fabs(vg)) + rp.floor_u));
+        ok = ok || 2.0 * fmax_f(e1, e2) > rp.err_grad_u;
+      }
+    }
+  }
+  if (ok) flag1[c] = 1;
+}
+
+// Thread = one grid: unflag all its cells unless the 27 neighbours of its
+// father cell exist and are refined.
+__global__ void flag_ensure(const int* active, int noct, AmrTreeView t, int ilevel, int* flag1) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= noct) return;
+  const int g = active[a];
+  const long f = t.father[g];
+  bool ok = true;
+  for (int j = 0; j < 27; ++j) {
+    const long nf = nbor_father_cell(t, f, j % 3, (j / 3) % 3, j / 9, ilevel);
+    if (nf == 0 || t.son[nf] == 0) ok = false;
+  }
+  if (!ok)
+    for (int ind = 1; ind <= 8; ++ind) flag1[cell_of(t, ind, g)] = 0;
+}
+
+// --- exclusive scan of 64-bit counters ----------------------------------
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ long long block_excl_scan(long long x, long long* sh, long long& total) {
+  // wave-level inclusive scan, then across the block's waves
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  long long v = x;
+  for (int o = 1; o < 64; o <<= 1) {
+    const long long y = __shfl_up(v, o, 64);
+    if (lane >= o) v += y;
+  }
+  if (lane == 63) sh[wid] = v;
+  __syncthreads();
+  long long base = 0, tot = 0;
+  const int nw = blockDim.x >> 6;
+  for (int w = 0; w < nw; ++w) {
+    if (w < wid) base += sh[w];
+    tot += sh[w];
+  }
+  __syncthreads();
+  total = tot;
+  return base + v - x;
+}
+
+// out[i] = sum of in[0..i) within the tile; sums[tile] = tile total.
+__global__ __launch_bounds__(kScanThreads) void scan_tiles(const long long* in, long long* out, long long* sums,
+                                                            long n) {
+  __shared__ long long sh[kScanThreads / 64];
+  const long base = (long)blockIdx.x * kScanTile + (long)threadIdx.x * kScanItems;
+  long long v[kScanItems], s = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = base + k < n ? in[base + k] : 0;
+    s += v[k];
+  }
+  long long total;
+  long long run = block_excl_scan(s, sh, total);
+  for (int k = 0; k < kScanItems; ++k) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// One block: exclusive scan of the tile totals; sums[ntiles] = grand total.
+__global__ __launch_bounds__(1024) void scan_sums(long long* sums, int ntiles) {
+  __shared__ long long sh[1024 / 64];
+  long long carry = 0;
+  for (int b0 = 0; b0 < ntiles; b0 += 1024) {
+    const int i = b0 + threadIdx.x;
+    const long long x = i < ntiles ? sums[i] : 0;
+    long long total;
+    const long long e = block_excl_scan(x, sh, total);
+    if (i < ntiles) sums[i] = carry + e;
+    carry += total;
+  }
+  if (threadIdx.x == 0) sums[ntiles] = carry;
+}
+
+__global__ void scan_add(long long* out, const long long* sums, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += sums[i / kScanTile];
+}
+
+// --- refine_fine ----------------------------------------------------------
+
+// Slot p = batch*256 + (ind-1)*32 + lane over the level's active list in
+// nvector=32 batches.  mark = create (low 32 bits) | kill << 32.
+__global__ void refine_mark(const int* active, int noct, AmrTreeView t, const int* flag1, long long* mark,
+                            long nslot) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nslot) return;
+  const int b = (int)(p >> 8), r = (int)(p & 255);
+  const int ind = (r >> 5) + 1, lane = r & 31;
+  const int a = b * 32 + lane;
+  long long m = 0;
+  if (a < noct) {
+    const long c = cell_of(t, ind, active[a]);
+    const int s = t.son[c], f = flag1[c];
+    if (f == 1 && s == 0) m = 1;
+    if (f == 0 && s > 0) m = 1LL << 32;
+  }
+  mark[p] = m;
+}
+
+struct RefineRun {
+  AmrTreeView t;      // read view of the same arrays as m
+  TreeMut m;
+  const int* active;  // level ilevel
+  int noct;
+  const long long* mark;
+  const long long* off;  // exclusive scan of mark
+  long nslot;
+  int* ring;
+  int cap;          // ngridmax
+  int head;         // ring index of headf
+  int numbf;        // free grids before this refine
+  int* newg;        // created grids in creation order
+  double* uold;
+  double* unew;
+  long us;
+  int* flag1;
+  int* flag2;
+  int interpol_type, interpol_var;
+  double smallr;
+  int* error;
+};
+
+__global__ void refine_make(RefineRun r) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= r.nslot || (r.mark[p] & 0xffffffffLL) == 0) return;
+  const int b = (int)(p >> 8), q = (int)(p & 255);
+  const int ind = (q >> 5) + 1, lane = q & 31;
+  const int igrid = r.active[b * 32 + lane];
+  const long c = cell_of(r.t, ind, igrid);
+  const int k = (int)(r.off[p] & 0xffffffffLL);
+  const int g = r.ring[(int)(((long)r.head + k) % r.cap)];
+  r.newg[k] = g;
+  r.m.son[c] = g;
+  r.m.father[g] = (int)c;
+  long nb[7];
+  nb[0] = c;
+  for (int in = 1; in <= 6; ++in) {
+    const long n = nbor_cell(r.t, igrid, ind, in);
+    if (n == 0) atomicOr(r.error, 2);  // "Fatal error in make_grid_fine"
+    nb[in] = n;
+    r.m.nbor[(long)(in - 1) * r.cap + g] = (int)n;
+  }
+  double u1[7][5], u2[8][5];
+  for (int j = 0; j < 7; ++j)
+    for (int v = 0; v < 5; ++v) u1[j][v] = r.uold[(long)v * r.us + nb[j]];
+  interpol_oct(u1, u2, r.interpol_type, r.interpol_var, r.smallr);
+  for (int j = 1; j <= 8; ++j) {
+    const long cc = cell_of(r.t, j, g);
+    for (int v = 0; v < 5; ++v) r.uold[(long)v * r.us + cc] = u2[j - 1][v];
+    r.m.son[cc] = 0;
+    r.flag1[cc] = 0;
+  }
+}
+
+__global__ void refine_kill(RefineRun r) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= r.nslot || (r.mark[p] >> 32) == 0) return;
+  const int b = (int)(p >> 8), q = (int)(p & 255);
+  const int ind = (q >> 5) + 1, lane = q & 31;
+  const long c = cell_of(r.t, ind, r.active[b * 32 + lane]);
+  const int k = (int)(r.off[p] >> 32);
+  const int g = r.m.son[c];
+  r.m.son[c] = 0;
+  r.m.father[g] = 0;
+  for (int in = 1; in <= 6; ++in) r.m.nbor[(long)(in - 1) * r.cap + g] = 0;
+  for (int j = 1; j <= 8; ++j) {
+    const long cc = cell_of(r.t, j, g);
+    r.m.son[cc] = 0;
+    r.flag1[cc] = 0;
+    r.flag2[cc] = 0;
+    for (int v = 0; v < 5; ++v) {
+      r.uold[(long)v * r.us + cc] = 0.0;
+      r.unew[(long)v * r.us + cc] = 0.0;
+    }
+  }
+  r.ring[(int)(((long)r.head + r.numbf + k) % r.cap)] = g;
+}
+
+// Survivors of the next level's list (father still set) in list order.
+__global__ void list_alive(const int* list, int n, const int* father, long long* mark) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) mark[i] = father[list[i]] != 0 ? 1 : 0;
+}
+
+__global__ void list_compact(const int* list, int n, const long long* mark, const long long* off, int* tmp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && mark[i]) tmp[off[i]] = list[i];
+}
+
+// list[0..nalive) = tmp[0..nalive) (if tmp), list[nalive..) = newg; pos follows.
+__global__ void list_assemble(int* list, const int* tmp, int nalive, const int* newg, int ncreate, int* pos) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nalive + ncreate) return;
+  if (i < nalive && !tmp) return;  // survivors already in place
+  const int g = i < nalive ? tmp[i] : newg[i - nalive];
+  list[i] = g;
+  pos[g] = i;
+}
+
+inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+hipError_t scan64(const long long* in, long long* out, long long* sums, long n, hipStream_t s) {
+  const long nt = (n + kScanTile - 1) / kScanTile;
+  if (nt == 0) return hipSuccess;
+  hipLaunchKernelGGL(scan_tiles, dim3((unsigned)nt), dim3(kScanThreads), 0, s, in, out, sums, n);
+  hipLaunchKernelGGL(scan_sums, dim3(1), dim3(1024), 0, s, sums, (int)nt);
+  hipLaunchKernelGGL(scan_add, dim3(nblk(n, 256)), dim3(256), 0, s, out, (const long long*)sums, n);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+// flag_fine(ilevel, icount) for one level (amr/flag_utils.f90:57-106).
+hipError_t launch_flag_fine(const int* active, int noct, const AmrTreeView& t, int ilevel, int icount, int levelmin,
+                            int nlevelmax, const double* u, long us, int* flag1, int* flag2, const RefineParams& rp,
+                            const HydroParams& hp, hipStream_t s) {
+  if (ilevel == nlevelmax || noct <= 0) return hipSuccess;
+  const long n = 8L * noct;
+  const unsigned nb = nblk(n, 256);
+  hipLaunchKernelGGL(flag_init, dim3(nb), dim3(256), 0, s, active, noct, t, flag1, ilevel < levelmin ? 1 : 0);
+  if (ilevel < levelmin) return hipGetLastError();
+  auto smooth = [&]() {
+    const int nthr[3] = {1, 2, 2};
+    for (int k = 0; k < 3; ++k) {
+      hipLaunchKernelGGL(flag_count, dim3(nb), dim3(256), 0, s, active, noct, t, (const int*)flag1, flag2, nthr[k]);
+      hipLaunchKernelGGL(flag_merge, dim3(nb), dim3(256), 0, s, active, noct, t, flag1, flag2);
+    }
+  };
+  smooth();
+  if (rp.err_grad_d >= 0.0 || rp.err_grad_p >= 0.0 || rp.err_grad_u >= 0.0)
+    hipLaunchKernelGGL(flag_hydro, dim3(nb), dim3(256), 0, s, active, noct, t, u, us, flag1, rp, hp);
+  for (int e = 0; e < rp.nexpand; ++e) smooth();
+  if (ilevel > levelmin && icount < 2)
+    hipLaunchKernelGGL(flag_ensure, dim3(nblk(noct, 128)), dim3(128), 0, s, active, noct, t, ilevel, flag1);
+  return hipGetLastError();
+}
+
+// Workspace of refine_fine for a level of noct octs whose next level has
+// noct_fine octs.
+size_t refine_work_bytes(int noct, int noct_fine) {
+  const long nslot = (long)((noct + 31) / 32) * 256;
+  const long nmax = nslot > noct_fine ? nslot : noct_fine;
+  const long nt = (nmax + kScanTile - 1) / kScanTile + 1;  // tile totals + grand total
+  return (size_t)(2 * nmax + nt) * sizeof(long long) + (size_t)(nslot + noct_fine) * sizeof(int) + 256;
+}
+
+// refine_fine(ilevel) (amr/refine_utils.f90:332-585).  The one host
+// synchronisation reads the number of octs to create and to kill.
+// On return *noct_fine, *head, *numbf describe the rebuilt next level and
+// the free list.  *nofree = 1 (and nothing is modified) when the free list
+// cannot hold the new octs.
+hipError_t launch_refine_fine(const int* active, int noct, const AmrTreeView& t, const TreeMut& m, int* active_fine,
+                              int* noct_fine, int* pos, int* ring, int* head, int* numbf, int* flag1, int* flag2,
+                              double* uold, double* unew, int interpol_type, int interpol_var, double smallr,
+                              void* work, int* d_error, int* nofree, hipStream_t s) {
+  *nofree = 0;
+  if (noct <= 0) return hipSuccess;
+  const long nslot = (long)((noct + 31) / 32) * 256;
+  const int nf = *noct_fine;
+  const long nmax = nslot > nf ? nslot : nf;
+  const long nt = (nmax + kScanTile - 1) / kScanTile + 1;
+  char* w = (char*)(((uintptr_t)work + 255) & ~(uintptr_t)255);
+  long long* mark = (long long*)w;
+  long long* off = mark + nmax;
+  long long* sums = off + nmax;  // nt entries
+  int* newg = (int*)(sums + nt);
+  int* tmp = newg + nslot;
+  hipLaunchKernelGGL(refine_mark, dim3(nblk(nslot, 256)), dim3(256), 0, s, active, noct, t, (const int*)flag1, mark,
+                     nslot);
+  hipError_t e = scan64(mark, off, sums, nslot, s);
+  if (e != hipSuccess) return e;
+  const long ntiles = (nslot + kScanTile - 1) / kScanTile;
+  long long tot = 0;
+  if ((e = hipMemcpyAsync(&tot, sums + ntiles, sizeof(tot), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+  const int ncreate = (int)(tot & 0xffffffffLL), nkill = (int)(tot >> 32);
+  if (ncreate == 0 && nkill == 0) return hipSuccess;
+  if (ncreate >= *numbf) {  // "No more free memory" (:415-423)
+    *nofree = 1;
+    return hipSuccess;
+  }
+  RefineRun r;
+  r.t = t;
+  r.m = m;
+  r.active = active;
+  r.noct = noct;
+  r.mark = mark;
+  r.off = off;
+  r.nslot = nslot;
+  r.ring = ring;
+  r.cap = t.ngridmax;
+  r.head = *head;
+  r.numbf = *numbf;
+  r.newg = newg;
+  r.uold = uold;
+  r.unew = unew;
+  r.us = t.ncell + 1;
+  r.flag1 = flag1;
+  r.flag2 = flag2;
+  r.interpol_type = interpol_type;
+  r.interpol_var = interpol_var;
+  r.smallr = smallr;
+  r.error = d_error;
+  if (ncreate > 0) hipLaunchKernelGGL(refine_make, dim3(nblk(nslot, 128)), dim3(128), 0, s, r);
+  if (nkill > 0) hipLaunchKernelGGL(refine_kill, dim3(nblk(nslot, 256)), dim3(256), 0, s, r);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  int nalive = nf;
+  if (nkill > 0 && nf > 0) {
+    hipLaunchKernelGGL(list_alive, dim3(nblk(nf, 256)), dim3(256), 0, s, (const int*)active_fine, nf,
+                       (const int*)m.father, mark);
+    if ((e = scan64(mark, off, sums, nf, s)) != hipSuccess) return e;
+    hipLaunchKernelGGL(list_compact, dim3(nblk(nf, 256)), dim3(256), 0, s, (const int*)active_fine, nf,
+                       (const long long*)mark, (const long long*)off, tmp);
+    nalive = nf - nkill;
+    if (nalive + ncreate > 0)
+      hipLaunchKernelGGL(list_assemble, dim3(nblk(nalive + ncreate, 256)), dim3(256), 0, s, active_fine,
+                         (const int*)tmp, nalive, (const int*)newg, ncreate, pos);
+  } else if (ncreate > 0) {
+    // nothing killed: survivors stay in place, the new octs are appended
+    hipLaunchKernelGGL(list_assemble, dim3(nblk(nf + ncreate, 256)), dim3(256), 0, s, active_fine,
+                       (const int*)nullptr, nf, (const int*)newg, ncreate, pos);
+  }
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  *noct_fine = nalive + ncreate;
+  *head = (int)(((long)*head + ncreate) % t.ngridmax);
+  *numbf = *numbf - ncreate + nkill;
+  return hipSuccess;
+}
+
+}  // namespace ramses

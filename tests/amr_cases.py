@@ -5,7 +5,7 @@ from helpers import conserved
 from ramses_amd.tree import OctTree
 
 
-def sphere_tree(levelmin=3, levelmax=5, r0=0.22):
+def sphere_tree(levelmin=3, levelmax=5, r0=0.22, extra_grids=0):
     """A refined ball around the box centre (radius shrinking with level),
     plus an off-centre blob, so every level has coarse/fine interfaces in
     every direction and around corners."""
@@ -13,7 +13,7 @@ def sphere_tree(levelmin=3, levelmax=5, r0=0.22):
         r = np.sqrt(((x - 0.5) ** 2).sum(1))
         r2 = np.sqrt(((x - np.array([0.2, 0.7, 0.35])) ** 2).sum(1))
         return (r < r0 * 0.75 ** (l - levelmin)) | (r2 < 0.08)
-    return OctTree(levelmin, levelmax, refine)
+    return OctTree(levelmin, levelmax, refine, extra_grids=extra_grids)
 
 
 def smooth_state(tree, gamma=5.0 / 3.0, seed=0, amp=0.3):
@@ -56,3 +56,57 @@ def reference_coarse_step(A, tree, levelmin, dt, dtold, nsub=2):
         if l > levelmin and icount == 2:
             dt[l - 1] = dtold[l] + dt[l]
     step(levelmin, 1)
+
+
+def reference_amr_step(A, levelmin, nlevelmax, dt, dtold, adaptive=True, smallc=1e-10, boxlen=1.0):
+    """One coarse step of amr_step (amr/amr_step.f90) driving the
+    reference's compiled routines, with refinement (refine_fine at
+    levelmin / second substeps, flag_fine after each level step)."""
+    def n(l):
+        return len(A.grids(l)) if 1 <= l <= nlevelmax else 0
+
+    def step(l, icount):
+        if n(l) == 0:
+            return
+        if adaptive and levelmin < nlevelmax and (l == levelmin or icount > 1):
+            for i in range(l, nlevelmax + 1):
+                A.refine_fine(i)
+        dtold[l] = dt[l]
+        dt[l] = A.courant(l, boxlen / smallc)[0]
+        if l > levelmin:
+            dt[l] = min(dt[l - 1] / 2.0, dt[l])
+        A.set_unew(l)
+        if l < nlevelmax and n(l + 1) > 0:
+            step(l + 1, 1)
+            step(l + 1, 2)
+        elif l < nlevelmax:
+            dtold[l + 1] = dt[l] / 2.0
+            dt[l + 1] = dt[l] / 2.0
+        A.godunov(l, dt[l])
+        A.set_uold(l)
+        A.upload(l)
+        if adaptive:
+            A.flag_fine(l, icount)
+        if l > levelmin and icount == 2:
+            dt[l - 1] = dtold[l] + dt[l]
+    step(levelmin, 1)
+
+
+def digest(a):
+    """sha256 of an array's little-endian bytes (int32 or float64)."""
+    import hashlib
+    a = np.asarray(a)
+    dt = "<i4" if a.dtype.kind in "iu" else "<f8"
+    return hashlib.sha256(np.ascontiguousarray(a, dtype=dt).tobytes()).hexdigest()
+
+
+def mesh_digests(son, father, nbor, flag1, grids, free, uold, unew):
+    """Digests of a whole adaptive mesh: tree arrays, flags, grid lists per
+    level (dict), free list and both state arrays."""
+    return dict(son=digest(son), father=digest(father), nbor=digest(nbor), flag1=digest(flag1),
+                grids={str(l): digest(g) for l, g in grids.items()}, nocts={str(l): len(g) for l, g in grids.items()},
+                free=digest(free), nfree=len(free), uold=digest(uold), unew=digest(unew))
+
+
+REFINE_CASE = dict(levelmin=3, levelmax=5, nlevelmax=6, extra_grids=20000, seed=3, err_grad_d=0.12,
+                   err_grad_p=0.25, gamma=5.0 / 3.0, riemann="hllc", slope_type=1, courant_factor=0.5)

@@ -23,6 +23,13 @@ Families
                       set_unew, godunov_fine and upload_fine
                       (oracle/ref_amr_glue.f90): dt per level and the cells
                       each routine changed.
+  amr_refine.json     F5: the same kind of tree with room for 20000 more
+                      grids, refined by the reference's own flag_fine and
+                      refine_fine (gradient criteria on density and
+                      pressure), then advanced two adaptive coarse steps in
+                      amr_step's order: sha256 digests of son / father /
+                      nbor / flag1, every level's grid list, the free list
+                      and uold / unew after each stage, and the level dts.
 """
 import json
 import os
@@ -278,12 +285,54 @@ def amr_sphere(R):
     print("amr_sphere.npz:", {l: len(out[f"idx{l}"]) for l in (3, 4, 5)})
 
 
+def amr_refine(R):
+    """F5: flag_fine on every level, refine_fine of levels 3-5, then two
+    adaptive coarse steps, on the REFINE_CASE tree; digests of the mesh
+    after each stage."""
+    sys.path.insert(0, os.path.dirname(HERE))
+    from amr_cases import REFINE_CASE as C, mesh_digests, reference_amr_step, smooth_state, sphere_tree
+    tree = sphere_tree(C["levelmin"], C["levelmax"], extra_grids=C["extra_grids"])
+    u = smooth_state(tree, seed=C["seed"])
+    A = oracle.ReferenceAMR(R)
+    R.set_params(gamma=C["gamma"], riemann=C["riemann"], slope_type=C["slope_type"],
+                 courant_factor=C["courant_factor"])
+    A.install(tree, nlevelmax=C["nlevelmax"])
+    A.set_interpol(1, 0)
+    A.put(0, u)
+    A.put(1, u)
+    A.set_refine(C["levelmin"], err_grad_d=C["err_grad_d"], err_grad_p=C["err_grad_p"])
+    L = C["nlevelmax"]
+
+    def snap():
+        son, father, nbor, flag1 = A.tree_arrays()
+        return mesh_digests(son, father, nbor, flag1, {l: A.grids(l) for l in range(1, L + 1)}, A.free_list(),
+                            A.get(0), A.get(1))
+    stages = {}
+    for l in range(L, 0, -1):
+        if len(A.grids(l)):
+            A.flag_fine(l, 2)
+    stages["flagged"] = snap()
+    for l in range(C["levelmin"], L):
+        A.refine_fine(l)
+    stages["refined"] = snap()
+    dt, dtold = [0.0] * (L + 2), [0.0] * (L + 2)
+    dts = []
+    for k in range(2):
+        reference_amr_step(A, C["levelmin"], L, dt, dtold)
+        stages[f"step{k + 1}"] = snap()
+        dts.append([float(x).hex() for x in dt])
+    with open(os.path.join(HERE, "amr_refine.json"), "w") as f:
+        json.dump(dict(case=C, stages=stages, dts=dts), f, indent=1, sort_keys=True)
+    print("amr_refine.json:", {k: v["nocts"] for k, v in stages.items()})
+
+
 def main():
     R = oracle.Reference()
-    unsplit_cases(R)
-    cmpdt_cases(R)
-    sedov16(R)
-    amr_sphere(R)
+    only = sys.argv[1:]
+    for name, fn in [("unsplit", unsplit_cases), ("cmpdt", cmpdt_cases), ("sedov16", sedov16),
+                     ("amr_sphere", amr_sphere), ("amr_refine", amr_refine)]:
+        if not only or name in only:
+            fn(R)
 
 
 if __name__ == "__main__":

@@ -174,3 +174,135 @@ def test_amr_subcycled_steps_vs_reference(gpu):
         for l in (3, 4, 5):
             assert got[l] == dt[l], (k, l)
     ra.configure()
+
+
+def _refine_setup(egd, egp, nlevelmax=6, seed=3):
+    import ramses_amd as ra
+    from ramses_amd.amr import AmrSolver, configure_interpolation, configure_refinement
+    R = oracle.Reference()
+    A = oracle.ReferenceAMR(R)
+    tree = sphere_tree(extra_grids=20000)
+    u = smooth_state(tree, seed=seed)
+    R.set_params(gamma=5 / 3, riemann="hllc", slope_type=1, courant_factor=0.5)
+    ra.configure(gamma=5 / 3, riemann="hllc", slope_type=1, courant_factor=0.5)
+    A.install(tree, nlevelmax=nlevelmax)
+    A.set_interpol(1, 0)
+    A.put(0, u)
+    A.put(1, u)
+    A.set_refine(3, err_grad_d=egd, err_grad_p=egp)
+    configure_interpolation(1, 0)
+    configure_refinement(err_grad_d=egd, err_grad_p=egp)
+    s = AmrSolver(tree, u, refine=True, nlevelmax=nlevelmax)
+    return A, s, tree
+
+
+def _same_tree(A, s, nlevelmax, tag):
+    rs, rf, rn, rfl = A.tree_arrays()
+    gs, gf, gn, gfl = s.tree_arrays()
+    assert np.array_equal(gs, rs), tag
+    assert np.array_equal(gf, rf), tag
+    assert np.array_equal(gn, rn), tag
+    for l in range(1, nlevelmax + 1):
+        assert np.array_equal(s.grids(l), A.grids(l)), (tag, l)
+    assert np.array_equal(s.free_list(), A.free_list()), tag
+    return gfl, rfl
+
+
+@needs_ref
+@pytest.mark.parametrize("egd,egp", [(0.12, 0.25), (0.35, 0.8)])
+def test_flag_and_refine_vs_reference(gpu, egd, egp):
+    """flag_fine on every level (icount 2 and 1), then refine_fine of
+    levels 3..5: flags, son/father/nbor, grid lists, free list and the
+    prolongated / cleared state, all bitwise."""
+    import ramses_amd as ra
+    A, s, tree = _refine_setup(egd, egp)
+    for icount in (2, 1):
+        for l in range(6, 0, -1):
+            if s.noct(l):
+                A.flag_fine(l, icount)
+                s.flag(l, icount, 3)
+        gfl, rfl = _same_tree(A, s, 6, f"flags icount={icount}")
+        assert np.array_equal(gfl, rfl), icount
+    made = killed = 0
+    for l in range(3, 6):
+        before = {k: s.noct(k) for k in range(1, 7)}
+        A.refine_fine(l)
+        s.refine(l)
+        s.check()
+        gfl, rfl = _same_tree(A, s, 6, f"refine {l}")
+        assert np.array_equal(gfl, rfl), l
+        assert np.array_equal(s.state("old"), A.get(0)), l
+        assert np.array_equal(s.state("new"), A.get(1)), l
+        made += max(0, s.noct(l + 1) - before[l + 1])
+        killed += max(0, before[l + 1] - s.noct(l + 1))
+    assert made + killed > 0
+    ra.configure()
+
+
+@needs_ref
+def test_amr_adaptive_steps_vs_reference(gpu):
+    """Three coarse steps of the adaptive multi-level step (refine at the
+    start of the levelmin step and of every second substep, flag after
+    every level step) against the reference's routines driven in
+    amr_step's order: tree, grid lists, free list, state and dts, bitwise."""
+    import ramses_amd as ra
+    from amr_cases import reference_amr_step
+    A, s, tree = _refine_setup(0.12, 0.25)
+    for l in range(6, 0, -1):
+        if s.noct(l):
+            A.flag_fine(l, 2)
+    s.flag_all()
+    dt = [0.0] * 8
+    dtold = [0.0] * 8
+    counts = []
+    for k in range(3):
+        reference_amr_step(A, 3, 6, dt, dtold)
+        s.coarse_step()
+        s.check()
+        counts.append({l: s.noct(l) for l in range(3, 7)})
+        gfl, rfl = _same_tree(A, s, 6, f"step {k}")
+        assert np.array_equal(gfl, rfl), k
+        assert np.array_equal(s.state("old"), A.get(0)), k
+        got = s.dt.cpu().numpy()
+        for l in range(3, 7):
+            if s.noct(l):
+                assert got[l] == dt[l], (k, l)
+    assert counts[0] != counts[-1] or counts[0][6] > 0
+    ra.configure()
+
+
+def test_adaptive_mesh_golden(gpu):
+    """flag_fine on every level, refine_fine of levels 3-5 and two adaptive
+    coarse steps against the committed digests of the reference's mesh
+    (tree, flags, grid lists, free list, uold, unew) and its level dts."""
+    import ramses_amd as ra
+    from amr_cases import REFINE_CASE as C, mesh_digests, smooth_state, sphere_tree
+    from ramses_amd.amr import AmrSolver, configure_interpolation, configure_refinement
+    with open(os.path.join(os.path.dirname(__file__), "golden", "amr_refine.json")) as f:
+        gold = json.load(f)
+    assert gold["case"] == C
+    tree = sphere_tree(C["levelmin"], C["levelmax"], extra_grids=C["extra_grids"])
+    u = smooth_state(tree, seed=C["seed"])
+    ra.configure(gamma=C["gamma"], riemann=C["riemann"], slope_type=C["slope_type"],
+                 courant_factor=C["courant_factor"])
+    configure_interpolation(1, 0)
+    configure_refinement(err_grad_d=C["err_grad_d"], err_grad_p=C["err_grad_p"])
+    L = C["nlevelmax"]
+    s = AmrSolver(tree, u, refine=True, nlevelmax=L)
+
+    def snap():
+        son, father, nbor, flag1 = s.tree_arrays()
+        return mesh_digests(son, father, nbor, flag1, {l: s.grids(l) for l in range(1, L + 1)}, s.free_list(),
+                            s.state("old"), s.state("new"))
+    s.flag_all()
+    assert snap() == gold["stages"]["flagged"]
+    for l in range(C["levelmin"], L):
+        s.refine(l)
+    assert snap() == gold["stages"]["refined"]
+    for k in range(2):
+        s.coarse_step()
+        s.check()
+        assert snap() == gold["stages"][f"step{k + 1}"], k
+        got = s.dt.cpu().numpy()
+        assert [float(x).hex() for x in got] == gold["dts"][k], k
+    ra.configure()

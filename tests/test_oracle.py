@@ -1,6 +1,8 @@
 """CPU tests of the oracle: pinned against the reference's own golden vectors,
 against the compiled reference where it is built, and against what the
 mathematics fixes independently of both (closed forms, invariants, symmetry)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -190,3 +192,41 @@ def test_cmpdt_closed_form(oracle_lib):
     w = 3 * c + np.abs(v).sum(0)
     expect = (dx / w * (np.sqrt(1 + 2 * C * 1e-4) - 1) / 1e-4).min()
     np.testing.assert_allclose(dt, expect, rtol=1e-13)
+
+
+@pytest.mark.skipif(not oracle.reference_available(), reason="reference build (oracle/_ref) not present")
+def test_refine_fixture_reproduces():
+    """The committed adaptive-mesh digests are what the reference's own
+    flag_fine / refine_fine / amr_step sequence produces here."""
+    import json
+    from amr_cases import REFINE_CASE as C, mesh_digests, reference_amr_step, smooth_state, sphere_tree
+    with open(os.path.join(os.path.dirname(__file__), "golden", "amr_refine.json")) as f:
+        gold = json.load(f)
+    R = oracle.Reference()
+    A = oracle.ReferenceAMR(R)
+    tree = sphere_tree(C["levelmin"], C["levelmax"], extra_grids=C["extra_grids"])
+    u = smooth_state(tree, seed=C["seed"])
+    R.set_params(gamma=C["gamma"], riemann=C["riemann"], slope_type=C["slope_type"],
+                 courant_factor=C["courant_factor"])
+    A.install(tree, nlevelmax=C["nlevelmax"])
+    A.set_interpol(1, 0)
+    A.put(0, u)
+    A.put(1, u)
+    A.set_refine(C["levelmin"], err_grad_d=C["err_grad_d"], err_grad_p=C["err_grad_p"])
+    L = C["nlevelmax"]
+    for l in range(L, 0, -1):
+        if len(A.grids(l)):
+            A.flag_fine(l, 2)
+    son, father, nbor, flag1 = A.tree_arrays()
+    got = mesh_digests(son, father, nbor, flag1, {l: A.grids(l) for l in range(1, L + 1)}, A.free_list(),
+                       A.get(0), A.get(1))
+    assert got == gold["stages"]["flagged"]
+    for l in range(C["levelmin"], L):
+        A.refine_fine(l)
+    dt, dtold = [0.0] * (L + 2), [0.0] * (L + 2)
+    reference_amr_step(A, C["levelmin"], L, dt, dtold)
+    son, father, nbor, flag1 = A.tree_arrays()
+    got = mesh_digests(son, father, nbor, flag1, {l: A.grids(l) for l in range(1, L + 1)}, A.free_list(),
+                       A.get(0), A.get(1))
+    assert got == gold["stages"]["step1"]
+    assert [float(x).hex() for x in dt] == gold["dts"][0]
