@@ -200,6 +200,85 @@ def level_bench(torch, ra, args):
     }
 
 
+AMR_WORKLOAD = dict(levelmin=6, levelmax=10, gamma=1.4, courant=0.8, err_grad_d=0.1, err_grad_p=0.1,
+                    fine_steps=50, ngridmax=1 << 21,
+                    name="3D Sedov blast, levelmin=6 levelmax=10, gradient refinement err_grad_d=err_grad_p=0.1, "
+                         "interpol_type=1, gamma=1.4, hllc, minmod, CFL 0.8, nsubcycle=2, 50 fine steps")
+
+
+def amr_bench(torch, ra, args):
+    """Adaptive oct-tree Sedov run: initial mesh from init_refine on the GPU
+    (untimed), then coarse steps of amr_step (refine, courant, godunov,
+    upload, flag on every level, subcycled) until `fine_steps` finest-level
+    steps; cell-updates = sum over level steps of 8 * octs."""
+    from ramses_amd.amr import AmrSolver, configure_interpolation, configure_refinement
+    from ramses_amd.ics import sedov_regions
+    from ramses_amd.tree import OctTree
+    w = AMR_WORKLOAD
+    nfine = args.fine_steps or w["fine_steps"]
+    lmin, lmax = w["levelmin"], args.levelmax or w["levelmax"]
+    tree = OctTree(lmin, lmin, extra_grids=w["ngridmax"] - sum(8 ** k for k in range(lmin)))
+    ic = lambda x, dx: sedov_regions(x, dx, gamma=w["gamma"])  # noqa: E731
+    out, finals = {}, {}
+    for prec in ("fp64", "fp32"):
+        ra.configure(gamma=w["gamma"], riemann="hllc", slope_type=1, courant_factor=w["courant"], precision=prec)
+        configure_interpolation(1, 0)
+        configure_refinement(err_grad_d=w["err_grad_d"], err_grad_p=w["err_grad_p"])
+        for warm in (True, False):
+            s = AmrSolver(tree, None, refine=True, nlevelmax=lmax)
+            s.init_refine(ic)
+            init_octs = {l: s.noct(l) for l in range(lmin, lmax + 1)}
+            torch.cuda.synchronize()
+            if warm:
+                for _ in range(max(1, args.warmup // 3)):
+                    s.coarse_step()
+                torch.cuda.synchronize()
+                del s
+                continue
+            t0 = time.perf_counter()
+            ncoarse = 0
+            per_step = []
+            while s.fine_steps < nfine:
+                a = time.perf_counter()
+                s.coarse_step()
+                torch.cuda.synchronize()
+                per_step.append(time.perf_counter() - a)
+                ncoarse += 1
+            wall = time.perf_counter() - t0
+            s.check()
+        cu = s.cell_updates
+        ach = 80.0 * cu / wall
+        out[prec] = {"cell_updates_per_s": cu / wall, "wall_s": wall, "coarse_steps": ncoarse,
+                     "fine_steps": s.fine_steps, "cell_updates": cu,
+                     "ms_per_coarse_step": 1e3 * wall / ncoarse,
+                     "octs_initial": init_octs, "octs_final": {l: s.noct(l) for l in range(lmin, lmax + 1)},
+                     "roofline": {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                                  "frac": ach / HBM_PEAK, "traffic": None}}
+        finals[prec] = (s.son.clone(), s.uold.clone(), float(s.time.item()))
+        del s
+    err = None
+    same_mesh = bool(torch.equal(finals["fp64"][0], finals["fp32"][0]))
+    if same_mesh:
+        a, b = finals["fp32"][1], finals["fp64"][1]
+        leaf = (finals["fp64"][0] == 0)
+        err = {n: float(((a[v] - b[v]).abs() * leaf).sum() / ((b[v]).abs() * leaf).sum().clamp_min(1e-300))
+               for v, n in enumerate(["rho", "mx", "my", "mz", "E"])}
+    ra.configure()
+    configure_refinement()
+    head = out["fp64"]
+    return {
+        "metric": "cell-updates/s FP64 vs FP32-mixed, % HBM roofline; p50/p99 solve latency",
+        "value": head["cell_updates_per_s"], "unit": "cell-updates/s", "n_gpus": 1, "steps": head["coarse_steps"],
+        "warmup": args.warmup, "ms_per_step": head["ms_per_coarse_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (Sedov initial conditions)",
+        "config": {"workload": w["name"], "levelmin": lmin, "levelmax": lmax, "fine_steps": nfine,
+                   "parallelism": "1 GPU"},
+        "roofline": head["roofline"], "variants": out, "fp32_same_mesh": same_mesh,
+        "fp32_rel_l1_vs_fp64": err, "t_end": {k: v[2] for k, v in finals.items()},
+        "fp32_over_fp64": out["fp32"]["cell_updates_per_s"] / head["cell_updates_per_s"],
+    }
+
+
 def serve_bench(torch, ra, args):
     """Continuous batching of independent periodic solves: sides log-uniform in
     {16,32,64,128} (seed 3), request i's velocity field a k^-4 Gaussian field at
@@ -272,8 +351,12 @@ def main():
     ap.add_argument("--octs", type=int, default=NOCT)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--variants", default="fp64-hllc,fp32-hllc,fp64-llf,fp32-llf")
-    ap.add_argument("--workload", default="unsplit", choices=["unsplit", "sedov64", "turb512", "turb256", "serve"],
-                    help="unsplit (default, the headline) or a resident-level workload")
+    ap.add_argument("--workload", default="unsplit",
+                    choices=["unsplit", "sedov64", "turb512", "turb256", "serve", "sedov_amr"],
+                    help="unsplit (default, the headline), a resident-level workload, the serving loop or the "
+                         "adaptive Sedov run")
+    ap.add_argument("--fine-steps", type=int, default=0, help="sedov_amr: finest-level steps (default 50)")
+    ap.add_argument("--levelmax", type=int, default=0, help="sedov_amr: deepest level (default 10)")
     ap.add_argument("--requests", type=int, default=2000, help="serve: number of solves")
     ap.add_argument("--load", default="1.2,0.8", help="serve: offered loads / measured capacity (comma list)")
     ap.add_argument("--steps-per-tick", type=int, default=8, help="serve: level steps per scheduler tick")
@@ -304,6 +387,9 @@ def main():
 
     if args.workload == "serve":
         print(json.dumps(serve_bench(torch, ra, args)), flush=True)
+        return
+    if args.workload == "sedov_amr":
+        print(json.dumps(amr_bench(torch, ra, args)), flush=True)
         return
     if args.workload != "unsplit":
         print(json.dumps(level_bench(torch, ra, args)), flush=True)

@@ -254,14 +254,16 @@ unsigned long ramses_amr_refine_work_bytes(int noct, int noct_fine);
  * active_fine[0..*noct_fine) is the grid list of level ilevel+1 (capacity
  * ngridmax); it is rebuilt as build_comm rebuilds it from the level's
  * linked list (amr/virtual_boundaries.f90:1286-1420) -- survivors in order,
- * then the new octs in creation order -- and pos[] follows.  Writes the tree
- * arrays of t, the state, flag1/flag2 and *fl.  Synchronises `stream` once
+ * then the new octs in creation order -- and pos[] follows.  xg (optional,
+ * 3 doubles per grid: xg[3*igrid+d], the grid centres of amr/amr_commons.f90
+ * in units of the coarse cell) is set for new octs (:663-673) and cleared
+ * for killed ones.  Writes the tree arrays of t, the state, flag1/flag2 and *fl.  Synchronises `stream` once
  * (to learn how many octs are made and killed).  Returns -9 and changes
  * nothing if the free list cannot hold the new octs ("No more free memory",
  * :415-423); *d_error gets bit 2 if a new oct lacks a neighbour cell. */
 int ramses_amr_refine_fine(const ramses_amr_tree *t, const ramses_amr_level *lv, int *active_fine, int *noct_fine,
-                           int *pos, ramses_amr_free *fl, int *flag1, int *flag2, double *uold, double *unew,
-                           void *work, unsigned long work_bytes, int *d_error, void *stream);
+                           int *pos, ramses_amr_free *fl, int *flag1, int *flag2, double *xg, double *uold,
+                           double *unew, void *work, unsigned long work_bytes, int *d_error, void *stream);
 
 /* ---------------------------------------------------------------------
  * Launch-shape tuning.  The reference's only performance knob is the

@@ -210,6 +210,11 @@ class ReferenceAMR:
         lib.ref_amr_get_free.restype = ctypes.c_int
         lib.ref_amr_get_tree.argtypes = [ip, ip, ip, ip]
         lib.ref_amr_set_flag1.argtypes = [ip]
+        lib.ref_amr_set_xg.argtypes = [_dp]
+        lib.ref_amr_get_xg.argtypes = [_dp]
+        lib.ref_amr_set_sedov.argtypes = [ctypes.c_double] * 6
+        lib.ref_amr_set_init.argtypes = [ctypes.c_int]
+        lib.ref_amr_init_flow_fine.argtypes = [ctypes.c_int]
         self.lib = lib
 
     def install(self, tree, nlevelmax=None):
@@ -281,6 +286,27 @@ class ReferenceAMR:
         out = np.zeros(self.tree.ngridmax, dtype=np.int32)
         n = self.lib.ref_amr_get_free(out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
         return out[:n].copy()
+
+    def set_xg(self, xg):
+        """Grid centres, xg[igrid-1, d]."""
+        self.lib.ref_amr_set_xg(_ptr(np.ascontiguousarray(xg, dtype=np.float64)))
+
+    def get_xg(self):
+        out = np.zeros((self.tree.ngridmax, 3))
+        self.lib.ref_amr_get_xg(_ptr(out))
+        return out
+
+    def set_sedov(self, d_amb=1.0, p_amb=1e-5, p_point=0.4, center=(0.5, 0.5, 0.5)):
+        self.lib.ref_amr_set_sedov(d_amb, p_amb, p_point, *center)
+
+    def set_init(self, flag):
+        self.lib.ref_amr_set_init(1 if flag else 0)
+
+    def init_flow(self):
+        self.lib.ref_amr_init_flow()
+
+    def init_flow_fine(self, l):
+        self.lib.ref_amr_init_flow_fine(l)
 
     def tree_arrays(self):
         """(son, father, nbor, flag1) as 0-based copies of the 1-based arrays."""

@@ -362,3 +362,77 @@ subroutine ref_amr_courant(ilevel, dt_start, dt, mass, etot, eint) bind(C, name=
   etot = ekin_tot
   eint = eint_tot
 end subroutine ref_amr_courant
+
+! Grid centres xg(igrid,1:3) from the caller (xg_in(3*(igrid-1)+d)), and back.
+subroutine ref_amr_set_xg(xg_in) bind(C, name="ref_amr_set_xg")
+  use iso_c_binding
+  use amr_commons
+  implicit none
+  real(c_double) :: xg_in(*)
+  integer :: i, d
+  do i = 1, ngridmax
+     do d = 1, ndim
+        xg(i,d) = xg_in(3*(i-1)+d)
+     end do
+  end do
+end subroutine ref_amr_set_xg
+
+subroutine ref_amr_get_xg(xg_out) bind(C, name="ref_amr_get_xg")
+  use iso_c_binding
+  use amr_commons
+  implicit none
+  real(c_double) :: xg_out(*)
+  integer :: i, d
+  do i = 1, ngridmax
+     do d = 1, ndim
+        xg_out(3*(i-1)+d) = xg(i,d)
+     end do
+  end do
+end subroutine ref_amr_get_xg
+
+! The Sedov set-up of namelist/sedov3d.nml (&INIT_PARAMS): a 'square' region
+! covering the box with (d_amb, p_amb) and a 'point' region at (xc,yc,zc)
+! depositing pressure p_point (hydro/init_flow_fine.f90:475-596).
+subroutine ref_amr_set_sedov(d_amb, p_amb, p_point, xc, yc, zc) bind(C, name="ref_amr_set_sedov")
+  use iso_c_binding
+  use amr_commons
+  use hydro_commons
+  implicit none
+  real(c_double), value :: d_amb, p_amb, p_point, xc, yc, zc
+  nregion = 2
+  region_type(1) = 'square'
+  region_type(2) = 'point'
+  x_center(1) = 0.5d0*boxlen; y_center(1) = 0.5d0*boxlen; z_center(1) = 0.5d0*boxlen
+  x_center(2) = xc; y_center(2) = yc; z_center(2) = zc
+  length_x(1:2) = (/10.0d0*boxlen, 1.0d0/)
+  length_y(1:2) = (/10.0d0*boxlen, 1.0d0/)
+  length_z(1:2) = (/10.0d0*boxlen, 1.0d0/)
+  exp_region(1:2) = 10.0d0
+  d_region(1:2) = (/d_amb, 0.0d0/)
+  u_region(1:2) = 0.0d0; v_region(1:2) = 0.0d0; w_region(1:2) = 0.0d0
+  p_region(1:2) = (/p_amb, p_point/)
+end subroutine ref_amr_set_sedov
+
+subroutine ref_amr_set_init(flag) bind(C, name="ref_amr_set_init")
+  use iso_c_binding
+  use amr_commons
+  implicit none
+  integer(c_int), value :: flag
+  init = (flag /= 0)
+end subroutine ref_amr_set_init
+
+! init_flow (hydro/init_flow_fine.f90:5-23): condinit on every level
+! >= levelmin, finest first, each followed by upload_fine.
+subroutine ref_amr_init_flow() bind(C, name="ref_amr_init_flow")
+  implicit none
+  call init_flow
+end subroutine ref_amr_init_flow
+
+subroutine ref_amr_init_flow_fine(ilevel) bind(C, name="ref_amr_init_flow_fine")
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: ilevel
+  integer :: il
+  il = ilevel
+  call init_flow_fine(il)
+end subroutine ref_amr_init_flow_fine

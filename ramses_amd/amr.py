@@ -65,7 +65,7 @@ def _bind(lib):
         "ramses_refine_config": [P(ramses_refine_params)],
         "ramses_amr_flag_fine": [T, L, ci, ci, ci, vp, vp, vp, vp],
         "ramses_amr_refine_work_bytes": [ci, ci],
-        "ramses_amr_refine_fine": [T, L, vp, P(ci), vp, P(ramses_amr_free), vp, vp, vp, vp, vp, ul, vp, vp],
+        "ramses_amr_refine_fine": [T, L, vp, P(ci), vp, P(ramses_amr_free), vp, vp, vp, vp, vp, vp, ul, vp, vp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -94,8 +94,9 @@ def configure_refinement(err_grad_d=-1.0, err_grad_u=-1.0, err_grad_p=-1.0, floo
 class AmrLevels:
     def __init__(self, tree, u, device="cuda", stream=None, boxlen=1.0, nlevelmax=None):
         """tree: tree.OctTree; u: conserved state (5, ncell) over every cell
-        of every level, icell-1 indexed (OctTree.fill).  nlevelmax: deepest
-        level refinement may create (default: the tree's levelmax)."""
+        of every level, icell-1 indexed (OctTree.fill), or None for zeros.
+        nlevelmax: deepest level refinement may create (default: the tree's
+        levelmax)."""
         import torch
         self.lib = load_library()
         _bind(self.lib)
@@ -126,6 +127,11 @@ class AmrLevels:
             self.lists[l] = torch.as_tensor(buf, device=device)
             self.nocts[l] = len(ids)
         self.pos = torch.as_tensor(pos, device=device)
+        # grid centres xg (amr/amr_commons.f90) in units of the coarse cell
+        xg = np.zeros((ngm + 1, 3))
+        for (l, gx, gy, gz), i in tree.grid_at.items():
+            xg[i] = (np.array([gx, gy, gz]) + 0.5) * 0.5 ** (l - 1)
+        self.xg = torch.as_tensor(xg, device=device)
         # free grids in increasing order, as the reference's free list after
         # allocating grids 1..n in order (amr/init_amr.f90:245-256)
         used = np.zeros(ngm + 1, dtype=bool)
@@ -136,10 +142,12 @@ class AmrLevels:
         ring[:len(free)] = free
         self.ring = torch.as_tensor(ring, device=device)
         self.free = ramses_amr_free(self.ring.data_ptr(), 0, len(free))
-        u = np.asarray(u, dtype=np.float64)
-        full = np.zeros((5, tree.ncell + 1))
-        full[:, 1:] = u
-        self.uold = torch.as_tensor(full, device=device).contiguous()
+        if u is None:
+            self.uold = torch.zeros((5, tree.ncell + 1), dtype=torch.float64, device=device)
+        else:
+            full = np.zeros((5, tree.ncell + 1))
+            full[:, 1:] = np.asarray(u, dtype=np.float64)
+            self.uold = torch.as_tensor(full, device=device).contiguous()
         self.unew = self.uold.clone()
         self.flag1 = torch.zeros(tree.ncell + 1, dtype=torch.int32, device=device)
         self.flag2 = torch.zeros_like(self.flag1)
@@ -231,11 +239,33 @@ class AmrLevels:
                                                ctypes.c_void_p(self.pos.data_ptr()), ctypes.byref(self.free),
                                                ctypes.c_void_p(self.flag1.data_ptr()),
                                                ctypes.c_void_p(self.flag2.data_ptr()),
+                                               ctypes.c_void_p(self.xg.data_ptr()),
                                                ctypes.c_void_p(self.uold.data_ptr()),
                                                ctypes.c_void_p(self.unew.data_ptr()), ctypes.c_void_p(w.data_ptr()),
                                                w.numel(), ctypes.c_void_p(self.error.data_ptr()), self._s()),
                "ramses_amr_refine_fine")
         self.nocts[l + 1] = nf.value
+
+    def cells(self, l):
+        """(icell, centres) of every cell of level l on the device, cell-major
+        (ind 1..8 outer, the level's grid order inner); centres in units of
+        the coarse cell (amr/amr_commons.f90 xg plus the cell offsets)."""
+        import torch
+        g = self.lists[l][:self.noct(l)].long()
+        ngm = self.tree.ngridmax
+        ind = torch.arange(8, device=self.device)
+        off = torch.stack([(ind & 1), (ind >> 1) & 1, ind >> 2], 1).double() - 0.5
+        x = self.xg[g][None, :, :] + off[:, None, :] * 0.5 ** l
+        icell = 1 + ind[:, None] * ngm + g[None, :]
+        return icell.reshape(-1), x.reshape(-1, 3)
+
+    def init_flow_fine(self, l, ic):
+        """init_flow_fine(l) (hydro/init_flow_fine.f90:28-470, the condinit
+        branch): every cell of level l takes ic(centres (N,3), dx) -> (5, N)."""
+        if self.noct(l) == 0:
+            return
+        icell, x = self.cells(l)
+        self.uold[:, icell] = ic(x * self.boxlen, 0.5 ** l * self.boxlen)
 
     # -- host views ------------------------------------------------------------
     def state(self, which="new"):
@@ -298,8 +328,49 @@ class AmrSolver(AmrLevels):
             if self.noct(l):
                 self.flag(l, 2, self.levelmin)
 
+    def init_flow(self, ic):
+        """init_flow (hydro/init_flow_fine.f90:5-23): condinit on every level
+        >= levelmin from the finest, each level followed by upload_fine."""
+        for l in range(self.nlevelmax, 0, -1):
+            if l >= self.levelmin:
+                self.init_flow_fine(l, ic)
+            self.upload(l)
+
+    def init_refine(self, ic):
+        """Build the initial mesh and state from the initial conditions ic as
+        init_refine and init_refine_2 do (amr/init_refine.f90:1-150):
+        repeated init_flow / flag / refine passes, then passes that refine,
+        re-impose the initial conditions level by level, upload and flag."""
+        L, lmin = self.nlevelmax, self.levelmin
+        for il in range(lmin + 1, L + 1):
+            self.init_flow(ic)
+            for l in range(L - 1, 0, -1):
+                self.flag(l, 2, lmin)
+            for l in range(lmin, L):
+                self.refine(l)
+            if self.noct(il) == 0:
+                break
+        self.init_flow(ic)
+        for _ in range(lmin, L + 2):
+            for l in range(1, L + 1):
+                if l >= lmin:
+                    self.refine(l)
+                self.init_flow_fine(l, ic)
+            for l in range(L, lmin - 1, -1):
+                self.upload(l)
+            for l in range(L, 0, -1):
+                self.flag(l, 2, lmin)
+
     def coarse_step(self):
+        """One coarse step of adaptive_loop (amr/adaptive_loop.f90:86-191):
+        amr_step(levelmin, 1), then with refinement on, upload_fine and
+        flag_fine of the levels below levelmin."""
         self.amr_step(self.levelmin, 1)
+        if self.adaptive and self.levelmin < self.nlevelmax:
+            for l in range(self.levelmin - 1, 0, -1):
+                self.upload(l)
+            for l in range(self.levelmin - 1, 0, -1):
+                self.flag(l, 2, self.levelmin)
 
     def amr_step(self, l, icount):
         import torch

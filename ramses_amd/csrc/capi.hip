@@ -47,10 +47,11 @@ hipError_t launch_flag_fine(const int* active, int noct, const AmrTreeView& t, i
                             int nlevelmax, const double* u, long us, int* flag1, int* flag2, const RefineParams& rp,
                             const HydroParams& hp, hipStream_t s);
 size_t refine_work_bytes(int noct, int noct_fine);
-hipError_t launch_refine_fine(const int* active, int noct, const AmrTreeView& t, const TreeMut& m, int* active_fine,
-                              int* noct_fine, int* pos, int* ring, int* head, int* numbf, int* flag1, int* flag2,
-                              double* uold, double* unew, int interpol_type, int interpol_var, double smallr,
-                              void* work, int* d_error, int* nofree, hipStream_t s);
+hipError_t launch_refine_fine(const int* active, int noct, int ilevel, const AmrTreeView& t, const TreeMut& m,
+                              int* active_fine, int* noct_fine, int* pos, int* ring, int* head, int* numbf,
+                              int* flag1, int* flag2, double* xg, double* uold, double* unew, int interpol_type,
+                              int interpol_var, double smallr, void* work, int* d_error, int* nofree,
+                              hipStream_t s);
 Tuning& tuning() {
   static Tuning t{0, 0, 0};
   return t;
@@ -397,8 +398,8 @@ unsigned long ramses_amr_refine_work_bytes(int noct, int noct_fine) {
 }
 
 int ramses_amr_refine_fine(const ramses_amr_tree* t, const ramses_amr_level* lv, int* active_fine, int* noct_fine,
-                           int* pos, ramses_amr_free* fl, int* flag1, int* flag2, double* uold, double* unew,
-                           void* work, unsigned long work_bytes, int* d_error, void* stream) {
+                           int* pos, ramses_amr_free* fl, int* flag1, int* flag2, double* xg, double* uold,
+                           double* unew, void* work, unsigned long work_bytes, int* d_error, void* stream) {
   const Config c = current();
   ramses::AmrTreeView v;
   int rc = amr_view(t, v);
@@ -414,9 +415,11 @@ int ramses_amr_refine_fine(const ramses_amr_tree* t, const ramses_amr_level* lv,
     return fail(-1, "work buffer too small (see ramses_amr_refine_work_bytes)");
   ramses::TreeMut m{t->son, t->father, t->nbor};
   int nofree = 0;
-  hipError_t e = ramses::launch_refine_fine(lv->active, lv->noct, v, m, active_fine, noct_fine, pos, fl->ring,
-                                            &fl->head, &fl->numbf, flag1, flag2, uold, unew, g_interpol_type,
-                                            g_interpol_var, c.hp.smallr, work, d_error, &nofree, (hipStream_t)stream);
+  if (lv->ilevel < 1) return fail(-1, "ilevel must be >= 1");
+  hipError_t e = ramses::launch_refine_fine(lv->active, lv->noct, lv->ilevel, v, m, active_fine, noct_fine, pos,
+                                            fl->ring, &fl->head, &fl->numbf, flag1, flag2, xg, uold, unew,
+                                            g_interpol_type, g_interpol_var, c.hp.smallr, work, d_error, &nofree,
+                                            (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "refine_fine");
   if (nofree) return fail(-9, "No more free memory: increase ngridmax");
   return 0;

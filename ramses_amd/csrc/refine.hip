@@ -269,6 +269,8 @@ struct RefineRun {
   long us;
   int* flag1;
   int* flag2;
+  double* xg;       // grid centres xg[3*igrid + d], or null
+  double dx;        // cell size of the level being refined (0.5^ilevel)
   int interpol_type, interpol_var;
   double smallr;
   int* error;
@@ -286,6 +288,10 @@ __global__ void refine_make(RefineRun r) {
   r.newg[k] = g;
   r.m.son[c] = g;
   r.m.father[g] = (int)c;
+  if (r.xg) {  // centre of the new oct = centre of its father cell (make_grid_fine :663-673)
+    const int p[3] = {(ind - 1) & 1, ((ind - 1) >> 1) & 1, (ind - 1) >> 2};
+    for (int d = 0; d < 3; ++d) r.xg[3L * g + d] = r.xg[3L * igrid + d] + ((double)p[d] - 0.5) * r.dx;
+  }
   long nb[7];
   nb[0] = c;
   for (int in = 1; in <= 6; ++in) {
@@ -316,6 +322,8 @@ __global__ void refine_kill(RefineRun r) {
   const int g = r.m.son[c];
   r.m.son[c] = 0;
   r.m.father[g] = 0;
+  if (r.xg)
+    for (int d = 0; d < 3; ++d) r.xg[3L * g + d] = 0.0;
   for (int in = 1; in <= 6; ++in) r.m.nbor[(long)(in - 1) * r.cap + g] = 0;
   for (int j = 1; j <= 8; ++j) {
     const long cc = cell_of(r.t, j, g);
@@ -403,10 +411,11 @@ size_t refine_work_bytes(int noct, int noct_fine) {
 // On return *noct_fine, *head, *numbf describe the rebuilt next level and
 // the free list.  *nofree = 1 (and nothing is modified) when the free list
 // cannot hold the new octs.
-hipError_t launch_refine_fine(const int* active, int noct, const AmrTreeView& t, const TreeMut& m, int* active_fine,
-                              int* noct_fine, int* pos, int* ring, int* head, int* numbf, int* flag1, int* flag2,
-                              double* uold, double* unew, int interpol_type, int interpol_var, double smallr,
-                              void* work, int* d_error, int* nofree, hipStream_t s) {
+hipError_t launch_refine_fine(const int* active, int noct, int ilevel, const AmrTreeView& t, const TreeMut& m,
+                              int* active_fine, int* noct_fine, int* pos, int* ring, int* head, int* numbf,
+                              int* flag1, int* flag2, double* xg, double* uold, double* unew, int interpol_type,
+                              int interpol_var, double smallr, void* work, int* d_error, int* nofree,
+                              hipStream_t s) {
   *nofree = 0;
   if (noct <= 0) return hipSuccess;
   const long nslot = (long)((noct + 31) / 32) * 256;
@@ -451,6 +460,8 @@ hipError_t launch_refine_fine(const int* active, int noct, const AmrTreeView& t,
   r.us = t.ncell + 1;
   r.flag1 = flag1;
   r.flag2 = flag2;
+  r.xg = xg;
+  r.dx = __builtin_ldexp(1.0, -ilevel);
   r.interpol_type = interpol_type;
   r.interpol_var = interpol_var;
   r.smallr = smallr;

@@ -11,6 +11,10 @@
   the requested 3D rms Mach number for c_s=1, optionally a lognormal density
   and an isothermal pressure P = rho/gamma.
 Both return conserved state u[var][z][y][x] as a float64 tensor.
+* ``sedov_regions(x, dx)`` -- the same Sedov set-up evaluated at arbitrary
+  cell centres x (N, 3) with the cell size dx of their level, as
+  init_flow_fine calls condinit on every level of an oct tree; returns the
+  conserved state (5, N).
 """
 import math
 
@@ -63,3 +67,21 @@ def sound_crossing_time(boxlen=1.0, cs=1.0):
 
 
 __all__ = ["sedov", "turbulent", "sound_crossing_time", "math"]
+
+
+def sedov_regions(x, dx, gamma=1.4, d_amb=1.0, p_amb=1e-5, p_point=0.4, center=(0.5, 0.5, 0.5)):
+    """region_condinit (hydro/init_flow_fine.f90:475-596) for the two regions
+    of namelist/sedov3d.nml -- a 'square' covering the box (d_amb, p_amb)
+    and a 'point' at `center` adding p_point*r/dx^3 with the CIC weight
+    r = prod max(1 - |x - c|/dx, 0) -- then condinit's conversion to
+    conserved variables (hydro/condinit.f90:36-71); zero velocity."""
+    import torch
+    w = torch.clamp(1.0 - (x - torch.tensor(center, dtype=x.dtype, device=x.device)).abs() / dx, min=0.0)
+    r = w[:, 0] * w[:, 1] * w[:, 2]
+    vol = dx * dx * dx
+    rho = torch.full_like(r, d_amb) + 0.0 * r / vol
+    p = torch.full_like(r, p_amb) + p_point * r / vol
+    u = torch.zeros((5, x.shape[0]), dtype=x.dtype, device=x.device)
+    u[0] = rho
+    u[4] = p / (gamma - 1.0)
+    return u

@@ -56,6 +56,39 @@ def reference_coarse_step(A, tree, levelmin, dt, dtold, nsub=2):
         if l > levelmin and icount == 2:
             dt[l - 1] = dtold[l] + dt[l]
     step(levelmin, 1)
+    if adaptive and levelmin < nlevelmax:  # amr/adaptive_loop.f90:143-191
+        for l in range(levelmin - 1, 0, -1):
+            A.upload(l)
+        for l in range(levelmin - 1, 0, -1):
+            A.flag_fine(l, 2)
+
+
+def reference_init_refine(A, levelmin, nlevelmax):
+    """init_refine + init_refine_2 (amr/init_refine.f90) on a tree installed
+    up to levelmin, with the reference's own init_flow / flag_fine /
+    refine_fine; init is set during the first phase as the reference sets
+    it (make_grid_fine then leaves new cells to init_flow)."""
+    def n(l):
+        return len(A.grids(l))
+    A.set_init(True)
+    for il in range(levelmin + 1, nlevelmax + 1):
+        A.init_flow()
+        for l in range(nlevelmax - 1, 0, -1):
+            A.flag_fine(l, 2)
+        for l in range(1, nlevelmax):
+            A.refine_fine(l)
+        if n(il) == 0:
+            break
+    A.set_init(False)
+    A.init_flow()
+    for _ in range(levelmin, nlevelmax + 2):
+        for l in range(1, nlevelmax + 1):
+            A.refine_fine(l)
+            A.init_flow_fine(l)
+        for l in range(nlevelmax, levelmin - 1, -1):
+            A.upload(l)
+        for l in range(nlevelmax, 0, -1):
+            A.flag_fine(l, 2)
 
 
 def reference_amr_step(A, levelmin, nlevelmax, dt, dtold, adaptive=True, smallc=1e-10, boxlen=1.0):
@@ -90,6 +123,39 @@ def reference_amr_step(A, levelmin, nlevelmax, dt, dtold, adaptive=True, smallc=
         if l > levelmin and icount == 2:
             dt[l - 1] = dtold[l] + dt[l]
     step(levelmin, 1)
+    if adaptive and levelmin < nlevelmax:  # amr/adaptive_loop.f90:143-191
+        for l in range(levelmin - 1, 0, -1):
+            A.upload(l)
+        for l in range(levelmin - 1, 0, -1):
+            A.flag_fine(l, 2)
+
+
+def reference_init_refine(A, levelmin, nlevelmax):
+    """init_refine + init_refine_2 (amr/init_refine.f90) on a tree installed
+    up to levelmin, with the reference's own init_flow / flag_fine /
+    refine_fine; init is set during the first phase as the reference sets
+    it (make_grid_fine then leaves new cells to init_flow)."""
+    def n(l):
+        return len(A.grids(l))
+    A.set_init(True)
+    for il in range(levelmin + 1, nlevelmax + 1):
+        A.init_flow()
+        for l in range(nlevelmax - 1, 0, -1):
+            A.flag_fine(l, 2)
+        for l in range(1, nlevelmax):
+            A.refine_fine(l)
+        if n(il) == 0:
+            break
+    A.set_init(False)
+    A.init_flow()
+    for _ in range(levelmin, nlevelmax + 2):
+        for l in range(1, nlevelmax + 1):
+            A.refine_fine(l)
+            A.init_flow_fine(l)
+        for l in range(nlevelmax, levelmin - 1, -1):
+            A.upload(l)
+        for l in range(nlevelmax, 0, -1):
+            A.flag_fine(l, 2)
 
 
 def digest(a):

@@ -306,3 +306,57 @@ def test_adaptive_mesh_golden(gpu):
         got = s.dt.cpu().numpy()
         assert [float(x).hex() for x in got] == gold["dts"][k], k
     ra.configure()
+
+
+@needs_ref
+def test_sedov_amr_init_and_steps_vs_reference(gpu):
+    """The adaptive Sedov set-up (namelist/sedov3d.nml regions, gamma 1.4,
+    hllc, minmod, CFL 0.8, err_grad_d = err_grad_p = 0.1, interpol_type 1)
+    at levelmin 3, levelmax 6: the initial mesh built by init_refine /
+    init_refine_2 on the GPU against the reference's own init_flow /
+    flag_fine / refine_fine, then three adaptive coarse steps -- mesh, grid
+    centres, state and dts bitwise."""
+    import ramses_amd as ra
+    from amr_cases import mesh_digests, reference_amr_step, reference_init_refine
+    from ramses_amd.amr import AmrSolver, configure_interpolation, configure_refinement
+    from ramses_amd.ics import sedov_regions
+    from ramses_amd.tree import OctTree
+    lmin, L = 3, 6
+    tree = OctTree(lmin, lmin, extra_grids=20000)
+    R = oracle.Reference()
+    A = oracle.ReferenceAMR(R)
+    R.set_params(gamma=1.4, riemann="hllc", slope_type=1, courant_factor=0.8)
+    ra.configure(gamma=1.4, riemann="hllc", slope_type=1, courant_factor=0.8)
+    A.install(tree, nlevelmax=L)
+    A.set_interpol(1, 0)
+    A.set_refine(lmin, err_grad_d=0.1, err_grad_p=0.1)
+    configure_interpolation(1, 0)
+    configure_refinement(err_grad_d=0.1, err_grad_p=0.1)
+    s = AmrSolver(tree, None, refine=True, nlevelmax=L)
+    A.set_xg(s.xg[1:].cpu().numpy())
+    A.set_sedov(1.0, 1e-5, 0.4, (0.5, 0.5, 0.5))
+    reference_init_refine(A, lmin, L)
+    s.init_refine(lambda x, dx: sedov_regions(x, dx, gamma=1.4))
+
+    def snap(src):
+        son, father, nbor, flag1 = src.tree_arrays()
+        if src is s:
+            return mesh_digests(son, father, nbor, flag1, {l: s.grids(l) for l in range(1, L + 1)}, s.free_list(),
+                                s.state("old"), s.state("new"))
+        return mesh_digests(son, father, nbor, flag1, {l: A.grids(l) for l in range(1, L + 1)}, A.free_list(),
+                            A.get(0), A.get(1))
+    assert snap(s) == snap(A)
+    assert np.array_equal(s.xg[1:].cpu().numpy(), A.get_xg())
+    assert s.noct(L) > 0
+    dt, dtold = [0.0] * (L + 2), [0.0] * (L + 2)
+    for k in range(3):
+        reference_amr_step(A, lmin, L, dt, dtold)
+        s.coarse_step()
+        s.check()
+        assert snap(s) == snap(A), k
+        got = s.dt.cpu().numpy()
+        for l in range(lmin, L + 1):
+            if s.noct(l):
+                assert got[l] == dt[l], (k, l)
+    assert np.array_equal(s.xg[1:].cpu().numpy(), A.get_xg())
+    ra.configure()
