@@ -232,13 +232,16 @@ int ramses_refine_config(const ramses_refine_params *p);
 int ramses_amr_flag_fine(const ramses_amr_tree *t, const ramses_amr_level *lv, int icount, int levelmin,
                          int nlevelmax, const double *uold, int *flag1, int *flag2, void *stream);
 
-/* The free-grid list (headf/tailf/numbf of amr/amr_commons.f90): a device
- * ring of ngridmax grid indices holding the numbf free grids from index
- * `head` on, in the reference's free-list order. */
+/* The free-grid list (headf/tailf/numbf/used_mem of amr/amr_commons.f90):
+ * a device ring of ngridmax grid indices holding the numbf free grids from
+ * index `head` on, in the reference's free-list order; used_mem is
+ * ngridmax-numbf as of the last oct creation, as make_grid_fine leaves it
+ * (amr/refine_utils.f90:655-661). */
 typedef struct ramses_amr_free {
   int *ring;
   int head;
   int numbf;
+  int used_mem;
 } ramses_amr_free;
 
 /* Device workspace of ramses_amr_refine_fine for a level of noct octs whose

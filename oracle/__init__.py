@@ -215,6 +215,9 @@ class ReferenceAMR:
         lib.ref_amr_set_sedov.argtypes = [ctypes.c_double] * 6
         lib.ref_amr_set_init.argtypes = [ctypes.c_int]
         lib.ref_amr_init_flow_fine.argtypes = [ctypes.c_int]
+        lib.ref_amr_set_dt.argtypes = [_dp, _dp]
+        lib.ref_amr_get_used_mem.restype = ctypes.c_int
+        lib.ref_amr_dump.argtypes = [ctypes.c_char_p, ctypes.c_double, ctypes.c_int, ctypes.c_int]
         self.lib = lib
 
     def install(self, tree, nlevelmax=None):
@@ -307,6 +310,24 @@ class ReferenceAMR:
 
     def init_flow_fine(self, l):
         self.lib.ref_amr_init_flow_fine(l)
+
+    def set_dt(self, dtold, dtnew):
+        n = self.nlevelmax
+        a = np.zeros(n)
+        b = np.zeros(n)
+        a[:min(n, len(dtold))] = dtold[:n]
+        b[:min(n, len(dtnew))] = dtnew[:n]
+        self.lib.ref_amr_set_dt(_ptr(a), _ptr(b))
+
+    def defrag(self):
+        self.lib.ref_amr_defrag()
+
+    def used_mem(self):
+        return self.lib.ref_amr_get_used_mem()
+
+    def dump(self, directory, t, nstep=0, nstep_coarse=0):
+        """dump_all's snapshot files into an existing directory."""
+        self.lib.ref_amr_dump(directory.encode(), t, nstep, nstep_coarse)
 
     def tree_arrays(self):
         """(son, father, nbor, flag1) as 0-based copies of the 1-based arrays."""

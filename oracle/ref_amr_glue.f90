@@ -94,7 +94,7 @@ subroutine ref_amr_set_level(ilevel, ngrid, igrid_in) bind(C, name="ref_amr_set_
      active(ilevel)%igrid(i) = igrid_in(i)
   end do
   active(ilevel)%ngrid = ngrid
-  numbtot(1,ilevel) = ngrid
+  numbtot(1:4,ilevel) = ngrid   ! as refine_fine leaves them with one domain
   ! the level's linked list in the same order (amr/amr_commons.f90 headl/taill)
   numbl(1,ilevel) = ngrid
   headl(1,ilevel) = 0; taill(1,ilevel) = 0
@@ -126,6 +126,7 @@ subroutine ref_amr_set_free(nfree, igrid_in) bind(C, name="ref_amr_set_free")
   integer :: i
   numbf = nfree
   used_mem = ngridmax - numbf
+  used_mem_tot = used_mem
   headf = 0; tailf = 0
   if (nfree > 0) then
      headf = igrid_in(1)
@@ -436,3 +437,68 @@ subroutine ref_amr_init_flow_fine(ilevel) bind(C, name="ref_amr_init_flow_fine")
   il = ilevel
   call init_flow_fine(il)
 end subroutine ref_amr_init_flow_fine
+
+! Level time steps dtold(1:nlevelmax), dtnew(1:nlevelmax)
+subroutine ref_amr_set_dt(dto, dtn) bind(C, name="ref_amr_set_dt")
+  use iso_c_binding
+  use amr_commons
+  implicit none
+  real(c_double) :: dto(*), dtn(*)
+  integer :: i
+  do i = 1, nlevelmax
+     dtold(i) = dto(i)
+     dtnew(i) = dtn(i)
+  end do
+end subroutine ref_amr_set_dt
+
+subroutine ref_amr_defrag() bind(C, name="ref_amr_defrag")
+  implicit none
+  call defrag
+end subroutine ref_amr_defrag
+
+! The snapshot part of dump_all (amr/output_amr.f90:5-206) for one domain:
+! defrag, then info / amr / hydro files into the existing directory dir_c
+! (NUL-terminated) with the counters advanced as dump_all advances them.
+subroutine ref_amr_dump(dir_c, tnow, nstep_in, nstep_coarse_in) bind(C, name="ref_amr_dump")
+  use iso_c_binding
+  use amr_commons
+  use hydro_commons
+  implicit none
+  character(kind=c_char), dimension(*) :: dir_c
+  real(c_double), value :: tnow
+  integer(c_int), value :: nstep_in, nstep_coarse_in
+  character(len=80) :: filedir, filename, filename_desc
+  character(len=5) :: nchar
+  integer :: i
+  filedir = ' '
+  i = 1
+  do while (dir_c(i) /= c_null_char .and. i < 79)
+     filedir(i:i) = dir_c(i)
+     i = i + 1
+  end do
+  filedir(i:i) = '/'
+  t = tnow
+  nstep = nstep_in
+  nstep_coarse = nstep_coarse_in
+  call defrag
+  call flag_coarse
+  call title(ifout, nchar)
+  ifout = ifout + 1
+  if (t >= tout(iout) .or. aexp >= aout(iout)) iout = iout + 1
+  filename = TRIM(filedir)//'info_'//TRIM(nchar)//'.txt'
+  call output_info(filename)
+  filename = TRIM(filedir)//'amr_'//TRIM(nchar)//'.out'
+  call backup_amr(filename)
+  filename = TRIM(filedir)//'hydro_'//TRIM(nchar)//'.out'
+  filename_desc = TRIM(filedir)//'hydro_file_descriptor.txt'
+  call backup_hydro(filename, filename_desc)
+end subroutine ref_amr_dump
+
+! used_mem / used_mem_tot (amr/amr_commons.f90)
+function ref_amr_get_used_mem() bind(C, name="ref_amr_get_used_mem") result(n)
+  use iso_c_binding
+  use amr_commons
+  implicit none
+  integer(c_int) :: n
+  n = used_mem
+end function ref_amr_get_used_mem

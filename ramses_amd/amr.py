@@ -46,7 +46,8 @@ class ramses_refine_params(ctypes.Structure):
 
 
 class ramses_amr_free(ctypes.Structure):
-    _fields_ = [("ring", ctypes.c_void_p), ("head", ctypes.c_int), ("numbf", ctypes.c_int)]
+    _fields_ = [("ring", ctypes.c_void_p), ("head", ctypes.c_int), ("numbf", ctypes.c_int),
+                ("used_mem", ctypes.c_int)]
 
 
 def _bind(lib):
@@ -141,7 +142,7 @@ class AmrLevels:
         ring = np.zeros(ngm, dtype=np.int32)
         ring[:len(free)] = free
         self.ring = torch.as_tensor(ring, device=device)
-        self.free = ramses_amr_free(self.ring.data_ptr(), 0, len(free))
+        self.free = ramses_amr_free(self.ring.data_ptr(), 0, len(free), ngm - len(free))
         if u is None:
             self.uold = torch.zeros((5, tree.ncell + 1), dtype=torch.float64, device=device)
         else:
@@ -267,6 +268,52 @@ class AmrLevels:
         icell, x = self.cells(l)
         self.uold[:, icell] = ic(x * self.boxlen, 0.5 ** l * self.boxlen)
 
+    def defrag(self):
+        """defrag (amr/load_balance.f90:993-1608): renumber the grids 1..n in
+        level-list order (level 1 first), remap father / nbor / son, move xg,
+        flag1 and uold with their grids (positions past n up to the old
+        highest grid become zero), leave unew where it is, and make the free
+        list n+1..ngridmax.  The reference runs it before every snapshot."""
+        import torch
+        ngm, nc, dev = self.tree.ngridmax, 1, self.device
+        order = torch.cat([self.lists[l][:self.noct(l)] for l in range(1, self.nlevelmax + 1)]).long()
+        n2 = order.numel()
+        gmax = int(order.max().item())
+        new = torch.zeros(ngm + 1, dtype=torch.long, device=dev)
+        new[order] = torch.arange(1, n2 + 1, device=dev)
+
+        def remap(c):
+            g = torch.remainder(c - nc - 1, ngm) + 1
+            return torch.where(c > nc, nc + torch.div(c - nc - 1, ngm, rounding_mode="floor") * ngm + new[g], c)
+
+        def regrid(dst, base, vals):
+            buf = torch.zeros(gmax, dtype=dst.dtype, device=dev)
+            buf[:n2] = vals
+            dst[base + 1:base + gmax + 1] = buf
+
+        regrid(self.father, 0, remap(self.father[order].long()).to(self.father.dtype))
+        for j in range(6):
+            regrid(self.nbor, j * ngm, remap(self.nbor[j * ngm + order].long()).to(self.nbor.dtype))
+        xg = torch.zeros((gmax, 3), dtype=self.xg.dtype, device=dev)
+        xg[:n2] = self.xg[order]
+        self.xg[1:gmax + 1] = xg
+        for ind in range(8):
+            base = nc + ind * ngm
+            regrid(self.son, base, new[self.son[base + order].long()].to(self.son.dtype))
+            regrid(self.flag1, base, self.flag1[base + order])
+            for v in range(5):
+                regrid(self.uold[v], base, self.uold[v, base + order])
+        start = 0
+        for l in range(1, self.nlevelmax + 1):
+            n = self.noct(l)
+            ids = torch.arange(start + 1, start + n + 1, dtype=torch.int32, device=dev)
+            self.lists[l][:n] = ids
+            self.pos[start + 1:start + n + 1] = torch.arange(n, dtype=torch.int32, device=dev)
+            start += n
+        self.ring[:ngm - n2] = torch.arange(n2 + 1, ngm + 1, dtype=torch.int32, device=dev)
+        self.free.head = 0
+        self.free.numbf = ngm - n2
+
     # -- host views ------------------------------------------------------------
     def state(self, which="new"):
         t = self.unew if which == "new" else self.uold
@@ -276,6 +323,14 @@ class AmrLevels:
         """(son, father, nbor, flag1) as the reference's 1-based arrays, 0-based here."""
         return (self.son[1:].cpu().numpy(), self.father[1:].cpu().numpy(), self.nbor[1:].cpu().numpy(),
                 self.flag1[1:].cpu().numpy())
+
+    def host_mesh(self):
+        """The mesh as a snapshot.Mesh (host copies)."""
+        from .snapshot import Mesh
+        son, father, nbor, flag1 = self.tree_arrays()
+        return Mesh(self.tree.ngridmax, self.nlevelmax, {l: self.grids(l) for l in range(1, self.nlevelmax + 1)},
+                    son, father, nbor, flag1, self.xg[1:].cpu().numpy(), self.state("old"), self.free_list(),
+                    self.free.used_mem)
 
     def free_list(self):
         r = self.ring.cpu().numpy()
@@ -318,7 +373,8 @@ class AmrSolver(AmrLevels):
         self.adaptive = refine
         self.dtold = torch.zeros_like(self.dt)
         self.time = torch.zeros(1, dtype=torch.float64, device=self.device)
-        self.fine_steps = 0
+        self.fine_steps = 0        # nstep: finest-level update_time calls
+        self.nstep_coarse = 0
         self.cell_updates = 0
 
     def flag_all(self):
@@ -366,6 +422,7 @@ class AmrSolver(AmrLevels):
         amr_step(levelmin, 1), then with refinement on, upload_fine and
         flag_fine of the levels below levelmin."""
         self.amr_step(self.levelmin, 1)
+        self.nstep_coarse += 1
         if self.adaptive and self.levelmin < self.nlevelmax:
             for l in range(self.levelmin - 1, 0, -1):
                 self.upload(l)
@@ -403,6 +460,17 @@ class AmrSolver(AmrLevels):
             self.flag(l, icount, self.levelmin)
         if l > self.levelmin and icount == 2:
             self.dt[l - 1] = self.dtold[l] + self.dt[l]
+
+    def write_output(self, outdir, run):
+        """dump_all for the hydro mesh: defrag, then the snapshot files
+        (snapshot.write_output) with this solver's time and level dts."""
+        from . import snapshot
+        self.defrag()
+        run.t = float(self.time.item())
+        run.nstep, run.nstep_coarse = self.fine_steps, self.nstep_coarse
+        run.dtold = tuple(self.dtold[1:self.nlevelmax + 1].cpu().numpy())
+        run.dtnew = tuple(self.dt[1:self.nlevelmax + 1].cpu().numpy())
+        return snapshot.write_output(self.host_mesh(), run, outdir)
 
     def cell_updates_per_coarse_step(self):
         """sum over levels of 8*noct_l*nsubcycle^(l-levelmin) on the current tree."""

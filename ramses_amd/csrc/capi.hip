@@ -49,9 +49,9 @@ hipError_t launch_flag_fine(const int* active, int noct, const AmrTreeView& t, i
 size_t refine_work_bytes(int noct, int noct_fine);
 hipError_t launch_refine_fine(const int* active, int noct, int ilevel, const AmrTreeView& t, const TreeMut& m,
                               int* active_fine, int* noct_fine, int* pos, int* ring, int* head, int* numbf,
-                              int* flag1, int* flag2, double* xg, double* uold, double* unew, int interpol_type,
-                              int interpol_var, double smallr, void* work, int* d_error, int* nofree,
-                              hipStream_t s);
+                              int* used_mem, int* flag1, int* flag2, double* xg, double* uold, double* unew,
+                              int interpol_type, int interpol_var, double smallr, void* work, int* d_error,
+                              int* nofree, hipStream_t s);
 Tuning& tuning() {
   static Tuning t{0, 0, 0};
   return t;
@@ -417,7 +417,8 @@ int ramses_amr_refine_fine(const ramses_amr_tree* t, const ramses_amr_level* lv,
   int nofree = 0;
   if (lv->ilevel < 1) return fail(-1, "ilevel must be >= 1");
   hipError_t e = ramses::launch_refine_fine(lv->active, lv->noct, lv->ilevel, v, m, active_fine, noct_fine, pos,
-                                            fl->ring, &fl->head, &fl->numbf, flag1, flag2, xg, uold, unew,
+                                            fl->ring, &fl->head, &fl->numbf, &fl->used_mem, flag1, flag2, xg,
+                                            uold, unew,
                                             g_interpol_type, g_interpol_var, c.hp.smallr, work, d_error, &nofree,
                                             (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "refine_fine");

@@ -413,9 +413,9 @@ size_t refine_work_bytes(int noct, int noct_fine) {
 // cannot hold the new octs.
 hipError_t launch_refine_fine(const int* active, int noct, int ilevel, const AmrTreeView& t, const TreeMut& m,
                               int* active_fine, int* noct_fine, int* pos, int* ring, int* head, int* numbf,
-                              int* flag1, int* flag2, double* xg, double* uold, double* unew, int interpol_type,
-                              int interpol_var, double smallr, void* work, int* d_error, int* nofree,
-                              hipStream_t s) {
+                              int* used_mem, int* flag1, int* flag2, double* xg, double* uold, double* unew,
+                              int interpol_type, int interpol_var, double smallr, void* work, int* d_error,
+                              int* nofree, hipStream_t s) {
   *nofree = 0;
   if (noct <= 0) return hipSuccess;
   const long nslot = (long)((noct + 31) / 32) * 256;
@@ -488,6 +488,7 @@ hipError_t launch_refine_fine(const int* active, int noct, int ilevel, const Amr
   if ((e = hipGetLastError()) != hipSuccess) return e;
   *noct_fine = nalive + ncreate;
   *head = (int)(((long)*head + ncreate) % t.ngridmax);
+  if (ncreate > 0) *used_mem = t.ngridmax - (*numbf - ncreate);  // make_grid_fine's last value
   *numbf = *numbf - ncreate + nkill;
   return hipSuccess;
 }

@@ -360,3 +360,53 @@ def test_sedov_amr_init_and_steps_vs_reference(gpu):
                 assert got[l] == dt[l], (k, l)
     assert np.array_equal(s.xg[1:].cpu().numpy(), A.get_xg())
     ra.configure()
+
+
+@needs_ref
+def test_snapshot_after_adaptive_steps_vs_reference(gpu):
+    """Two adaptive coarse steps, then dump_all on both sides: the GPU
+    defrag + snapshot writer give the reference's files byte for byte, and
+    a further step after the defrag stays bitwise."""
+    import shutil
+    import tempfile
+    import ramses_amd as ra
+    from amr_cases import mesh_digests, reference_amr_step
+    from ramses_amd import snapshot
+    A, s, tree = _refine_setup(0.12, 0.25)
+    xg = s.xg[1:].cpu().numpy()
+    A.set_xg(xg)
+    for l in range(6, 0, -1):
+        if s.noct(l):
+            A.flag_fine(l, 2)
+    s.flag_all()
+    L = 6
+    dt, dtold = [0.0] * 8, [0.0] * 8
+    for _ in range(2):
+        reference_amr_step(A, 3, L, dt, dtold)
+        s.coarse_step()
+    s.check()
+    base = tempfile.mkdtemp(prefix="snap")
+    try:
+        os.makedirs(os.path.join(base, "ref"))
+        A.set_dt(dtold[1:L + 1], dt[1:L + 1])
+        t = float(s.time.item())
+        A.dump(os.path.join(base, "ref"), t, s.fine_steps, 2)
+        run = snapshot.RunInfo(levelmin=3, gamma=5 / 3, ordering="planar", bound_key=(-1.0, 1e30))
+        out = s.write_output(os.path.join(base, "ours"), run)
+        for name in ("info_00001.txt", "amr_00001.out00001", "hydro_00001.out00001", "hydro_file_descriptor.txt"):
+            assert open(os.path.join(out, name), "rb").read() == \
+                open(os.path.join(base, "ref", name), "rb").read(), name
+    finally:
+        shutil.rmtree(base)
+    reference_amr_step(A, 3, L, dt, dtold)
+    s.coarse_step()
+    s.check()
+    son, father, nbor, flag1 = s.tree_arrays()
+    rs, rf, rn, rfl = A.tree_arrays()
+    assert np.array_equal(son, rs) and np.array_equal(father, rf) and np.array_equal(nbor, rn)
+    assert np.array_equal(flag1, rfl)
+    for l in range(1, L + 1):
+        assert np.array_equal(s.grids(l), A.grids(l)), l
+    assert np.array_equal(s.free_list(), A.free_list())
+    assert np.array_equal(s.state("old"), A.get(0))
+    ra.configure()
