@@ -228,7 +228,9 @@ int ramses_refine_config(const ramses_refine_params *p);
  * hydro/godunov_utils.f90:125-263), nexpand more buffers, and for
  * ilevel > levelmin with icount < 2 the refinement rule (ensure_ref_rules).
  * flag1 and flag2 have ncoarse+8*ngridmax+1 entries, element 0 zero;
- * flag2 is workspace.  No-op at ilevel == nlevelmax (nsubcycle is 2). */
+ * flag2 is workspace.  No-op at ilevel == nlevelmax.  ensure_ref_rules
+ * runs for icount < 2, i.e. icount < nsubcycle(ilevel-1) with nsubcycle 2;
+ * a caller whose level ilevel-1 does not subcycle passes icount = 2. */
 int ramses_amr_flag_fine(const ramses_amr_tree *t, const ramses_amr_level *lv, int icount, int levelmin,
                          int nlevelmax, const double *uold, int *flag1, int *flag2, void *stream);
 

@@ -217,6 +217,11 @@ class ReferenceAMR:
         lib.ref_amr_init_flow_fine.argtypes = [ctypes.c_int]
         lib.ref_amr_set_dt.argtypes = [_dp, _dp]
         lib.ref_amr_get_used_mem.restype = ctypes.c_int
+        lib.ref_set_regions.argtypes = [ctypes.c_int, ip] + [_dp] * 12
+        lib.ref_condinit.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_double]
+        lib.ref_amr_set_nsubcycle.argtypes = [ctypes.c_int, ip]
+        lib.ref_amr_set_boxlen.argtypes = [ctypes.c_double]
+        lib.ref_amr_set_outputs.argtypes = [ctypes.c_int, _dp]
         lib.ref_amr_dump.argtypes = [ctypes.c_char_p, ctypes.c_double, ctypes.c_int, ctypes.c_int]
         self.lib = lib
 
@@ -324,6 +329,42 @@ class ReferenceAMR:
 
     def used_mem(self):
         return self.lib.ref_amr_get_used_mem()
+
+    def set_boxlen(self, b):
+        self.lib.ref_amr_set_boxlen(b)
+
+    def set_outputs(self, tout):
+        a = np.ascontiguousarray(tout, dtype=np.float64)
+        self.lib.ref_amr_set_outputs(len(a), _ptr(a))
+
+    def set_nsubcycle(self, ns):
+        a = np.ascontiguousarray(ns, dtype=np.int32)
+        self.lib.ref_amr_set_nsubcycle(len(a), a.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+
+    def set_regions(self, regions):
+        """namelist.Region-like objects (type, center, length, exp, d, u, p)."""
+        n = len(regions)
+        ip = ctypes.POINTER(ctypes.c_int)
+        rt = np.array([1 if r.type == "square" else 2 for r in regions], dtype=np.int32)
+        cols = [np.array([r.center[i] for r in regions]) for i in range(3)] + \
+               [np.array([r.length[i] for r in regions]) for i in range(3)] + \
+               [np.array([r.exp for r in regions]), np.array([r.d for r in regions])] + \
+               [np.array([r.u[i] for r in regions]) for i in range(3)] + [np.array([r.p for r in regions])]
+        self._reg_keep = [np.ascontiguousarray(c, dtype=np.float64) for c in cols]
+        self.lib.ref_set_regions(n, rt.ctypes.data_as(ip), *[_ptr(c) for c in self._reg_keep])
+
+    def condinit(self, x, dx):
+        """condinit on points x (N, 3) -> conserved (5, N)."""
+        n = x.shape[0]
+        out = np.zeros((NVAR, n))
+        for i0 in range(0, n, 32):
+            m = min(32, n - i0)
+            xb = np.zeros((3, 32))
+            xb[:, :m] = x[i0:i0 + m].T
+            ub = np.zeros((NVAR, 32))
+            self.lib.ref_condinit(m, _ptr(xb), _ptr(ub), dx)
+            out[:, i0:i0 + m] = ub[:, :m]
+        return out
 
     def dump(self, directory, t, nstep=0, nstep_coarse=0):
         """dump_all's snapshot files into an existing directory."""

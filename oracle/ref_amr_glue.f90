@@ -502,3 +502,83 @@ function ref_amr_get_used_mem() bind(C, name="ref_amr_get_used_mem") result(n)
   integer(c_int) :: n
   n = used_mem
 end function ref_amr_get_used_mem
+
+! Regions of &INIT_PARAMS (hydro/hydro_parameters.f90, amr/amr_parameters.f90):
+! rtype(k) = 1 'square', 2 'point'; arrays of length n.
+subroutine ref_set_regions(n, rtype, xc, yc, zc, lx, ly, lz, ex, d, u, v, w, p) bind(C, name="ref_set_regions")
+  use iso_c_binding
+  use amr_commons
+  use hydro_commons
+  implicit none
+  integer(c_int), value :: n
+  integer(c_int) :: rtype(*)
+  real(c_double) :: xc(*), yc(*), zc(*), lx(*), ly(*), lz(*), ex(*), d(*), u(*), v(*), w(*), p(*)
+  integer :: k
+  nregion = n
+  do k = 1, n
+     if (rtype(k) == 1) then
+        region_type(k) = 'square'
+     else
+        region_type(k) = 'point'
+     end if
+     x_center(k) = xc(k); y_center(k) = yc(k); z_center(k) = zc(k)
+     length_x(k) = lx(k); length_y(k) = ly(k); length_z(k) = lz(k)
+     exp_region(k) = ex(k)
+     d_region(k) = d(k); u_region(k) = u(k); v_region(k) = v(k); w_region(k) = w(k); p_region(k) = p(k)
+  end do
+end subroutine ref_set_regions
+
+! condinit (hydro/condinit.f90) on nn <= nvector points: x(nvector,3) in,
+! uu(nvector,nvar) out, both column-major.
+subroutine ref_condinit(nn, x, uu, dx) bind(C, name="ref_condinit")
+  use iso_c_binding
+  use amr_parameters
+  use hydro_parameters
+  implicit none
+  integer(c_int), value :: nn
+  real(c_double) :: x(nvector,3), uu(nvector,nvar)
+  real(c_double), value :: dx
+  real(dp) :: xx(nvector,ndim), u(nvector,nvar), dxl
+  integer :: n
+  xx = x(:,1:ndim)
+  dxl = dx
+  n = nn
+  u = 0
+  call condinit(xx, u, dxl, n)
+  uu = u
+end subroutine ref_condinit
+
+! nsubcycle(1:n) of &RUN_PARAMS (amr/amr_commons.f90:51)
+subroutine ref_amr_set_nsubcycle(n, ns) bind(C, name="ref_amr_set_nsubcycle")
+  use iso_c_binding
+  use amr_commons
+  implicit none
+  integer(c_int), value :: n
+  integer(c_int) :: ns(*)
+  integer :: i
+  do i = 1, n
+     nsubcycle(i) = ns(i)
+  end do
+end subroutine ref_amr_set_nsubcycle
+
+subroutine ref_amr_set_boxlen(b) bind(C, name="ref_amr_set_boxlen")
+  use iso_c_binding
+  use amr_parameters
+  implicit none
+  real(c_double), value :: b
+  boxlen = b
+end subroutine ref_amr_set_boxlen
+
+! noutput and tout(1:noutput) of &OUTPUT_PARAMS (amr/amr_parameters.f90:107-119)
+subroutine ref_amr_set_outputs(n, t_out) bind(C, name="ref_amr_set_outputs")
+  use iso_c_binding
+  use amr_parameters
+  implicit none
+  integer(c_int), value :: n
+  real(c_double) :: t_out(*)
+  integer :: i
+  noutput = n
+  do i = 1, n
+     tout(i) = t_out(i)
+  end do
+end subroutine ref_amr_set_outputs

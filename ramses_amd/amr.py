@@ -346,9 +346,9 @@ class AmrLevels:
 
 
 class AmrSolver(AmrLevels):
-    """amr_step (amr/amr_step.f90) with nsubcycle=2: per coarse step the
-    levels are advanced recursively, the fine levels first, in the
-    reference's order of operations:
+    """amr_step (amr/amr_step.f90) with nsubcycle 1 or 2 per level: per
+    coarse step the levels are advanced recursively, the fine levels first,
+    in the reference's order of operations:
 
       refine_fine(i), i = l..nlevelmax, at l == levelmin or icount > 1  :40-93
       newdt_fine(l) -> dt_l = min(courant_fine(l), dt_{l-1}/nsubcycle)  :327-330
@@ -364,11 +364,16 @@ class AmrSolver(AmrLevels):
     """
 
     def __init__(self, tree, u, nsubcycle=2, refine=False, **kw):
+        """nsubcycle: 1 or 2 for every level, or a list per level (index
+        0 is level 1), as &RUN_PARAMS nsubcycle (amr/amr_commons.f90:51)."""
         super().__init__(tree, u, **kw)
         import torch
-        if nsubcycle != 2:
-            raise ValueError("nsubcycle must be 2")
+        nsub = [nsubcycle] * (self.nlevelmax + 1) if isinstance(nsubcycle, int) else list(nsubcycle)
+        nsub = (nsub + [2] * (self.nlevelmax + 1))[:self.nlevelmax + 1]
+        if any(n not in (1, 2) for n in nsub):
+            raise ValueError("nsubcycle must be 1 or 2 per level")
         self.levelmin = tree.levelmin
+        self.nsub = {l: nsub[l - 1] for l in range(1, self.nlevelmax + 1)}
         self.nsubcycle = nsubcycle
         self.adaptive = refine
         self.dtold = torch.zeros_like(self.dt)
@@ -441,15 +446,16 @@ class AmrSolver(AmrLevels):
         self.dtold[l] = self.dt[l]
         self.courant(l, self.boxlen / _config_cache.get("smallc", 1e-10))
         if l > self.levelmin:
-            self.dt[l] = torch.minimum(self.dt[l - 1] / float(self.nsubcycle), self.dt[l])
+            self.dt[l] = torch.minimum(self.dt[l - 1] / float(self.nsub[l - 1]), self.dt[l])
         self.set_unew(l)
         if l < self.nlevelmax and self.noct(l + 1) > 0:
             self.amr_step(l + 1, 1)
-            self.amr_step(l + 1, 2)
+            if self.nsub[l] == 2:
+                self.amr_step(l + 1, 2)
         else:
             if l < self.nlevelmax:
-                self.dtold[l + 1] = self.dt[l] / float(self.nsubcycle)
-                self.dt[l + 1] = self.dt[l] / float(self.nsubcycle)
+                self.dtold[l + 1] = self.dt[l] / float(self.nsub[l])
+                self.dt[l + 1] = self.dt[l] / float(self.nsub[l])
             self.time += self.dt[l]        # update_time (amr/update_time.f90:217,331)
             self.fine_steps += 1
         self.godunov(l)
@@ -457,9 +463,14 @@ class AmrSolver(AmrLevels):
         self.set_uold(l)
         self.upload(l)
         if self.adaptive:
-            self.flag(l, icount, self.levelmin)
-        if l > self.levelmin and icount == 2:
-            self.dt[l - 1] = self.dtold[l] + self.dt[l]
+            # ensure_ref_rules runs for icount < nsubcycle(l-1) (amr/flag_utils.f90:94-98)
+            ic = icount if (l <= self.levelmin or self.nsub[l - 1] == 2) else 2
+            self.flag(l, ic, self.levelmin)
+        if l > self.levelmin:
+            if self.nsub[l - 1] == 1:
+                self.dt[l - 1] = self.dt[l]
+            if icount == 2:
+                self.dt[l - 1] = self.dtold[l] + self.dt[l]
 
     def write_output(self, outdir, run):
         """dump_all for the hydro mesh: defrag, then the snapshot files
@@ -473,6 +484,9 @@ class AmrSolver(AmrLevels):
         return snapshot.write_output(self.host_mesh(), run, outdir)
 
     def cell_updates_per_coarse_step(self):
-        """sum over levels of 8*noct_l*nsubcycle^(l-levelmin) on the current tree."""
-        return sum(8 * self.noct(l) * self.nsubcycle ** (l - self.levelmin)
-                   for l in range(self.levelmin, self.nlevelmax + 1))
+        """sum over levels of 8*noct_l*prod(nsubcycle of the coarser levels)."""
+        tot, mult = 0, 1
+        for l in range(self.levelmin, self.nlevelmax + 1):
+            tot += 8 * self.noct(l) * mult
+            mult *= self.nsub[l]
+        return tot

@@ -56,45 +56,15 @@ def reference_coarse_step(A, tree, levelmin, dt, dtold, nsub=2):
         if l > levelmin and icount == 2:
             dt[l - 1] = dtold[l] + dt[l]
     step(levelmin, 1)
-    if adaptive and levelmin < nlevelmax:  # amr/adaptive_loop.f90:143-191
-        for l in range(levelmin - 1, 0, -1):
-            A.upload(l)
-        for l in range(levelmin - 1, 0, -1):
-            A.flag_fine(l, 2)
 
 
-def reference_init_refine(A, levelmin, nlevelmax):
-    """init_refine + init_refine_2 (amr/init_refine.f90) on a tree installed
-    up to levelmin, with the reference's own init_flow / flag_fine /
-    refine_fine; init is set during the first phase as the reference sets
-    it (make_grid_fine then leaves new cells to init_flow)."""
-    def n(l):
-        return len(A.grids(l))
-    A.set_init(True)
-    for il in range(levelmin + 1, nlevelmax + 1):
-        A.init_flow()
-        for l in range(nlevelmax - 1, 0, -1):
-            A.flag_fine(l, 2)
-        for l in range(1, nlevelmax):
-            A.refine_fine(l)
-        if n(il) == 0:
-            break
-    A.set_init(False)
-    A.init_flow()
-    for _ in range(levelmin, nlevelmax + 2):
-        for l in range(1, nlevelmax + 1):
-            A.refine_fine(l)
-            A.init_flow_fine(l)
-        for l in range(nlevelmax, levelmin - 1, -1):
-            A.upload(l)
-        for l in range(nlevelmax, 0, -1):
-            A.flag_fine(l, 2)
-
-
-def reference_amr_step(A, levelmin, nlevelmax, dt, dtold, adaptive=True, smallc=1e-10, boxlen=1.0):
+def reference_amr_step(A, levelmin, nlevelmax, dt, dtold, adaptive=True, smallc=1e-10, boxlen=1.0, nsub=None):
     """One coarse step of amr_step (amr/amr_step.f90) driving the
     reference's compiled routines, with refinement (refine_fine at
-    levelmin / second substeps, flag_fine after each level step)."""
+    levelmin / second substeps, flag_fine after each level step).
+    nsub: {level: 1 or 2} (default 2 everywhere)."""
+    ns = nsub or {}
+
     def n(l):
         return len(A.grids(l)) if 1 <= l <= nlevelmax else 0
 
@@ -107,21 +77,25 @@ def reference_amr_step(A, levelmin, nlevelmax, dt, dtold, adaptive=True, smallc=
         dtold[l] = dt[l]
         dt[l] = A.courant(l, boxlen / smallc)[0]
         if l > levelmin:
-            dt[l] = min(dt[l - 1] / 2.0, dt[l])
+            dt[l] = min(dt[l - 1] / float(ns.get(l - 1, 2)), dt[l])
         A.set_unew(l)
         if l < nlevelmax and n(l + 1) > 0:
             step(l + 1, 1)
-            step(l + 1, 2)
+            if ns.get(l, 2) == 2:
+                step(l + 1, 2)
         elif l < nlevelmax:
-            dtold[l + 1] = dt[l] / 2.0
-            dt[l + 1] = dt[l] / 2.0
+            dtold[l + 1] = dt[l] / float(ns.get(l, 2))
+            dt[l + 1] = dt[l] / float(ns.get(l, 2))
         A.godunov(l, dt[l])
         A.set_uold(l)
         A.upload(l)
         if adaptive:
             A.flag_fine(l, icount)
-        if l > levelmin and icount == 2:
-            dt[l - 1] = dtold[l] + dt[l]
+        if l > levelmin:
+            if ns.get(l - 1, 2) == 1:
+                dt[l - 1] = dt[l]
+            if icount == 2:
+                dt[l - 1] = dtold[l] + dt[l]
     step(levelmin, 1)
     if adaptive and levelmin < nlevelmax:  # amr/adaptive_loop.f90:143-191
         for l in range(levelmin - 1, 0, -1):

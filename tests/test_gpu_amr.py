@@ -345,8 +345,18 @@ def test_sedov_amr_init_and_steps_vs_reference(gpu):
                                 s.state("old"), s.state("new"))
         return mesh_digests(son, father, nbor, flag1, {l: A.grids(l) for l in range(1, L + 1)}, A.free_list(),
                             A.get(0), A.get(1))
-    assert snap(s) == snap(A)
     assert np.array_equal(s.xg[1:].cpu().numpy(), A.get_xg())
+    go, ro = s.state("old"), A.get(0)
+    bad = np.nonzero((go != ro).any(0))[0]
+    if len(bad):
+        ngm = tree.ngridmax
+        lev = s.xg.new_zeros(1)  # noqa: F841
+        grid = (bad - 1) % ngm + 1
+        levels = {l: int(np.isin(grid, s.grids(l)).sum()) for l in range(1, L + 1)}
+        k = bad[:4]
+        raise AssertionError(f"uold differs in {len(bad)} cells, per level {levels}; "
+                             f"cells {k.tolist()}: gpu {go[:, k].T.tolist()} ref {ro[:, k].T.tolist()}")
+    assert snap(s) == snap(A)
     assert s.noct(L) > 0
     dt, dtold = [0.0] * (L + 2), [0.0] * (L + 2)
     for k in range(3):
@@ -410,3 +420,149 @@ def test_snapshot_after_adaptive_steps_vs_reference(gpu):
     assert np.array_equal(s.free_list(), A.free_list())
     assert np.array_equal(s.state("old"), A.get(0))
     ra.configure()
+
+
+@needs_ref
+def test_sedov_amr_no_subcycling_vs_reference(gpu):
+    """nsubcycle=1 on every level (as namelist/sedov3d.nml sets it): one
+    step per level per coarse step, the coarse dt taken from the finest
+    level; init_refine and three coarse steps bitwise."""
+    import ramses_amd as ra
+    from amr_cases import mesh_digests, reference_amr_step, reference_init_refine
+    from ramses_amd.amr import AmrSolver, configure_interpolation, configure_refinement
+    from ramses_amd.ics import sedov_regions
+    from ramses_amd.tree import OctTree
+    lmin, L = 3, 6
+    tree = OctTree(lmin, lmin, extra_grids=20000)
+    R = oracle.Reference()
+    A = oracle.ReferenceAMR(R)
+    R.set_params(gamma=1.4, riemann="hllc", slope_type=1, courant_factor=0.8)
+    ra.configure(gamma=1.4, riemann="hllc", slope_type=1, courant_factor=0.8)
+    A.install(tree, nlevelmax=L)
+    A.set_interpol(1, 0)
+    A.set_refine(lmin, err_grad_d=0.1, err_grad_p=0.1)
+    A.set_nsubcycle([1] * 10)
+    configure_interpolation(1, 0)
+    configure_refinement(err_grad_d=0.1, err_grad_p=0.1)
+    s = AmrSolver(tree, None, refine=True, nlevelmax=L, nsubcycle=1)
+    A.set_xg(s.xg[1:].cpu().numpy())
+    A.set_sedov(1.0, 1e-5, 0.4, (0.5, 0.5, 0.5))
+    reference_init_refine(A, lmin, L)
+    s.init_refine(lambda x, dx: sedov_regions(x, dx, gamma=1.4))
+    dt, dtold = [0.0] * (L + 2), [0.0] * (L + 2)
+    ns = {l: 1 for l in range(1, L + 1)}
+    for k in range(3):
+        reference_amr_step(A, lmin, L, dt, dtold, nsub=ns)
+        s.coarse_step()
+        s.check()
+        son, father, nbor, flag1 = s.tree_arrays()
+        rs, rf, rn, rfl = A.tree_arrays()
+        assert np.array_equal(son, rs) and np.array_equal(flag1, rfl), k
+        for l in range(1, L + 1):
+            assert np.array_equal(s.grids(l), A.grids(l)), (k, l)
+        assert np.array_equal(s.state("old"), A.get(0)), k
+        got = s.dt.cpu().numpy()
+        for l in range(lmin, L + 1):
+            if s.noct(l):
+                assert got[l] == dt[l], (k, l)
+    A.set_nsubcycle([2] * 10)
+    ra.configure()
+
+
+SMALL_SEDOV_NML = """&RUN_PARAMS
+hydro=.true.
+nsubcycle=2,2,2,1,1,1
+nstepmax=2
+/
+&AMR_PARAMS
+levelmin=3
+levelmax=6
+ngridtot=20000
+nexpand=1
+boxlen=0.5
+/
+&INIT_PARAMS
+nregion=2
+region_type(1)='square'
+region_type(2)='point'
+x_center=0.25,0.25
+y_center=0.25,0.25
+z_center=0.25,0.25
+length_x=10.0,1.0
+length_y=10.0,1.0
+length_z=10.0,1.0
+exp_region=10.0,10.0
+d_region=1.0,0.0
+p_region=1e-5,0.4
+/
+&OUTPUT_PARAMS
+noutput=1
+tout=1.0
+/
+&HYDRO_PARAMS
+gamma=1.4
+courant_factor=0.8
+riemann='hllc'
+slope_type=1
+/
+&REFINE_PARAMS
+err_grad_d=0.1
+err_grad_p=0.1
+interpol_type=1
+/
+"""
+
+
+@needs_ref
+def test_namelist_run_matches_reference(gpu):
+    """A parameter file run end to end (ramses_amd.run: namelist, region
+    initial conditions in a box of length 0.5, init_refine, two adaptive
+    coarse steps with mixed subcycling, snapshot) against the reference's
+    own routines driven through the same sequence: the four snapshot files
+    are byte-identical."""
+    import shutil
+    import tempfile
+    import ramses_amd as ra
+    from amr_cases import reference_amr_step, reference_init_refine
+    from ramses_amd import namelist, run
+    from ramses_amd.tree import OctTree
+    base = tempfile.mkdtemp(prefix="nml")
+    try:
+        path = os.path.join(base, "p.nml")
+        with open(path, "w") as f:
+            f.write(SMALL_SEDOV_NML)
+        s, outs = run.run(path, outdir=os.path.join(base, "o"), log=lambda *a: None)
+        p = namelist.Params.from_namelist(path)
+        L, lmin = p.levelmax, p.levelmin
+        R = oracle.Reference()
+        A = oracle.ReferenceAMR(R)
+        R.set_params(gamma=p.gamma, riemann=p.riemann, slope_type=p.slope_type, courant_factor=p.courant_factor)
+        tree = OctTree(lmin, lmin, extra_grids=p.ngridmax - sum(8 ** k for k in range(lmin)))
+        A.install(tree, nlevelmax=L)
+        A.set_boxlen(p.boxlen)
+        A.set_interpol(p.interpol_type, p.interpol_var)
+        A.set_refine(lmin, err_grad_d=p.err_grad_d, err_grad_p=p.err_grad_p)
+        A.set_nsubcycle(p.nsubcycle[:10])
+        A.set_outputs(p.tout)
+        xg = np.zeros((tree.ngridmax, 3))
+        for (l, gx, gy, gz), i in tree.grid_at.items():
+            xg[i - 1] = (np.array([gx, gy, gz]) + 0.5) * 0.5 ** (l - 1)
+        A.set_xg(xg)
+        A.set_regions(p.regions)
+        reference_init_refine(A, lmin, L)
+        dt, dtold = [0.0] * (L + 2), [0.0] * (L + 2)
+        ns = {l: p.nsubcycle[l - 1] for l in range(1, L + 1)}
+        for _ in range(p.nstepmax):
+            reference_amr_step(A, lmin, L, dt, dtold, boxlen=p.boxlen, nsub=ns)
+        A.set_dt(dtold[1:L + 1], dt[1:L + 1])
+        os.makedirs(os.path.join(base, "ref"))
+        A.dump(os.path.join(base, "ref"), float(s.time.item()), s.fine_steps, s.nstep_coarse)
+        out = outs[-1]
+        for name in ("info_00001.txt", "amr_00001.out00001", "hydro_00001.out00001", "hydro_file_descriptor.txt"):
+            assert open(os.path.join(out, name), "rb").read() == \
+                open(os.path.join(base, "ref", name), "rb").read(), name
+        A.set_nsubcycle([2] * 10)
+        A.set_outputs([])
+    finally:
+        shutil.rmtree(base)
+        ra.configure()
