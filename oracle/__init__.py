@@ -278,6 +278,9 @@ class ReferenceAMR:
     def flag_fine(self, l, icount):
         self.lib.ref_amr_flag_fine(l, icount)
 
+    def flag_coarse(self):
+        self.lib.ref_amr_flag_coarse()
+
     def refine_fine(self, l):
         self.lib.ref_amr_refine_fine(l)
 

@@ -582,3 +582,8 @@ subroutine ref_amr_set_outputs(n, t_out) bind(C, name="ref_amr_set_outputs")
      tout(i) = t_out(i)
   end do
 end subroutine ref_amr_set_outputs
+
+subroutine ref_amr_flag_coarse() bind(C, name="ref_amr_flag_coarse")
+  implicit none
+  call flag_coarse
+end subroutine ref_amr_flag_coarse

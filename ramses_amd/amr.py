@@ -425,7 +425,7 @@ class AmrSolver(AmrLevels):
     def coarse_step(self):
         """One coarse step of adaptive_loop (amr/adaptive_loop.f90:86-191):
         amr_step(levelmin, 1), then with refinement on, upload_fine and
-        flag_fine of the levels below levelmin."""
+        flag_fine of the levels below levelmin and flag_coarse."""
         self.amr_step(self.levelmin, 1)
         self.nstep_coarse += 1
         if self.adaptive and self.levelmin < self.nlevelmax:
@@ -433,6 +433,7 @@ class AmrSolver(AmrLevels):
                 self.upload(l)
             for l in range(self.levelmin - 1, 0, -1):
                 self.flag(l, 2, self.levelmin)
+            self.flag1[1:2] = 1             # flag_coarse (amr/flag_utils.f90:22-52)
 
     def amr_step(self, l, icount):
         import torch

@@ -102,6 +102,7 @@ def reference_amr_step(A, levelmin, nlevelmax, dt, dtold, adaptive=True, smallc=
             A.upload(l)
         for l in range(levelmin - 1, 0, -1):
             A.flag_fine(l, 2)
+        A.flag_coarse()
 
 
 def reference_init_refine(A, levelmin, nlevelmax):
