@@ -278,7 +278,7 @@ int ramses_amr_refine_fine(const ramses_amr_tree *t, const ramses_amr_level *lv,
  * (ramses_amd/autotune.py).  value 0 restores the built-in default.
  * --------------------------------------------------------------------- */
 #define RAMSES_TUNE_UNSPLIT_BLOCK 1 /* threads per unsplit workgroup: multiple of 64, <= 768 */
-#define RAMSES_TUNE_LEVEL_TILE 2    /* level tile: 1 = 32x8, 2 = 16x8, 3 = 8x8 columns */
+#define RAMSES_TUNE_LEVEL_TILE 2    /* level tile: 1 = 32x8, 2 = 16x8, 3 = 8x8, 4 = 32x16 columns */
 #define RAMSES_TUNE_LEVEL_TZ 3      /* z planes per level workgroup: a divisor of n, >= 4 */
 
 /* Set one tuning knob (amr/amr_parameters.f90:49-53 is the reference's

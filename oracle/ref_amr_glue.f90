@@ -26,6 +26,8 @@ subroutine ref_amr_init(nlevelmax_in, ngridmax_in) bind(C, name="ref_amr_init")
   ncoarse = nx*ny*nz
   ncpu = 1; myid = 1
   hydro = .true.       ! set by read_params for the hydro solver
+  ifout = 1; iout = 1; noutput = 0; t = 0; nstep = 0; nstep_coarse = 0
+  nsubcycle = 2
   nlevelmax = nlevelmax_in
   ngridmax = ngridmax_in
   ncell = ncoarse + twotondim*ngridmax

@@ -62,7 +62,7 @@ def tune_level(run, n, key, verbose=False):
         _set(LEVEL_TILE, tile)
         _set(LEVEL_TZ, tz)
         return _cache[key]
-    tiles = [t for t, w in ((1, 32), (2, 16), (3, 8)) if n % w == 0]
+    tiles = [t for t, w in ((1, 32), (2, 16), (3, 8), (4, 32)) if n % w == 0]
     tzs = [0] + [z for z in (8, 16, 32, 64, 128) if n % z == 0 and z < n]
     best = None
     for t in tiles:

@@ -458,6 +458,7 @@ hipError_t launch_uniform_godunov(const double* u, double* unew, int n, double d
   if (tile == 1 && n % 32 == 0) { TX = 32; TY = 8; }
   else if (tile == 2 && n % 16 == 0) { TX = 16; TY = 8; }
   else if (tile == 3) { TX = 8; TY = 8; }
+  else if (tile == 4 && n % 32 == 0) { TX = 32; TY = 16; }
   else if (n % 32 == 0) { TX = 32; TY = 8; }
   else if (n % 16 == 0) { TX = 16; TY = 8; }
   else { TX = 8; TY = 8; }
@@ -467,10 +468,12 @@ hipError_t launch_uniform_godunov(const double* u, double* unew, int n, double d
   const int tz = tuning().level_tz;
   if (tz >= 4 && tz <= n && n % tz == 0) TZ = tz;
   if (fp32) {
+    if (TX == 32 && TY == 16) return launch_tile<float, 32, 16>(a, TZ, s);
     if (TX == 32) return launch_tile<float, 32, 8>(a, TZ, s);
     if (TX == 16) return launch_tile<float, 16, 8>(a, TZ, s);
     return launch_tile<float, 8, 8>(a, TZ, s);
   }
+  if (TX == 32 && TY == 16) return launch_tile<double, 32, 16>(a, TZ, s);
   if (TX == 32) return launch_tile<double, 32, 8>(a, TZ, s);
   if (TX == 16) return launch_tile<double, 16, 8>(a, TZ, s);
   return launch_tile<double, 8, 8>(a, TZ, s);

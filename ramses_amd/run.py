@@ -41,13 +41,15 @@ def build(params, precision="fp64", device="cuda"):
     return s
 
 
-def run(path, outdir=".", precision="fp64", max_coarse=None, log=print):
+def run(path, outdir=".", precision="fp64", max_coarse=None, log=print, ordering="hilbert", bound_key=None):
+    """ordering / bound_key: the domain decomposition the snapshot records
+    (one domain; the Hilbert key range by default)."""
     from . import snapshot
     from .namelist import Params
     p = Params.from_namelist(path)
     s = build(p, precision)
     info = snapshot.RunInfo(levelmin=p.levelmin, gamma=p.gamma, smallr=p.smallr, boxlen=p.boxlen,
-                            noutput=len(p.tout), tout=tuple(p.tout))
+                            noutput=len(p.tout), tout=tuple(p.tout), ordering=ordering, bound_key=bound_key)
     t_end = p.tout[-1] if p.tout else float("inf")
     nmax = p.nstepmax if max_coarse is None else min(p.nstepmax, max_coarse)
     t0 = time.perf_counter()

@@ -170,8 +170,9 @@ def write_amr(mesh, run, path):
     r.rec(_i4(ngrid))
     r.rec(_f8(run.boxlen))
     r.rec(_i4(run.noutput, run.iout, run.ifout))
-    r.rec(_f8(*run.tout[:run.noutput]))
-    r.rec(_f8(*run.aout[:run.noutput]))
+    huge = np.finfo(np.float64).max   # default of tout / aout (amr/amr_parameters.f90:118-119)
+    r.rec(_f8(*(list(run.tout) + [huge] * run.noutput)[:run.noutput]))
+    r.rec(_f8(*(list(run.aout) + [huge] * run.noutput)[:run.noutput]))
     r.rec(_f8(run.t))
     r.rec(_f8(*dtold))
     r.rec(_f8(*dtnew))
