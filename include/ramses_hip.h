@@ -270,6 +270,50 @@ int ramses_amr_refine_fine(const ramses_amr_tree *t, const ramses_amr_level *lv,
                            int *pos, ramses_amr_free *fl, int *flag1, int *flag2, double *xg, double *uold,
                            double *unew, void *work, unsigned long work_bytes, int *d_error, void *stream);
 
+/* ---------------------------------------------------------------------------
+ * One coarse step of the main loop on a device-resident mesh.
+ * ------------------------------------------------------------------------- */
+#define RAMSES_MAX_LEVELS 32
+
+/* The whole mesh of amr/amr_commons.f90 + hydro/hydro_commons.f90 for one
+ * domain.  Device arrays as in ramses_amr_tree / ramses_amr_level (element 0
+ * unused); active[l] has room for ngridmax grids; dt, dtold hold dtnew(l)
+ * and dtold(l) at index l (RAMSES_MAX_LEVELS+2 entries); time is t.
+ * noct, free, the counters and work are updated by the library; work is a
+ * device workspace the library grows as needed (start with NULL / 0 and
+ * release it with ramses_amr_mesh_release). */
+typedef struct ramses_amr_mesh {
+  ramses_amr_tree tree;
+  int *active[RAMSES_MAX_LEVELS + 1];
+  int noct[RAMSES_MAX_LEVELS + 1];
+  int *pos;
+  double *xg;
+  int *flag1, *flag2;
+  ramses_amr_free free;
+  double *uold, *unew;
+  double *dt, *dtold, *time;
+  int *error;
+  int levelmin, nlevelmax;
+  int nsubcycle[RAMSES_MAX_LEVELS + 1]; /* 1 or 2 for level l at index l */
+  int adaptive;                         /* refine and flag during the step */
+  double boxlen;
+  long long nstep, nstep_coarse, cell_updates;
+  void *work;
+  unsigned long work_bytes;
+} ramses_amr_mesh;
+
+/* amr_step(levelmin, 1) (amr/amr_step.f90) followed by the end of a coarse
+ * step of adaptive_loop (amr/adaptive_loop.f90:143-191: upload_fine and
+ * flag_fine below levelmin, flag_coarse): per level, refine_fine of the
+ * finer levels (at levelmin and on second substeps), newdt_fine /
+ * courant_fine, set_unew, the finer levels' substeps, godunov_fine,
+ * set_uold, upload_fine and flag_fine, with every dt kept on the device.
+ * nstep counts finest-level steps (amr/update_time.f90:332). */
+int ramses_amr_coarse_step(ramses_amr_mesh *m, void *stream);
+
+/* Frees the workspace a mesh's coarse steps allocated. */
+int ramses_amr_mesh_release(ramses_amr_mesh *m);
+
 /* ---------------------------------------------------------------------
  * Launch-shape tuning.  The reference's only performance knob is the
  * compile-time vector length nvector (amr/amr_parameters.f90:49-53, set by

@@ -50,6 +50,21 @@ class ramses_amr_free(ctypes.Structure):
                 ("used_mem", ctypes.c_int)]
 
 
+MAX_LEVELS = 32  # RAMSES_MAX_LEVELS
+
+
+class ramses_amr_mesh(ctypes.Structure):
+    _fields_ = [("tree", ramses_amr_tree), ("active", ctypes.c_void_p * (MAX_LEVELS + 1)),
+                ("noct", ctypes.c_int * (MAX_LEVELS + 1)), ("pos", ctypes.c_void_p), ("xg", ctypes.c_void_p),
+                ("flag1", ctypes.c_void_p), ("flag2", ctypes.c_void_p), ("free", ramses_amr_free),
+                ("uold", ctypes.c_void_p), ("unew", ctypes.c_void_p), ("dt", ctypes.c_void_p),
+                ("dtold", ctypes.c_void_p), ("time", ctypes.c_void_p), ("error", ctypes.c_void_p),
+                ("levelmin", ctypes.c_int), ("nlevelmax", ctypes.c_int),
+                ("nsubcycle", ctypes.c_int * (MAX_LEVELS + 1)), ("adaptive", ctypes.c_int),
+                ("boxlen", ctypes.c_double), ("nstep", ctypes.c_longlong), ("nstep_coarse", ctypes.c_longlong),
+                ("cell_updates", ctypes.c_longlong), ("work", ctypes.c_void_p), ("work_bytes", ctypes.c_ulong)]
+
+
 def _bind(lib):
     if getattr(lib, "_amr_bound", False):
         return
@@ -67,6 +82,8 @@ def _bind(lib):
         "ramses_amr_flag_fine": [T, L, ci, ci, ci, vp, vp, vp, vp],
         "ramses_amr_refine_work_bytes": [ci, ci],
         "ramses_amr_refine_fine": [T, L, vp, P(ci), vp, P(ramses_amr_free), vp, vp, vp, vp, vp, vp, ul, vp, vp],
+        "ramses_amr_coarse_step": [P(ramses_amr_mesh), vp],
+        "ramses_amr_mesh_release": [P(ramses_amr_mesh)],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -106,6 +123,8 @@ class AmrLevels:
         self.boxlen = boxlen
         self.device = device
         self.nlevelmax = max(tree.nlevel, tree.levelmax if nlevelmax is None else nlevelmax)
+        if self.nlevelmax > MAX_LEVELS - 1:
+            raise ValueError(f"at most {MAX_LEVELS - 1} levels")
         ngm = tree.ngridmax
 
         def dev_int(a):  # element 0 unused so that p[i] is the reference's element i
@@ -154,7 +173,7 @@ class AmrLevels:
         self.flag2 = torch.zeros_like(self.flag1)
         self.work = torch.empty(0, dtype=torch.uint8, device=device)
         self.error = torch.zeros(1, dtype=torch.int32, device=device)
-        self.dt = torch.zeros(self.nlevelmax + 2, dtype=torch.float64, device=device)
+        self.dt = torch.zeros(MAX_LEVELS + 2, dtype=torch.float64, device=device)
 
     # -- bookkeeping ---------------------------------------------------------
     def noct(self, l):
@@ -381,6 +400,13 @@ class AmrSolver(AmrLevels):
         self.fine_steps = 0        # nstep: finest-level update_time calls
         self.nstep_coarse = 0
         self.cell_updates = 0
+        self._mesh = ramses_amr_mesh()
+
+    def __del__(self):
+        try:
+            self.lib.ramses_amr_mesh_release(ctypes.byref(self._mesh))
+        except Exception:
+            pass
 
     def flag_all(self):
         """The initial refinement map: flag_fine(l, 2) from the finest level
@@ -424,54 +450,31 @@ class AmrSolver(AmrLevels):
 
     def coarse_step(self):
         """One coarse step of adaptive_loop (amr/adaptive_loop.f90:86-191):
-        amr_step(levelmin, 1), then with refinement on, upload_fine and
-        flag_fine of the levels below levelmin and flag_coarse."""
-        self.amr_step(self.levelmin, 1)
-        self.nstep_coarse += 1
-        if self.adaptive and self.levelmin < self.nlevelmax:
-            for l in range(self.levelmin - 1, 0, -1):
-                self.upload(l)
-            for l in range(self.levelmin - 1, 0, -1):
-                self.flag(l, 2, self.levelmin)
-            self.flag1[1:2] = 1             # flag_coarse (amr/flag_utils.f90:22-52)
-
-    def amr_step(self, l, icount):
-        import torch
-        if self.noct(l) == 0:
-            return
-        if self.adaptive and self.levelmin < self.nlevelmax and (l == self.levelmin or icount > 1):
-            for i in range(l, self.nlevelmax + 1):
-                self.refine(i)
-        # newdt_fine (pm/newdt_fine.f90:51, hydro/courant_fine.f90):
-        # dtold = dtnew, then dtnew = courant_fine starting from boxlen/smallc
-        self.dtold[l] = self.dt[l]
-        self.courant(l, self.boxlen / _config_cache.get("smallc", 1e-10))
-        if l > self.levelmin:
-            self.dt[l] = torch.minimum(self.dt[l - 1] / float(self.nsub[l - 1]), self.dt[l])
-        self.set_unew(l)
-        if l < self.nlevelmax and self.noct(l + 1) > 0:
-            self.amr_step(l + 1, 1)
-            if self.nsub[l] == 2:
-                self.amr_step(l + 1, 2)
-        else:
-            if l < self.nlevelmax:
-                self.dtold[l + 1] = self.dt[l] / float(self.nsub[l])
-                self.dt[l + 1] = self.dt[l] / float(self.nsub[l])
-            self.time += self.dt[l]        # update_time (amr/update_time.f90:217,331)
-            self.fine_steps += 1
-        self.godunov(l)
-        self.cell_updates += 8 * self.noct(l)
-        self.set_uold(l)
-        self.upload(l)
-        if self.adaptive:
-            # ensure_ref_rules runs for icount < nsubcycle(l-1) (amr/flag_utils.f90:94-98)
-            ic = icount if (l <= self.levelmin or self.nsub[l - 1] == 2) else 2
-            self.flag(l, ic, self.levelmin)
-        if l > self.levelmin:
-            if self.nsub[l - 1] == 1:
-                self.dt[l - 1] = self.dt[l]
-            if icount == 2:
-                self.dt[l - 1] = self.dtold[l] + self.dt[l]
+        amr_step(levelmin, 1) and, with refinement on, upload_fine and
+        flag_fine below levelmin and flag_coarse -- run by the library's
+        native driver (ramses_amr_coarse_step) in one call."""
+        m = self._mesh
+        m.tree = self.t
+        for l in range(1, self.nlevelmax + 1):
+            m.active[l] = self.lists[l].data_ptr()
+            m.noct[l] = self.noct(l)
+            m.nsubcycle[l] = self.nsub[l]
+        m.pos, m.xg = self.pos.data_ptr(), self.xg.data_ptr()
+        m.flag1, m.flag2 = self.flag1.data_ptr(), self.flag2.data_ptr()
+        m.free = self.free
+        m.uold, m.unew = self.uold.data_ptr(), self.unew.data_ptr()
+        m.dt, m.dtold, m.time = self.dt.data_ptr(), self.dtold.data_ptr(), self.time.data_ptr()
+        m.error = self.error.data_ptr()
+        m.levelmin, m.nlevelmax = self.levelmin, self.nlevelmax
+        m.adaptive = 1 if self.adaptive else 0
+        m.boxlen = self.boxlen
+        m.nstep, m.nstep_coarse, m.cell_updates = self.fine_steps, self.nstep_coarse, self.cell_updates
+        rc = self.lib.ramses_amr_coarse_step(ctypes.byref(m), self._s())
+        for l in range(1, self.nlevelmax + 1):
+            self.nocts[l] = m.noct[l]
+        self.free.head, self.free.numbf, self.free.used_mem = m.free.head, m.free.numbf, m.free.used_mem
+        self.fine_steps, self.nstep_coarse, self.cell_updates = m.nstep, m.nstep_coarse, m.cell_updates
+        _check(rc, "ramses_amr_coarse_step")
 
     def write_output(self, outdir, run):
         """dump_all for the hydro mesh: defrag, then the snapshot files

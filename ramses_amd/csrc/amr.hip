@@ -81,47 +81,52 @@ __device__ __forceinline__ size_t sslot(int i3, int j3, int k3) {
 }
 
 // Flux reset at faces touching a refined cell, then the level update
-// unew += flux(i3) - flux(i3+1) per dimension.  Thread = one oct.
-__global__ void amr_update(AmrLevelArgs a) {
-  const int oct = blockIdx.x * blockDim.x + threadIdx.x;
-  if (oct >= a.noct) return;
+// unew += flux(i3) - flux(i3+1) per dimension.  Workgroup = one nvector
+// batch: first one thread per (face, lane) resets fluxes, then one thread
+// per (cell, variable, lane) accumulates the three dimensions in order.
+__global__ __launch_bounds__(256) void amr_update(AmrLevelArgs a) {
   const int nv = a.nv;
-  const int b = oct / nv, lane = oct % nv;
+  const int b = blockIdx.x;
+  const int nlane = min(nv, a.noct - b * nv);
   const AmrTreeView& t = a.t;
-  const int igrid = a.active[oct];
   const size_t fvs = (size_t)nv * 27;
-  double* flux = a.flux + (size_t)b * fvs * 15 + lane;
-  const unsigned char* ok = a.ok + (size_t)b * nv * 216 + lane;
-  for (int d = 0; d < 3; ++d) {
+  double* flux = a.flux + (size_t)b * fvs * 15;
+  const unsigned char* ok = a.ok + (size_t)b * nv * 216;
+  for (int it = threadIdx.x; it < 36 * nv; it += blockDim.x) {
+    const int lane = it % nv, f = it / nv;
+    if (lane >= nlane) continue;
+    const int d = f / 12, r = f % 12;
     const int i0 = d == 0, j0 = d == 1, k0 = d == 2;
-    for (int k3 = 1; k3 <= 2 + k0; ++k3)
-      for (int j3 = 1; j3 <= 2 + j0; ++j3)
-        for (int i3 = 1; i3 <= 2 + i0; ++i3) {
-          if (ok[sslot(i3 - i0, j3 - j0, k3 - k0) * nv] || ok[sslot(i3, j3, k3) * nv]) {
+    const int ni = 2 + i0, nj = 2 + j0;
+    const int i3 = 1 + r % ni, j3 = 1 + (r / ni) % nj, k3 = 1 + r / (ni * nj);
+    if (ok[sslot(i3 - i0, j3 - j0, k3 - k0) * nv + lane] || ok[sslot(i3, j3, k3) * nv + lane]) {
 #pragma unroll
-            for (int v = 0; v < 5; ++v) flux[(fslot(i3, j3, k3) + (size_t)27 * (v + 5 * d)) * nv] = 0.0;
-          }
-        }
-  }
-  for (int d = 0; d < 3; ++d) {
-    const int i0 = d == 0, j0 = d == 1, k0 = d == 2;
-    for (int ind = 1; ind <= 8; ++ind) {
-      const int i2 = (ind - 1) & 1, j2 = ((ind - 1) >> 1) & 1, k2 = (ind - 1) >> 2;
-      const int i3 = 1 + i2, j3 = 1 + j2, k3 = 1 + k2;
-      const long c = cell_of(t, ind, igrid);
-#pragma unroll
-      for (int v = 0; v < 5; ++v) {
-        const size_t base = (size_t)27 * (v + 5 * d);
-        const double fl = flux[(fslot(i3, j3, k3) + base) * nv];
-        const double fr = flux[(fslot(i3 + i0, j3 + j0, k3 + k0) + base) * nv];
-        a.unew[(long)v * a.ustride + c] = a.unew[(long)v * a.ustride + c] + (fl - fr);
-      }
+      for (int v = 0; v < 5; ++v) flux[(fslot(i3, j3, k3) + (size_t)27 * (v + 5 * d)) * nv + lane] = 0.0;
     }
+  }
+  __syncthreads();
+  for (int it = threadIdx.x; it < 40 * nv; it += blockDim.x) {
+    const int lane = it % nv, q = it / nv;
+    if (lane >= nlane) continue;
+    const int ind = q / 5 + 1, v = q % 5;
+    const int i2 = (ind - 1) & 1, j2 = ((ind - 1) >> 1) & 1, k2 = (ind - 1) >> 2;
+    const int i3 = 1 + i2, j3 = 1 + j2, k3 = 1 + k2;
+    const long c = (long)v * a.ustride + cell_of(t, ind, a.active[b * nv + lane]);
+    double acc = a.unew[c];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int i0 = d == 0, j0 = d == 1, k0 = d == 2;
+      const size_t base = (size_t)27 * (v + 5 * d);
+      const double fl = flux[(fslot(i3, j3, k3) + base) * nv + lane];
+      const double fr = flux[(fslot(i3 + i0, j3 + j0, k3 + k0) + base) * nv + lane];
+      acc = acc + (fl - fr);
+    }
+    a.unew[c] = acc;
   }
 }
 
 // Conservative correction of the coarser leaf cells next to the level's
-// octs.  Thread = one cell of a coarse-level oct.  A coarse leaf c receives
+// octs.  Thread = (one cell of a coarse-level oct, one variable).  A coarse leaf c receives
 // from each fine oct o that sits across one of its faces the four (reset)
 // fluxes of o's boundary faces on c's side, times -1/8 (c left of o) or
 // +1/8 (c right of o).  The reference adds them in loop order: batch of o,
@@ -129,15 +134,18 @@ __global__ void amr_update(AmrLevelArgs a) {
 // rebuilt here from the position of o in the level's active list.
 __global__ void amr_reflux(AmrLevelArgs a) {
   const long tix = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tix >= 8L * a.noct_coarse) return;
+  if (tix >= 40L * a.noct_coarse) return;
+  const long cidx = tix / 5;
+  const int v = (int)(tix % 5);
   const AmrTreeView& t = a.t;
-  const int g = a.active_coarse[tix >> 3];
-  const int ind = (int)(tix & 7) + 1;
+  const int g = a.active_coarse[cidx >> 3];
+  const int ind = (int)(cidx & 7) + 1;
   const long c = cell_of(t, ind, g);
   if (t.son[c] != 0) return;  // only leaves receive the correction (:811,863)
   const int p[3] = {(ind - 1) & 1, ((ind - 1) >> 1) & 1, (ind - 1) >> 2};
   // up to 6 contributing octs: key = batch*6 + 2*d + side
   int keys[6], octs[6], ne = 0;
+#pragma unroll
   for (int in = 1; in <= 6; ++in) {
     const int ax = (in - 1) >> 1, plus = (in - 1) & 1;
     const bool inside = plus ? (p[ax] == 0) : (p[ax] == 1);
@@ -164,7 +172,7 @@ __global__ void amr_reflux(AmrLevelArgs a) {
   const int nv = a.nv;
   const size_t fvs = (size_t)nv * 27;
   const double oneontwotondim = 1.0 / 8.0;
-  for (int v = 0; v < 5; ++v) {
+  {
     double s = a.unew[(long)v * a.ustride + c];
     for (int e = 0; e < ne; ++e) {
       const int d = (keys[e] % 6) >> 1, side = keys[e] & 1;
@@ -240,6 +248,40 @@ __global__ void amr_courant(const int* active, int noct, AmrTreeView t, const do
   }
 }
 
+// The level time-step bookkeeping of amr_step (amr/amr_step.f90), one
+// thread, so a coarse step keeps every dt on the device:
+//   op 0  newdt_fine prologue: dtold(l) = dtnew(l); dtnew(l) = start, capped
+//         by dtnew(l-1)/nsubcycle(l-1) below levelmin (:327-330; the courant
+//         reduction then min-combines into it -- min is exact, so the order
+//         of the three minima does not matter)
+//   op 1  no finer level: dtold(l+1) = dtnew(l+1) = dtnew(l)/nsubcycle(l),
+//         t += dtnew(l) (:353-362, amr/update_time.f90:217,331)
+//   op 2  end of the step: dtnew(l-1) = dtnew(l) if nsubcycle(l-1) == 1, or
+//         dtold(l) + dtnew(l) after the second substep (:573-576)
+__global__ void amr_dt_kernel(int op, double* dt, double* dtold, double* time, int l, int coarser, int nsub,
+                              int icount, double start) {
+  if (op == 0) {
+    dtold[l] = dt[l];
+    double v = start;
+    if (coarser) v = fmin_f(dt[l - 1] / (double)nsub, v);
+    dt[l] = v;
+  } else if (op == 1) {
+    if (coarser) {  // here: a finer level index exists
+      dtold[l + 1] = dt[l] / (double)nsub;
+      dt[l + 1] = dt[l] / (double)nsub;
+    }
+    time[0] = time[0] + dt[l];
+  } else {
+    if (nsub == 1) dt[l - 1] = dt[l];
+    if (icount == 2) dt[l - 1] = dtold[l] + dt[l];
+  }
+}
+
+__global__ void set_ints(int* p, int n, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 }  // namespace
 
 hipError_t launch_unsplit(const UnsplitArgs& a, bool fp32, hipStream_t stream);
@@ -266,11 +308,11 @@ hipError_t launch_amr_godunov(const AmrLevelArgs& a, double dx, const double* d_
   u.hp = a.hp;
   e = launch_unsplit(u, fp32, s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(amr_update, dim3((a.noct + 127) / 128), dim3(128), 0, s, a);
+  hipLaunchKernelGGL(amr_update, dim3(nbatch), dim3(256), 0, s, a);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (a.noct_coarse > 0 && a.active_coarse) {
-    const long n = 8L * a.noct_coarse;
+    const long n = 40L * a.noct_coarse;
     hipLaunchKernelGGL(amr_reflux, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
   }
   return hipGetLastError();
@@ -300,6 +342,18 @@ hipError_t launch_amr_courant(const int* active, int noct, const AmrTreeView& t,
   const long n = 8L * noct;
   hipLaunchKernelGGL(amr_courant, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, active, noct, t, u, ustride,
                      dx, hp, d_dt);
+  return hipGetLastError();
+}
+
+hipError_t launch_amr_dt(int op, double* dt, double* dtold, double* time, int l, int coarser, int nsub, int icount,
+                         double start, hipStream_t s) {
+  hipLaunchKernelGGL(amr_dt_kernel, dim3(1), dim3(1), 0, s, op, dt, dtold, time, l, coarser, nsub, icount, start);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_ints(int* p, int n, int v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(set_ints, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
   return hipGetLastError();
 }
 

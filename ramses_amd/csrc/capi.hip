@@ -33,6 +33,9 @@ hipError_t launch_amr_upload(const int* active, int noct, const AmrTreeView& t, 
                              double smallr, hipStream_t s);
 hipError_t launch_amr_courant(const int* active, int noct, const AmrTreeView& t, const double* u, long ustride,
                               double dx, const HydroParams& hp, double* d_dt, hipStream_t s);
+hipError_t launch_amr_dt(int op, double* dt, double* dtold, double* time, int l, int coarser, int nsub, int icount,
+                         double start, hipStream_t s);
+hipError_t launch_set_ints(int* p, int n, int v, hipStream_t s);
 struct RefineParams {
   double err_grad_d, err_grad_u, err_grad_p;
   double floor_d, floor_u, floor_p;
@@ -423,6 +426,147 @@ int ramses_amr_refine_fine(const ramses_amr_tree* t, const ramses_amr_level* lv,
                                             (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "refine_fine");
   if (nofree) return fail(-9, "No more free memory: increase ngridmax");
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// amr_step (amr/amr_step.f90) over a device-resident mesh, host recursion.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct StepCtx {
+  ramses_amr_mesh* m;
+  hipStream_t s;
+  ramses::HydroParams hp;
+};
+
+ramses_amr_level level_of(const ramses_amr_mesh* m, int l) {
+  ramses_amr_level lv;
+  lv.ilevel = l;
+  lv.dx = __builtin_ldexp(1.0, -l) * m->boxlen;
+  lv.active = m->active[l];
+  lv.noct = m->noct[l];
+  lv.pos = m->pos;
+  lv.active_coarse = (l > 1 && m->noct[l - 1] > 0) ? m->active[l - 1] : nullptr;
+  lv.noct_coarse = (l > 1) ? m->noct[l - 1] : 0;
+  return lv;
+}
+
+int ensure_work(ramses_amr_mesh* m, size_t need, hipStream_t s) {
+  if (m->work && m->work_bytes >= need) return 0;
+  hipError_t e;
+  if (m->work) {
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "coarse step sync");
+    (void)hipFree(m->work);
+    m->work = nullptr;
+    m->work_bytes = 0;
+  }
+  const size_t bytes = need + need / 4 + 4096;
+  if ((e = hipMalloc(&m->work, bytes)) != hipSuccess) return hip_fail(e, "coarse step workspace");
+  m->work_bytes = bytes;
+  return 0;
+}
+
+int amr_step_rec(StepCtx& c, int l, int icount) {
+  ramses_amr_mesh* m = c.m;
+  if (m->noct[l] == 0) return 0;
+  int rc;
+  if (m->adaptive && m->levelmin < m->nlevelmax && (l == m->levelmin || icount > 1)) {
+    for (int i = l; i < m->nlevelmax; ++i) {
+      if (m->noct[i] == 0) continue;
+      const size_t need = ramses::refine_work_bytes(m->noct[i], m->noct[i + 1]);
+      if ((rc = ensure_work(m, need, c.s))) return rc;
+      ramses_amr_level lv = level_of(m, i);
+      rc = ramses_amr_refine_fine(&m->tree, &lv, m->active[i + 1], &m->noct[i + 1], m->pos, &m->free, m->flag1,
+                                  m->flag2, m->xg, m->uold, m->unew, m->work, m->work_bytes, m->error, c.s);
+      if (rc) return rc;
+    }
+  }
+  ramses_amr_level lv = level_of(m, l);
+  ramses::AmrTreeView v;
+  if ((rc = amr_view(&m->tree, v))) return rc;
+  // newdt_fine: dtold = dtnew, dtnew = min(boxlen/smallc, courant_factor*dx/smallc,
+  // dtnew(l-1)/nsubcycle(l-1)), then the courant reduction over the leaves
+  double start = c.hp.courant_factor * lv.dx / c.hp.smallc;
+  const double cap = m->boxlen / c.hp.smallc;
+  if (cap < start) start = cap;
+  const int coarser = l > m->levelmin;
+  hipError_t e = ramses::launch_amr_dt(0, m->dt, m->dtold, m->time, l, coarser, coarser ? m->nsubcycle[l - 1] : 1,
+                                       0, start, c.s);
+  if (e != hipSuccess) return hip_fail(e, "dt launch");
+  e = ramses::launch_amr_courant(lv.active, lv.noct, v, m->uold, v.ncell + 1, lv.dx, c.hp, m->dt + l, c.s);
+  if (e != hipSuccess) return hip_fail(e, "courant launch");
+  if ((rc = ramses_amr_copy(&m->tree, &lv, m->uold, m->unew, c.s))) return rc;  // set_unew
+  if (l < m->nlevelmax && m->noct[l + 1] > 0) {
+    if ((rc = amr_step_rec(c, l + 1, 1))) return rc;
+    if (m->nsubcycle[l] == 2 && (rc = amr_step_rec(c, l + 1, 2))) return rc;
+  } else {
+    e = ramses::launch_amr_dt(1, m->dt, m->dtold, m->time, l, l < m->nlevelmax, m->nsubcycle[l], 0, 0.0, c.s);
+    if (e != hipSuccess) return hip_fail(e, "dt launch");
+    m->nstep += 1;
+  }
+  lv = level_of(m, l);  // the finer levels' refinements changed noct(l+1) only
+  if ((rc = ensure_work(m, amr_work(lv.noct, current().nvector), c.s))) return rc;
+  rc = ramses_amr_godunov(&m->tree, &lv, m->uold, m->unew, m->dt + l, m->work, m->work_bytes, m->error, c.s);
+  if (rc) return rc;
+  m->cell_updates += 8LL * lv.noct;
+  if ((rc = ramses_amr_copy(&m->tree, &lv, m->unew, m->uold, c.s))) return rc;  // set_uold
+  if ((rc = ramses_amr_upload(&m->tree, &lv, m->uold, c.s))) return rc;
+  if (m->adaptive) {
+    // ensure_ref_rules runs for icount < nsubcycle(l-1) (amr/flag_utils.f90:94-98)
+    const int ic = (l <= m->levelmin || m->nsubcycle[l - 1] == 2) ? icount : 2;
+    rc = ramses_amr_flag_fine(&m->tree, &lv, ic, m->levelmin, m->nlevelmax, m->uold, m->flag1, m->flag2, c.s);
+    if (rc) return rc;
+  }
+  if (l > m->levelmin) {
+    e = ramses::launch_amr_dt(2, m->dt, m->dtold, m->time, l, 1, m->nsubcycle[l - 1], icount, 0.0, c.s);
+    if (e != hipSuccess) return hip_fail(e, "dt launch");
+  }
+  return 0;
+}
+
+}  // namespace
+
+int ramses_amr_coarse_step(ramses_amr_mesh* m, void* stream) {
+  if (!m) return fail(-1, "null mesh");
+  if (m->levelmin < 1 || m->nlevelmax < m->levelmin || m->nlevelmax > RAMSES_MAX_LEVELS - 1)
+    return fail(-1, "levels out of range");
+  if (!m->uold || !m->unew || !m->dt || !m->dtold || !m->time || !m->error || !m->pos || !m->flag1 || !m->flag2)
+    return fail(-1, "null mesh array");
+  for (int l = 1; l <= m->nlevelmax; ++l) {
+    if (m->noct[l] < 0 || m->noct[l] > m->tree.ngridmax) return fail(-1, "noct out of range");
+    if (m->noct[l] > 0 && !m->active[l]) return fail(-1, "null grid list");
+    if (m->nsubcycle[l] != 1 && m->nsubcycle[l] != 2) return fail(-2, "nsubcycle must be 1 or 2");
+  }
+  if (!(m->boxlen > 0)) return fail(-1, "boxlen must be > 0");
+  StepCtx c{m, (hipStream_t)stream, current().hp};
+  int rc = amr_step_rec(c, m->levelmin, 1);
+  if (rc) return rc;
+  m->nstep_coarse += 1;
+  if (m->adaptive && m->levelmin < m->nlevelmax) {
+    for (int l = m->levelmin - 1; l >= 1; --l) {
+      ramses_amr_level lv = level_of(m, l);
+      if ((rc = ramses_amr_upload(&m->tree, &lv, m->uold, c.s))) return rc;
+    }
+    for (int l = m->levelmin - 1; l >= 1; --l) {
+      ramses_amr_level lv = level_of(m, l);
+      rc = ramses_amr_flag_fine(&m->tree, &lv, 2, m->levelmin, m->nlevelmax, m->uold, m->flag1, m->flag2, c.s);
+      if (rc) return rc;
+    }
+    hipError_t e = ramses::launch_set_ints(m->flag1 + 1, m->tree.ncoarse, 1, c.s);  // flag_coarse
+    if (e != hipSuccess) return hip_fail(e, "flag_coarse launch");
+  }
+  return 0;
+}
+
+int ramses_amr_mesh_release(ramses_amr_mesh* m) {
+  if (!m) return 0;
+  if (m->work) {
+    hipError_t e = hipFree(m->work);
+    m->work = nullptr;
+    m->work_bytes = 0;
+    if (e != hipSuccess) return hip_fail(e, "hipFree");
+  }
   return 0;
 }
 

@@ -148,19 +148,15 @@ __global__ void flag_hydro(const int* active, int noct, AmrTreeView t, const dou
   if (ok) flag1[c] = 1;
 }
 
-// Thread = one grid: unflag all its cells unless the 27 neighbours of its
-// father cell exist and are refined.
+// Thread = (grid, father neighbour j of 27): a missing or unrefined
+// neighbour unflags all cells of the grid (every such thread writes the
+// same zeros), as ensure_ref_rules does for a grid with any such neighbour.
 __global__ void flag_ensure(const int* active, int noct, AmrTreeView t, int ilevel, int* flag1) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= noct) return;
-  const int g = active[a];
-  const long f = t.father[g];
-  bool ok = true;
-  for (int j = 0; j < 27; ++j) {
-    const long nf = nbor_father_cell(t, f, j % 3, (j / 3) % 3, j / 9, ilevel);
-    if (nf == 0 || t.son[nf] == 0) ok = false;
-  }
-  if (!ok)
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 27L * noct) return;
+  const int g = active[i / 27], j = (int)(i % 27);
+  const long nf = nbor_father_cell(t, t.father[g], j % 3, (j / 3) % 3, j / 9, ilevel);
+  if (nf == 0 || t.son[nf] == 0)
     for (int ind = 1; ind <= 8; ++ind) flag1[cell_of(t, ind, g)] = 0;
 }
 
@@ -393,7 +389,7 @@ hipError_t launch_flag_fine(const int* active, int noct, const AmrTreeView& t, i
     hipLaunchKernelGGL(flag_hydro, dim3(nb), dim3(256), 0, s, active, noct, t, u, us, flag1, rp, hp);
   for (int e = 0; e < rp.nexpand; ++e) smooth();
   if (ilevel > levelmin && icount < 2)
-    hipLaunchKernelGGL(flag_ensure, dim3(nblk(noct, 128)), dim3(128), 0, s, active, noct, t, ilevel, flag1);
+    hipLaunchKernelGGL(flag_ensure, dim3(nblk(27L * noct, 256)), dim3(256), 0, s, active, noct, t, ilevel, flag1);
   return hipGetLastError();
 }
 

@@ -52,3 +52,23 @@ def test_product_does_not_import_oracle():
             if f.endswith((".py", ".hip", ".hpp", ".cpp", ".h")):
                 text = open(os.path.join(root, f)).read()
                 assert "import oracle" not in text and not re.search(r"\bhydro_ref\b|libramses_ref", text), f
+
+
+def test_mesh_struct_layout_matches_header(tmp_path):
+    """The ctypes mirror of ramses_amr_mesh has the C layout of the header."""
+    import shutil
+    import subprocess
+    from ramses_amd import amr
+    if not shutil.which("gcc"):
+        pytest.skip("no host C compiler")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = tmp_path / "m.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "include/ramses_hip.h"\n'
+                   'int main(){printf("%zu %zu %zu %zu %zu\\n", sizeof(ramses_amr_mesh), offsetof(ramses_amr_mesh, free),'
+                   ' offsetof(ramses_amr_mesh, levelmin), offsetof(ramses_amr_mesh, boxlen),'
+                   ' offsetof(ramses_amr_mesh, work));return 0;}\n')
+    exe = tmp_path / "m"
+    subprocess.run(["gcc", "-I", root, str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    M = amr.ramses_amr_mesh
+    assert got == [ctypes.sizeof(M), M.free.offset, M.levelmin.offset, M.boxlen.offset, M.work.offset]

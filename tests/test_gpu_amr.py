@@ -303,7 +303,7 @@ def test_adaptive_mesh_golden(gpu):
         s.coarse_step()
         s.check()
         assert snap() == gold["stages"][f"step{k + 1}"], k
-        got = s.dt.cpu().numpy()
+        got = s.dt.cpu().numpy()[:L + 2]
         assert [float(x).hex() for x in got] == gold["dts"][k], k
     ra.configure()
 
