@@ -225,6 +225,35 @@ __global__ void amr_upload(const int* active, int noct, AmrTreeView t, double* u
   for (int v = 0; v < 5; ++v) u[(long)v * ustride + c] = out[v];
 }
 
+// set_uold followed by upload_fine of the level in one pass
+// (hydro/godunov_fine.f90:135-232, hydro/interpol_hydro.f90:5-263): a leaf
+// takes unew, a refined cell the average of its children's uold (set_uold
+// writes every cell and upload_fine then rewrites the refined ones, so the
+// result is the same).
+__global__ void amr_finish(const int* active, int noct, AmrTreeView t, const double* unew, double* u, long ustride,
+                           double smallr) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 8L * noct) return;
+  const int igrid = active[i / 8];
+  const long c = cell_of(t, (int)(i % 8) + 1, igrid);
+  const int s = t.son[c];
+  if (s <= 0) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) u[(long)v * ustride + c] = unew[(long)v * ustride + c];
+    return;
+  }
+  double getx = 0.0;
+  for (int ind = 1; ind <= 8; ++ind) getx = getx + fmax_f(u[cell_of(t, ind, s)], smallr);
+  double out[5];
+  out[0] = getx / 8.0;
+  for (int v = 1; v < 5; ++v) {
+    getx = 0.0;
+    for (int ind = 1; ind <= 8; ++ind) getx = getx + u[(long)v * ustride + cell_of(t, ind, s)];
+    out[v] = getx / 8.0;
+  }
+  for (int v = 0; v < 5; ++v) u[(long)v * ustride + c] = out[v];
+}
+
 // courant_fine over the level's leaf cells (hydro/courant_fine.f90:44-131):
 // block max of the cmpdt wave speed, block dt limit min-combined (exact,
 // see cmpdt_w).
@@ -342,6 +371,15 @@ hipError_t launch_amr_courant(const int* active, int noct, const AmrTreeView& t,
   const long n = 8L * noct;
   hipLaunchKernelGGL(amr_courant, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, active, noct, t, u, ustride,
                      dx, hp, d_dt);
+  return hipGetLastError();
+}
+
+hipError_t launch_amr_finish(const int* active, int noct, const AmrTreeView& t, const double* unew, double* u,
+                             long ustride, double smallr, hipStream_t s) {
+  if (noct <= 0) return hipSuccess;
+  const long n = 8L * noct;
+  hipLaunchKernelGGL(amr_finish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, active, noct, t, unew, u,
+                     ustride, smallr);
   return hipGetLastError();
 }
 

@@ -36,6 +36,8 @@ hipError_t launch_amr_courant(const int* active, int noct, const AmrTreeView& t,
 hipError_t launch_amr_dt(int op, double* dt, double* dtold, double* time, int l, int coarser, int nsub, int icount,
                          double start, hipStream_t s);
 hipError_t launch_set_ints(int* p, int n, int v, hipStream_t s);
+hipError_t launch_amr_finish(const int* active, int noct, const AmrTreeView& t, const double* unew, double* u,
+                             long ustride, double smallr, hipStream_t s);
 struct RefineParams {
   double err_grad_d, err_grad_u, err_grad_p;
   double floor_d, floor_u, floor_p;
@@ -510,8 +512,8 @@ int amr_step_rec(StepCtx& c, int l, int icount) {
   rc = ramses_amr_godunov(&m->tree, &lv, m->uold, m->unew, m->dt + l, m->work, m->work_bytes, m->error, c.s);
   if (rc) return rc;
   m->cell_updates += 8LL * lv.noct;
-  if ((rc = ramses_amr_copy(&m->tree, &lv, m->unew, m->uold, c.s))) return rc;  // set_uold
-  if ((rc = ramses_amr_upload(&m->tree, &lv, m->uold, c.s))) return rc;
+  e = ramses::launch_amr_finish(lv.active, lv.noct, v, m->unew, m->uold, v.ncell + 1, c.hp.smallr, c.s);
+  if (e != hipSuccess) return hip_fail(e, "set_uold/upload launch");
   if (m->adaptive) {
     // ensure_ref_rules runs for icount < nsubcycle(l-1) (amr/flag_utils.f90:94-98)
     const int ic = (l <= m->levelmin || m->nsubcycle[l - 1] == 2) ? icount : 2;

@@ -5,8 +5,8 @@
 //
 // flag_fine      flag_init   test_flag: a refined cell stays flagged while a
 //                            child is refined or flagged (:168-208)
-//                flag_count  smooth_fine's neighbour count into flag2
-//                flag_merge  ... and the merge back into flag1 (:555-677)
+//                flag_smooth one Jacobi pass of smooth_fine (:555-677),
+//                            flag1 and flag2 used as ping-pong buffers
 //                flag_hydro  hydro_flag / hydro_refine gradient criteria
 //                            (hydro/hydro_flag.f90:1-180,
 //                            hydro/godunov_utils.f90:125-263)
@@ -78,23 +78,29 @@ __global__ void flag_init(const int* active, int noct, AmrTreeView t, int* flag1
   flag1[c] = f;
 }
 
-__global__ void flag_count(const int* active, int noct, AmrTreeView t, const int* flag1, int* flag2, int nthr) {
+// One pass of smooth_fine's dilation (amr/flag_utils.f90:555-677): a cell
+// is flagged when it was, or when at least nthr of its six neighbours at
+// the level were (flag1[0] == 0 stands for a missing neighbour).  The
+// reference counts into flag2 and merges into flag1; reading `in` and
+// writing `out` gives the same Jacobi pass in one launch.
+__global__ void flag_smooth(const int* active, int noct, AmrTreeView t, const int* in, int* out, int nthr) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   int g, ind;
   long c;
   if (!level_cell(active, noct, t, i, g, ind, c)) return;
   int n = 0;
-  for (int in = 1; in <= 6; ++in) n += flag1[nbor_cell(t, g, ind, in)];  // flag1[0] == 0
-  flag2[c] = n >= nthr ? 1 : 0;
+#pragma unroll
+  for (int nb = 1; nb <= 6; ++nb) n += in[nbor_cell(t, g, ind, nb)];
+  out[c] = (in[c] == 1 || n >= nthr) ? 1 : 0;
 }
 
-__global__ void flag_merge(const int* active, int noct, AmrTreeView t, int* flag1, int* flag2) {
+// Copy of the level's flags from one buffer to the other.
+__global__ void flag_copy(const int* active, int noct, AmrTreeView t, const int* in, int* out) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   int g, ind;
   long c;
   if (!level_cell(active, noct, t, i, g, ind, c)) return;
-  if (flag1[c] == 1) flag2[c] = 0;
-  if (flag2[c] == 1) flag1[c] = 1;
+  out[c] = in[c];
 }
 
 struct Prim3 {
@@ -203,7 +209,13 @@ __global__ __launch_bounds__(kScanThreads) void scan_tiles(const long long* in, 
     if (base + k < n) out[base + k] = run;
     run += v[k];
   }
-  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+  if (threadIdx.x == 0) {
+    sums[blockIdx.x] = total;
+    if (gridDim.x == 1) {  // one tile: its exclusive offsets are final
+      sums[0] = 0;
+      sums[1] = total;
+    }
+  }
 }
 
 // One block: exclusive scan of the tile totals; sums[ntiles] = grand total.
@@ -361,8 +373,10 @@ hipError_t scan64(const long long* in, long long* out, long long* sums, long n, 
   const long nt = (n + kScanTile - 1) / kScanTile;
   if (nt == 0) return hipSuccess;
   hipLaunchKernelGGL(scan_tiles, dim3((unsigned)nt), dim3(kScanThreads), 0, s, in, out, sums, n);
-  hipLaunchKernelGGL(scan_sums, dim3(1), dim3(1024), 0, s, sums, (int)nt);
-  hipLaunchKernelGGL(scan_add, dim3(nblk(n, 256)), dim3(256), 0, s, out, (const long long*)sums, n);
+  if (nt > 1) {
+    hipLaunchKernelGGL(scan_sums, dim3(1), dim3(1024), 0, s, sums, (int)nt);
+    hipLaunchKernelGGL(scan_add, dim3(nblk(n, 256)), dim3(256), 0, s, out, (const long long*)sums, n);
+  }
   return hipGetLastError();
 }
 
@@ -377,17 +391,24 @@ hipError_t launch_flag_fine(const int* active, int noct, const AmrTreeView& t, i
   const unsigned nb = nblk(n, 256);
   hipLaunchKernelGGL(flag_init, dim3(nb), dim3(256), 0, s, active, noct, t, flag1, ilevel < levelmin ? 1 : 0);
   if (ilevel < levelmin) return hipGetLastError();
+  // the flags ping-pong between flag1 and flag2, one launch per pass
+  int* cur = flag1;
+  int* other = flag2;
   auto smooth = [&]() {
     const int nthr[3] = {1, 2, 2};
     for (int k = 0; k < 3; ++k) {
-      hipLaunchKernelGGL(flag_count, dim3(nb), dim3(256), 0, s, active, noct, t, (const int*)flag1, flag2, nthr[k]);
-      hipLaunchKernelGGL(flag_merge, dim3(nb), dim3(256), 0, s, active, noct, t, flag1, flag2);
+      hipLaunchKernelGGL(flag_smooth, dim3(nb), dim3(256), 0, s, active, noct, t, (const int*)cur, other, nthr[k]);
+      int* tmp = cur;
+      cur = other;
+      other = tmp;
     }
   };
   smooth();
   if (rp.err_grad_d >= 0.0 || rp.err_grad_p >= 0.0 || rp.err_grad_u >= 0.0)
-    hipLaunchKernelGGL(flag_hydro, dim3(nb), dim3(256), 0, s, active, noct, t, u, us, flag1, rp, hp);
+    hipLaunchKernelGGL(flag_hydro, dim3(nb), dim3(256), 0, s, active, noct, t, u, us, cur, rp, hp);
   for (int e = 0; e < rp.nexpand; ++e) smooth();
+  if (cur != flag1)
+    hipLaunchKernelGGL(flag_copy, dim3(nb), dim3(256), 0, s, active, noct, t, (const int*)cur, flag1);
   if (ilevel > levelmin && icount < 2)
     hipLaunchKernelGGL(flag_ensure, dim3(nblk(27L * noct, 256)), dim3(256), 0, s, active, noct, t, ilevel, flag1);
   return hipGetLastError();
