@@ -256,6 +256,7 @@ def amr_bench(torch, ra, args):
                                   "frac": ach / HBM_PEAK, "traffic": None}}
         finals[prec] = (s.son.clone(), s.uold.clone(), float(s.time.item()))
         del s
+    cpu = None if args.no_cpu else amr_cpu_baseline(w, lmin, lmax)
     err = None
     same_mesh = bool(torch.equal(finals["fp64"][0], finals["fp32"][0]))
     if same_mesh:
@@ -276,7 +277,48 @@ def amr_bench(torch, ra, args):
         "roofline": head["roofline"], "variants": out, "fp32_same_mesh": same_mesh,
         "fp32_rel_l1_vs_fp64": err, "t_end": {k: v[2] for k, v in finals.items()},
         "fp32_over_fp64": out["fp32"]["cell_updates_per_s"] / head["cell_updates_per_s"],
+        **({"cpu_baseline": cpu} if cpu else {}),
     }
+
+
+def amr_cpu_baseline(w, lmin, lmax, budget_s=15.0):
+    """The reference's own compiled level routines (oracle/_ref: init_flow,
+    flag_fine, refine_fine, courant_fine, godunov_fine, set_unew/uold,
+    upload_fine) driven in amr_step's order on the same adaptive Sedov
+    workload, one host core; cell-updates counted as on the GPU."""
+    import numpy as np
+    import oracle
+    if not oracle.reference_available():
+        return None
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from amr_cases import reference_amr_step, reference_init_refine
+    from ramses_amd.tree import OctTree
+    R = oracle.Reference()
+    A = oracle.ReferenceAMR(R)
+    R.set_params(gamma=w["gamma"], riemann="hllc", slope_type=1, courant_factor=w["courant"])
+    tree = OctTree(lmin, lmin, extra_grids=w["ngridmax"] - sum(8 ** k for k in range(lmin)))
+    A.install(tree, nlevelmax=lmax)
+    A.set_interpol(1, 0)
+    A.set_refine(lmin, err_grad_d=w["err_grad_d"], err_grad_p=w["err_grad_p"])
+    xg = np.zeros((tree.ngridmax, 3))
+    for (l, gx, gy, gz), i in tree.grid_at.items():
+        xg[i - 1] = (np.array([gx, gy, gz]) + 0.5) * 0.5 ** (l - 1)
+    A.set_xg(xg)
+    A.set_sedov(1.0, 1e-5, 0.4, (0.5, 0.5, 0.5))
+    reference_init_refine(A, lmin, lmax)
+    dt, dtold = [0.0] * (lmax + 2), [0.0] * (lmax + 2)
+    stats, steps = {"cells": 0}, 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s and steps < 8:
+        reference_amr_step(A, lmin, lmax, dt, dtold, stats=stats)
+        steps += 1
+    el = time.perf_counter() - t0
+    cells = stats["cells"]
+    A.install(OctTree(1, 1), nlevelmax=1)  # release the reference's arrays
+    return {"value": cells / el, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
+            "sample": f"{steps} coarse steps of the same adaptive Sedov run ({cells} cell-updates) in {el:.1f} s "
+                      f"after init_refine; flang -O2 build of the reference's level routines driven in amr_step's "
+                      f"order"}
 
 
 def serve_bench(torch, ra, args):

@@ -58,11 +58,13 @@ def reference_coarse_step(A, tree, levelmin, dt, dtold, nsub=2):
     step(levelmin, 1)
 
 
-def reference_amr_step(A, levelmin, nlevelmax, dt, dtold, adaptive=True, smallc=1e-10, boxlen=1.0, nsub=None):
+def reference_amr_step(A, levelmin, nlevelmax, dt, dtold, adaptive=True, smallc=1e-10, boxlen=1.0, nsub=None,
+                       stats=None):
     """One coarse step of amr_step (amr/amr_step.f90) driving the
     reference's compiled routines, with refinement (refine_fine at
     levelmin / second substeps, flag_fine after each level step).
-    nsub: {level: 1 or 2} (default 2 everywhere)."""
+    nsub: {level: 1 or 2} (default 2 everywhere); stats["cells"] counts
+    8 cells per oct per godunov_fine call."""
     ns = nsub or {}
 
     def n(l):
@@ -87,6 +89,8 @@ def reference_amr_step(A, levelmin, nlevelmax, dt, dtold, adaptive=True, smallc=
             dtold[l + 1] = dt[l] / float(ns.get(l, 2))
             dt[l + 1] = dt[l] / float(ns.get(l, 2))
         A.godunov(l, dt[l])
+        if stats is not None:
+            stats["cells"] = stats.get("cells", 0) + 8 * n(l)
         A.set_uold(l)
         A.upload(l)
         if adaptive:
