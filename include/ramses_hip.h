@@ -177,7 +177,10 @@ typedef struct ramses_amr_level {
 } ramses_amr_level;
 
 /* interpol_type (0-3) and interpol_var (0-2) of the buffer-oct
- * prolongation, hydro/hydro_parameters.f90:88-89 (defaults 1 and 0). */
+ * prolongation, hydro/hydro_parameters.f90:88-89 (defaults 1 and 0).
+ * interpol_var also selects what the restriction averages (upl,
+ * hydro/interpol_hydro.f90:226-261): total energy for 0, internal energy
+ * for 1 and 2. */
 int ramses_amr_config(int interpol_type, int interpol_var);
 
 /* Device workspace the godunov sweep of a level with `noct` octs needs
@@ -199,7 +202,8 @@ int ramses_amr_copy(const ramses_amr_tree *t, const ramses_amr_level *lv, const 
                     void *stream);
 
 /* upload_fine(ilevel) (hydro/interpol_hydro.f90:5-263): refined cells of
- * the level take the average of their children. */
+ * the level take the average of their children (energy per the configured
+ * interpol_var). */
 int ramses_amr_upload(const ramses_amr_tree *t, const ramses_amr_level *lv, double *u, void *stream);
 
 /* courant_fine(ilevel) (hydro/courant_fine.f90:1-159) over the level's leaf

@@ -221,6 +221,9 @@ class ReferenceAMR:
         lib.ref_condinit.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_double]
         lib.ref_amr_set_nsubcycle.argtypes = [ctypes.c_int, ip]
         lib.ref_amr_set_boxlen.argtypes = [ctypes.c_double]
+        lib.ref_amr_count_create.argtypes = [ctypes.c_int]
+        lib.ref_amr_count_create.restype = ctypes.c_int
+        lib.ref_amr_numbf.restype = ctypes.c_int
         lib.ref_amr_set_outputs.argtypes = [ctypes.c_int, _dp]
         lib.ref_amr_dump.argtypes = [ctypes.c_char_p, ctypes.c_double, ctypes.c_int, ctypes.c_int]
         self.lib = lib
@@ -282,6 +285,14 @@ class ReferenceAMR:
         self.lib.ref_amr_flag_coarse()
 
     def refine_fine(self, l):
+        """refine_fine(l) + build_comm(l+1).  The reference stops the whole
+        process (exit status 0) when the free list is too short; that case
+        is refused here with an exception instead."""
+        if l < self.nlevelmax:
+            need, free = self.lib.ref_amr_count_create(l), self.lib.ref_amr_numbf()
+            if need > 0 and need >= free:
+                raise MemoryError(f"refine_fine({l}) would need {need} grids with {free} free "
+                                  "(the reference would stop: No more free memory)")
         self.lib.ref_amr_refine_fine(l)
 
     def set_flag1(self, flag1):

@@ -589,3 +589,31 @@ subroutine ref_amr_flag_coarse() bind(C, name="ref_amr_flag_coarse")
   implicit none
   call flag_coarse
 end subroutine ref_amr_flag_coarse
+
+! How many octs refine_fine(ilevel) would create (leaf cells with flag1 == 1),
+! and the free-grid count: lets a caller refuse a refinement that would make
+! the reference stop with "No more free memory" (amr/refine_utils.f90:415-423).
+function ref_amr_count_create(ilevel) bind(C, name="ref_amr_count_create") result(n)
+  use iso_c_binding
+  use amr_commons
+  implicit none
+  integer(c_int), value :: ilevel
+  integer(c_int) :: n
+  integer :: i, ind, c
+  n = 0
+  if (ilevel < 1 .or. ilevel > nlevelmax) return
+  do ind = 1, twotondim
+     do i = 1, active(ilevel)%ngrid
+        c = ncoarse + (ind-1)*ngridmax + active(ilevel)%igrid(i)
+        if (flag1(c) == 1 .and. son(c) == 0) n = n + 1
+     end do
+  end do
+end function ref_amr_count_create
+
+function ref_amr_numbf() bind(C, name="ref_amr_numbf") result(n)
+  use iso_c_binding
+  use amr_commons
+  implicit none
+  integer(c_int) :: n
+  n = numbf
+end function ref_amr_numbf

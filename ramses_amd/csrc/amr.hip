@@ -203,16 +203,13 @@ __global__ void amr_copy_level(const int* active, int noct, AmrTreeView t, const
   for (int v = 0; v < 5; ++v) dst[(long)v * ustride + c] = src[(long)v * ustride + c];
 }
 
-// upload_fine: refined cells of the level get the average of their children
-// (hydro/interpol_hydro.f90:5-263: density with the smallr floor, sums over
-// the children in order, /8).
-__global__ void amr_upload(const int* active, int noct, AmrTreeView t, double* u, long ustride, double smallr) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= 8L * noct) return;
-  const int igrid = active[i / 8];
-  const long c = cell_of(t, (int)(i % 8) + 1, igrid);
-  const int s = t.son[c];
-  if (s <= 0) return;
+// upl for one refined cell c with son oct s: the average of its children
+// (hydro/interpol_hydro.f90:72-263). Density with the smallr floor, then
+// momenta and total energy, each summed over the children in order and /8;
+// with interpol_var 1 or 2 the energy is instead rebuilt from the children's
+// average internal energy plus the kinetic energy of the averaged state.
+__device__ inline void upl_cell(const AmrTreeView& t, double* u, long ustride, long c, int s, double smallr,
+                                int ivar_mode) {
   double getx = 0.0;
   for (int ind = 1; ind <= 8; ++ind) getx = getx + fmax_f(u[cell_of(t, ind, s)], smallr);
   double out[5];
@@ -222,7 +219,37 @@ __global__ void amr_upload(const int* active, int noct, AmrTreeView t, double* u
     for (int ind = 1; ind <= 8; ++ind) getx = getx + u[(long)v * ustride + cell_of(t, ind, s)];
     out[v] = getx / 8.0;
   }
+  if (ivar_mode == 1 || ivar_mode == 2) {
+    getx = 0.0;
+    for (int ind = 1; ind <= 8; ++ind) {
+      const long cs = cell_of(t, ind, s);
+      const double rho = fmax_f(u[cs], smallr);
+      double ekin = 0.0;
+      for (int d = 1; d <= 3; ++d) {
+        const double m = u[(long)d * ustride + cs];
+        ekin = ekin + 0.5 * (m * m) / rho;
+      }
+      getx = getx + u[4 * ustride + cs] - ekin;
+    }
+    const double rho = fmax_f(out[0], smallr);
+    double ekin = 0.0;
+    for (int d = 1; d <= 3; ++d) ekin = ekin + 0.5 * (out[d] * out[d]) / rho;
+    out[4] = getx / 8.0 + ekin;
+  }
   for (int v = 0; v < 5; ++v) u[(long)v * ustride + c] = out[v];
+}
+
+// upload_fine: refined cells of the level get the average of their children
+// (hydro/interpol_hydro.f90:5-67).
+__global__ void amr_upload(const int* active, int noct, AmrTreeView t, double* u, long ustride, double smallr,
+                           int ivar_mode) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 8L * noct) return;
+  const int igrid = active[i / 8];
+  const long c = cell_of(t, (int)(i % 8) + 1, igrid);
+  const int s = t.son[c];
+  if (s <= 0) return;
+  upl_cell(t, u, ustride, c, s, smallr, ivar_mode);
 }
 
 // set_uold followed by upload_fine of the level in one pass
@@ -231,7 +258,7 @@ __global__ void amr_upload(const int* active, int noct, AmrTreeView t, double* u
 // writes every cell and upload_fine then rewrites the refined ones, so the
 // result is the same).
 __global__ void amr_finish(const int* active, int noct, AmrTreeView t, const double* unew, double* u, long ustride,
-                           double smallr) {
+                           double smallr, int ivar_mode) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= 8L * noct) return;
   const int igrid = active[i / 8];
@@ -242,16 +269,7 @@ __global__ void amr_finish(const int* active, int noct, AmrTreeView t, const dou
     for (int v = 0; v < 5; ++v) u[(long)v * ustride + c] = unew[(long)v * ustride + c];
     return;
   }
-  double getx = 0.0;
-  for (int ind = 1; ind <= 8; ++ind) getx = getx + fmax_f(u[cell_of(t, ind, s)], smallr);
-  double out[5];
-  out[0] = getx / 8.0;
-  for (int v = 1; v < 5; ++v) {
-    getx = 0.0;
-    for (int ind = 1; ind <= 8; ++ind) getx = getx + u[(long)v * ustride + cell_of(t, ind, s)];
-    out[v] = getx / 8.0;
-  }
-  for (int v = 0; v < 5; ++v) u[(long)v * ustride + c] = out[v];
+  upl_cell(t, u, ustride, c, s, smallr, ivar_mode);
 }
 
 // courant_fine over the level's leaf cells (hydro/courant_fine.f90:44-131):
@@ -357,11 +375,11 @@ hipError_t launch_amr_copy(const int* active, int noct, const AmrTreeView& t, co
 }
 
 hipError_t launch_amr_upload(const int* active, int noct, const AmrTreeView& t, double* u, long ustride,
-                             double smallr, hipStream_t s) {
+                             double smallr, int ivar_mode, hipStream_t s) {
   if (noct <= 0) return hipSuccess;
   const long n = 8L * noct;
   hipLaunchKernelGGL(amr_upload, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, active, noct, t, u, ustride,
-                     smallr);
+                     smallr, ivar_mode);
   return hipGetLastError();
 }
 
@@ -375,11 +393,11 @@ hipError_t launch_amr_courant(const int* active, int noct, const AmrTreeView& t,
 }
 
 hipError_t launch_amr_finish(const int* active, int noct, const AmrTreeView& t, const double* unew, double* u,
-                             long ustride, double smallr, hipStream_t s) {
+                             long ustride, double smallr, int ivar_mode, hipStream_t s) {
   if (noct <= 0) return hipSuccess;
   const long n = 8L * noct;
   hipLaunchKernelGGL(amr_finish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, active, noct, t, unew, u,
-                     ustride, smallr);
+                     ustride, smallr, ivar_mode);
   return hipGetLastError();
 }
 

@@ -30,14 +30,14 @@ hipError_t launch_amr_godunov(const AmrLevelArgs& a, double dx, const double* d_
 hipError_t launch_amr_copy(const int* active, int noct, const AmrTreeView& t, const double* src, double* dst,
                            long ustride, hipStream_t s);
 hipError_t launch_amr_upload(const int* active, int noct, const AmrTreeView& t, double* u, long ustride,
-                             double smallr, hipStream_t s);
+                             double smallr, int ivar_mode, hipStream_t s);
 hipError_t launch_amr_courant(const int* active, int noct, const AmrTreeView& t, const double* u, long ustride,
                               double dx, const HydroParams& hp, double* d_dt, hipStream_t s);
 hipError_t launch_amr_dt(int op, double* dt, double* dtold, double* time, int l, int coarser, int nsub, int icount,
                          double start, hipStream_t s);
 hipError_t launch_set_ints(int* p, int n, int v, hipStream_t s);
 hipError_t launch_amr_finish(const int* active, int noct, const AmrTreeView& t, const double* unew, double* u,
-                             long ustride, double smallr, hipStream_t s);
+                             long ustride, double smallr, int ivar_mode, hipStream_t s);
 struct RefineParams {
   double err_grad_d, err_grad_u, err_grad_p;
   double floor_d, floor_u, floor_p;
@@ -344,7 +344,8 @@ int ramses_amr_upload(const ramses_amr_tree* t, const ramses_amr_level* lv, doub
   int rc = amr_view(t, v);
   if (rc) return rc;
   if (!lv || !u || (lv->noct > 0 && !lv->active)) return fail(-1, "null argument");
-  hipError_t e = ramses::launch_amr_upload(lv->active, lv->noct, v, u, v.ncell + 1, c.hp.smallr, (hipStream_t)stream);
+  hipError_t e = ramses::launch_amr_upload(lv->active, lv->noct, v, u, v.ncell + 1, c.hp.smallr, g_interpol_var,
+                                           (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "amr upload launch");
   return 0;
 }
@@ -512,7 +513,8 @@ int amr_step_rec(StepCtx& c, int l, int icount) {
   rc = ramses_amr_godunov(&m->tree, &lv, m->uold, m->unew, m->dt + l, m->work, m->work_bytes, m->error, c.s);
   if (rc) return rc;
   m->cell_updates += 8LL * lv.noct;
-  e = ramses::launch_amr_finish(lv.active, lv.noct, v, m->unew, m->uold, v.ncell + 1, c.hp.smallr, c.s);
+  e = ramses::launch_amr_finish(lv.active, lv.noct, v, m->unew, m->uold, v.ncell + 1, c.hp.smallr, g_interpol_var,
+                                c.s);
   if (e != hipSuccess) return hip_fail(e, "set_uold/upload launch");
   if (m->adaptive) {
     // ensure_ref_rules runs for icount < nsubcycle(l-1) (amr/flag_utils.f90:94-98)

@@ -176,12 +176,12 @@ def test_amr_subcycled_steps_vs_reference(gpu):
     ra.configure()
 
 
-def _refine_setup(egd, egp, nlevelmax=6, seed=3):
+def _refine_setup(egd, egp, nlevelmax=6, seed=3, extra_grids=20000):
     import ramses_amd as ra
     from ramses_amd.amr import AmrSolver, configure_interpolation, configure_refinement
     R = oracle.Reference()
     A = oracle.ReferenceAMR(R)
-    tree = sphere_tree(extra_grids=20000)
+    tree = sphere_tree(extra_grids=extra_grids)
     u = smooth_state(tree, seed=seed)
     R.set_params(gamma=5 / 3, riemann="hllc", slope_type=1, courant_factor=0.5)
     ra.configure(gamma=5 / 3, riemann="hllc", slope_type=1, courant_factor=0.5)
@@ -567,3 +567,114 @@ def test_namelist_run_matches_reference(gpu):
     finally:
         shutil.rmtree(base)
         ra.configure()
+
+
+@needs_ref
+@pytest.mark.parametrize("itype,ivar", [(0, 0), (2, 1), (3, 2)])
+def test_refine_interpolation_variants_vs_reference(gpu, itype, ivar):
+    """make_grid_fine prolongates new octs with interpol_hydro: every
+    interpolation type / variable choice gives the reference's new cells,
+    and one adaptive coarse step stays bitwise."""
+    import ramses_amd as ra
+    from amr_cases import reference_amr_step
+    from ramses_amd.amr import configure_interpolation
+    A, s, tree = _refine_setup(0.12, 0.25, extra_grids=60000)
+    A.set_interpol(itype, ivar)
+    configure_interpolation(itype, ivar)
+    for l in range(6, 0, -1):
+        if s.noct(l):
+            A.flag_fine(l, 2)
+    s.flag_all()
+    for l in range(3, 6):
+        A.refine_fine(l)
+        s.refine(l)
+    s.check()
+    _same_tree(A, s, 6, "refine")
+    assert np.array_equal(s.state("old"), A.get(0))
+    dt, dtold = [0.0] * 8, [0.0] * 8
+    reference_amr_step(A, 3, 6, dt, dtold)
+    s.coarse_step()
+    s.check()
+    _same_tree(A, s, 6, "step")
+    assert np.array_equal(s.state("old"), A.get(0))
+    A.set_interpol(1, 0)
+    configure_interpolation(1, 0)
+    ra.configure()
+
+
+@needs_ref
+def test_velocity_criterion_and_nexpand_vs_reference(gpu):
+    """err_grad_u (hydro_refine's velocity test with sound-speed floor) and
+    nexpand = 2 buffers: flags on every level and one refinement bitwise."""
+    import ramses_amd as ra
+    from ramses_amd.amr import configure_refinement
+    A, s, tree = _refine_setup(-1.0, -1.0)
+    A.set_refine(3, err_grad_u=0.3, nexpand=2)
+    configure_refinement(err_grad_u=0.3, nexpand=2)
+    for l in range(6, 0, -1):
+        if s.noct(l):
+            A.flag_fine(l, 2)
+            s.flag(l, 2, 3)
+    gfl, rfl = _same_tree(A, s, 6, "flags")
+    assert np.array_equal(gfl, rfl)
+    assert 0 < int(gfl.sum()) < len(gfl)
+    for l in range(3, 6):
+        A.refine_fine(l)
+        s.refine(l)
+    s.check()
+    _same_tree(A, s, 6, "refine")
+    assert np.array_equal(s.state("old"), A.get(0))
+    configure_refinement()
+    ra.configure()
+
+
+def test_refine_out_of_grids_is_an_error_and_changes_nothing(gpu):
+    """With too few free grids refine_fine returns the reference's "No more
+    free memory" condition as an error (status -9) and leaves the mesh as it
+    was: level 3 wants 57 new octs, 50 grids are free."""
+    import ramses_amd as ra
+    from ramses_amd.amr import AmrSolver, configure_refinement
+    tree = sphere_tree(extra_grids=50)
+    u = smooth_state(tree, seed=3)
+    ra.configure(gamma=5 / 3, riemann="hllc", slope_type=1, courant_factor=0.5)
+    configure_refinement(err_grad_d=0.12, err_grad_p=0.25)
+    s = AmrSolver(tree, u, refine=True, nlevelmax=6)
+    s.flag_all()
+    before = (s.tree_arrays(), {l: s.grids(l).copy() for l in range(1, 7)}, s.free_list().copy(), s.state("old"))
+    with pytest.raises(ra.RamsesError, match="free memory"):
+        s.refine(3)
+    after = (s.tree_arrays(), {l: s.grids(l) for l in range(1, 7)}, s.free_list(), s.state("old"))
+    assert all(np.array_equal(a, b) for a, b in zip(before[0], after[0]))
+    assert all(np.array_equal(before[1][l], after[1][l]) for l in range(1, 7))
+    assert np.array_equal(before[2], after[2]) and np.array_equal(before[3], after[3])
+    assert int(s.error.item()) == 0
+    configure_refinement()
+    ra.configure()
+
+
+def test_adaptive_fp32_mixed_close_to_fp64(gpu):
+    """FP32-mixed adaptive coarse step: same refinement decisions here and
+    a relative L1 difference on the leaves below 1e-5 per variable."""
+    import ramses_amd as ra
+    from ramses_amd.amr import AmrSolver, configure_interpolation, configure_refinement
+    tree = sphere_tree(extra_grids=20000)
+    u = smooth_state(tree, seed=3)
+    out = {}
+    for prec in ("fp64", "fp32"):
+        ra.configure(gamma=5 / 3, riemann="hllc", slope_type=1, courant_factor=0.5, precision=prec)
+        configure_interpolation(1, 0)
+        configure_refinement(err_grad_d=0.12, err_grad_p=0.25)
+        s = AmrSolver(tree, u, refine=True, nlevelmax=6)
+        s.flag_all()
+        s.coarse_step()
+        s.check()
+        out[prec] = (s.tree_arrays()[0], s.state("old"))
+    son64, u64 = out["fp64"]
+    son32, u32 = out["fp32"]
+    assert np.array_equal(son64, son32)
+    leaf = son64 == 0
+    for v in range(5):
+        a, b = u32[v][leaf], u64[v][leaf]
+        assert np.abs(a - b).sum() / np.abs(b).sum() < 1e-5, v
+    configure_refinement()
+    ra.configure()
