@@ -46,24 +46,38 @@ __global__ __launch_bounds__(1024) void amr_gather(AmrLevelArgs a) {
   const size_t us = (size_t)nv * 216;               // variable stride in uin
   double* uin = a.uin + (size_t)b * us * 5 + lane;
   unsigned char* ok = a.ok + (size_t)b * us + lane;
+  // Every load of a branch is issued before its first store: the stores into
+  // uin could alias uold as far as the compiler knows, so interleaving them
+  // would serialise the 40 loads into a chain of memory latencies.
   if (igrid_n > 0) {
+    double val[8][5];
+    unsigned char okv[8];
+#pragma unroll
+    for (int ind = 1; ind <= 8; ++ind) {
+      const long c = cell_of(t, ind, igrid_n);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) val[ind - 1][v] = a.uold[(long)v * a.ustride + c];
+      okv[ind - 1] = t.son[c] > 0 ? 1 : 0;
+    }
+#pragma unroll
     for (int ind = 1; ind <= 8; ++ind) {
       const int i2 = (ind - 1) & 1, j2 = ((ind - 1) >> 1) & 1, k2 = (ind - 1) >> 2;
       const int i3 = 1 + 2 * (i1 - 1) + i2, j3 = 1 + 2 * (j1 - 1) + j2, k3 = 1 + 2 * (k1 - 1) + k2;
       const size_t s = (size_t)((i3 + 1) + 6 * ((j3 + 1) + 6 * (k3 + 1))) * nv;
-      const long c = cell_of(t, ind, igrid_n);
 #pragma unroll
-      for (int v = 0; v < 5; ++v) uin[s + (size_t)v * us] = a.uold[(long)v * a.ustride + c];
-      ok[s] = t.son[c] > 0 ? 1 : 0;
+      for (int v = 0; v < 5; ++v) uin[s + (size_t)v * us] = val[ind - 1][v];
+      ok[s] = okv[ind - 1];
     }
   } else {
     long nb7[7];
     nbor_father7(t, nf, a.ilevel, nb7);
     double u1[7][5], u2[8][5];
+#pragma unroll
     for (int q = 0; q < 7; ++q)
 #pragma unroll
       for (int v = 0; v < 5; ++v) u1[q][v] = a.uold[(long)v * a.ustride + nb7[q]];
     interpol_oct(u1, u2, a.interpol_type, a.interpol_var, a.hp.smallr);
+#pragma unroll
     for (int ind = 1; ind <= 8; ++ind) {
       const int i2 = (ind - 1) & 1, j2 = ((ind - 1) >> 1) & 1, k2 = (ind - 1) >> 2;
       const int i3 = 1 + 2 * (i1 - 1) + i2, j3 = 1 + 2 * (j1 - 1) + j2, k3 = 1 + 2 * (k1 - 1) + k2;
@@ -126,12 +140,15 @@ __global__ __launch_bounds__(256) void amr_update(AmrLevelArgs a) {
 }
 
 // Conservative correction of the coarser leaf cells next to the level's
-// octs.  Thread = (one cell of a coarse-level oct, one variable).  A coarse leaf c receives
-// from each fine oct o that sits across one of its faces the four (reset)
-// fluxes of o's boundary faces on c's side, times -1/8 (c left of o) or
-// +1/8 (c right of o).  The reference adds them in loop order: batch of o,
-// then dimension, then side, then face (k3, j3, i3); the same order is
-// rebuilt here from the position of o in the level's active list.
+// octs.  Thread = (one cell of a coarse-level oct, one variable): the tree
+// walk is repeated per variable, which keeps five times more threads in
+// flight on the small coarse levels where this kernel is latency-bound.  A
+// coarse leaf c receives from each fine oct o that sits across one of its
+// faces the four (reset) fluxes of o's boundary faces on c's side, times
+// -1/8 (c left of o) or +1/8 (c right of o).  The reference adds them in
+// loop order: batch of o, then dimension, then side, then face (k3, j3, i3);
+// the same order is rebuilt here from the position of o in the level's
+// active list.
 __global__ void amr_reflux(AmrLevelArgs a) {
   const long tix = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (tix >= 40L * a.noct_coarse) return;
@@ -143,8 +160,8 @@ __global__ void amr_reflux(AmrLevelArgs a) {
   const long c = cell_of(t, ind, g);
   if (t.son[c] != 0) return;  // only leaves receive the correction (:811,863)
   const int p[3] = {(ind - 1) & 1, ((ind - 1) >> 1) & 1, (ind - 1) >> 2};
-  // up to 6 contributing octs: key = batch*6 + 2*d + side
-  int keys[6], octs[6], ne = 0;
+  // the six neighbour cells first (independent loads), then their sons
+  long nc[6];
 #pragma unroll
   for (int in = 1; in <= 6; ++in) {
     const int ax = (in - 1) >> 1, plus = (in - 1) & 1;
@@ -152,46 +169,52 @@ __global__ void amr_reflux(AmrLevelArgs a) {
     int q[3] = {p[0], p[1], p[2]};
     q[ax] = 1 - p[ax];
     const int gn = inside ? g : t.son[nbor_of(t, g, in)];
-    if (gn <= 0) continue;
-    const long n = cell_of(t, 1 + q[0] + 2 * q[1] + 4 * q[2], gn);
-    const int o = t.son[n];
-    if (o <= 0) continue;
+    nc[in - 1] = gn > 0 ? cell_of(t, 1 + q[0] + 2 * q[1] + 4 * q[2], gn) : 0;
+  }
+  int so[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) so[k] = nc[k] > 0 ? t.son[nc[k]] : 0;
+  int pz[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) pz[k] = so[k] > 0 ? a.pos[so[k]] : 0;
+  // up to 6 contributing octs: key = batch*6 + 2*d + side
+  int keys[6], posn[6], ne = 0;
+#pragma unroll
+  for (int in = 1; in <= 6; ++in) {
+    if (so[in - 1] <= 0) continue;
+    const int ax = (in - 1) >> 1, plus = (in - 1) & 1;
     // c is on o's minus side when n is c's plus neighbour
     const int side = plus ? 0 : 1;
-    const int position = a.pos[o];
-    keys[ne] = (position / a.nv) * 6 + 2 * ax + side;
-    octs[ne] = o;
+    keys[ne] = (pz[in - 1] / a.nv) * 6 + 2 * ax + side;
+    posn[ne] = pz[in - 1];
     ++ne;
   }
   if (ne == 0) return;
   for (int i = 1; i < ne; ++i)  // insertion sort, at most 6 entries
     for (int k = i; k > 0 && keys[k - 1] > keys[k]; --k) {
       const int tk = keys[k]; keys[k] = keys[k - 1]; keys[k - 1] = tk;
-      const int to = octs[k]; octs[k] = octs[k - 1]; octs[k - 1] = to;
+      const int tp = posn[k]; posn[k] = posn[k - 1]; posn[k - 1] = tp;
     }
   const int nv = a.nv;
   const size_t fvs = (size_t)nv * 27;
   const double oneontwotondim = 1.0 / 8.0;
-  {
-    double s = a.unew[(long)v * a.ustride + c];
-    for (int e = 0; e < ne; ++e) {
-      const int d = (keys[e] % 6) >> 1, side = keys[e] & 1;
-      const int position = a.pos[octs[e]];
-      const int b = position / nv, lane = position % nv;
-      const double* flux = a.flux + (size_t)b * fvs * 15 + lane + (size_t)27 * (v + 5 * d) * nv;
-      const int i0 = d == 0, j0 = d == 1, k0 = d == 2;
-      if (side == 0) {
-        for (int k3 = 1; k3 <= 2 - k0; ++k3)
-          for (int j3 = 1; j3 <= 2 - j0; ++j3)
-            for (int i3 = 1; i3 <= 2 - i0; ++i3) s = s - flux[fslot(i3, j3, k3) * nv] * oneontwotondim;
-      } else {
-        for (int k3 = 1 + k0; k3 <= 2; ++k3)
-          for (int j3 = 1 + j0; j3 <= 2; ++j3)
-            for (int i3 = 1 + i0; i3 <= 2; ++i3) s = s + flux[fslot(i3 + i0, j3 + j0, k3 + k0) * nv] * oneontwotondim;
-      }
+  double s = a.unew[(long)v * a.ustride + c];
+  for (int e = 0; e < ne; ++e) {
+    const int d = (keys[e] % 6) >> 1, side = keys[e] & 1;
+    const int b = posn[e] / nv, lane = posn[e] % nv;
+    const double* flux = a.flux + (size_t)b * fvs * 15 + lane + (size_t)27 * (v + 5 * d) * nv;
+    const int i0 = d == 0, j0 = d == 1, k0 = d == 2;
+    if (side == 0) {
+      for (int k3 = 1; k3 <= 2 - k0; ++k3)
+        for (int j3 = 1; j3 <= 2 - j0; ++j3)
+          for (int i3 = 1; i3 <= 2 - i0; ++i3) s = s - flux[fslot(i3, j3, k3) * nv] * oneontwotondim;
+    } else {
+      for (int k3 = 1 + k0; k3 <= 2; ++k3)
+        for (int j3 = 1 + j0; j3 <= 2; ++j3)
+          for (int i3 = 1 + i0; i3 <= 2; ++i3) s = s + flux[fslot(i3 + i0, j3 + j0, k3 + k0) * nv] * oneontwotondim;
     }
-    a.unew[(long)v * a.ustride + c] = s;
   }
+  a.unew[(long)v * a.ustride + c] = s;
 }
 
 // set_unew / set_uold over the level's active cells (hydro/godunov_fine.f90:58-65, 193-197).

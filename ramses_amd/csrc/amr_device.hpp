@@ -33,8 +33,10 @@ __device__ inline int nbor_father_cell(const AmrTreeView& t, long f, int i1, int
     if (iiz > t.nz - 1) iiz = 0;
     return 1 + iix + iiy * t.nx + iiz * nxny;
   }
-  const int pos = (int)((f - t.ncoarse - 1) / t.ngridmax) + 1;
-  const int g = (int)(f - t.ncoarse - (long)(pos - 1) * t.ngridmax);
+  // cell indices fit in 32 bits (checked at the entry points): 32-bit division
+  const unsigned r = (unsigned)(f - t.ncoarse - 1);
+  const int pos = (int)(r / (unsigned)t.ngridmax) + 1;
+  const int g = (int)(r - (unsigned)(pos - 1) * (unsigned)t.ngridmax) + 1;
   const int px = (pos - 1) & 1, py = ((pos - 1) >> 1) & 1, pz = (pos - 1) >> 2;
   const int cx = px + i1 - 1, cy = py + j1 - 1, cz = pz + k1 - 1;
   int gg = g;
@@ -67,8 +69,10 @@ __device__ inline void nbor_father7(const AmrTreeView& t, long f, int ilevel, lo
     }
     return;
   }
-  const int pos = (int)((f - t.ncoarse - 1) / t.ngridmax) + 1;
-  const int g = (int)(f - t.ncoarse - (long)(pos - 1) * t.ngridmax);
+  // cell indices fit in 32 bits (checked at the entry points): 32-bit division
+  const unsigned r = (unsigned)(f - t.ncoarse - 1);
+  const int pos = (int)(r / (unsigned)t.ngridmax) + 1;
+  const int g = (int)(r - (unsigned)(pos - 1) * (unsigned)t.ngridmax) + 1;
   const int p[3] = {(pos - 1) & 1, ((pos - 1) >> 1) & 1, (pos - 1) >> 2};
   for (int in = 1; in <= 6; ++in) {
     const int a = (in - 1) >> 1, plus = (in - 1) & 1;

@@ -254,6 +254,8 @@ int amr_view(const ramses_amr_tree* t, ramses::AmrTreeView& v) {
   if (!t || !t->son || !t->father || !t->nbor) return fail(-1, "null tree array");
   if (t->ncoarse < 1 || t->ngridmax < 1 || t->nx < 1 || t->ny < 1 || t->nz < 1) return fail(-1, "bad tree sizes");
   if (t->ncoarse != t->nx * t->ny * t->nz) return fail(-1, "ncoarse must be nx*ny*nz");
+  // cell indices are 32-bit, as the reference's integer arrays are
+  if ((long)t->ncoarse + 8L * t->ngridmax > 2147483647L) return fail(-1, "ncoarse + 8*ngridmax exceeds int32");
   v.son = t->son;
   v.father = t->father;
   v.nbor = t->nbor;
