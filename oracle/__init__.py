@@ -380,6 +380,53 @@ class ReferenceAMR:
             out[:, i0:i0 + m] = ub[:, :m]
         return out
 
+    def restart(self, outdir, nrestart, nlevelmax, ngridmax):
+        """The reference's restart (init_amr + init_hydro, ref_amr_restart)
+        from outdir/output_NNNNN, NNNNN = nrestart.  The reference stops the
+        process on a missing or incompatible file, so the file header is
+        checked here first and a mismatch raises ValueError."""
+        import os
+        import struct
+        import types
+        tag = f"{nrestart:05d}"
+        amr = os.path.join(outdir, f"output_{tag}", f"amr_{tag}.out00001")
+        hyd = os.path.join(outdir, f"output_{tag}", f"hydro_{tag}.out00001")
+        for f in (amr, hyd):
+            if not os.path.exists(f):
+                raise ValueError(f"{f} not found")
+        with open(amr, "rb") as f:
+            data = f.read(4096)
+        vals, p = [], 0
+        for _ in range(7):      # ncpu, ndim, (nx,ny,nz), nlevelmax, ngridmax, nboundary, ngrid
+            (n,) = struct.unpack_from("<i", data, p)
+            vals.append(struct.unpack_from(f"<{n // 4}i", data, p + 4))
+            p += n + 8
+        ncpu, ndim, nxyz, lmax, _, nbound, ngrid = vals
+        if (ncpu[0], ndim[0], tuple(nxyz), nbound[0]) != (1, 3, (1, 1, 1), 0) or lmax[0] > nlevelmax \
+                or ngrid[0] > ngridmax:
+            raise ValueError(f"{amr} is not compatible with a one-domain restart at these sizes")
+        ip = ctypes.POINTER(ctypes.c_int)
+        self.lib.ref_amr_restart.argtypes = [ctypes.c_int] * 3
+        self.lib.ref_amr_get_run.argtypes = [ctypes.c_int, _dp, ip, _dp, _dp]
+        cwd = os.getcwd()
+        os.chdir(outdir)
+        try:
+            self.lib.ref_amr_restart(nlevelmax, ngridmax, nrestart)
+        finally:
+            os.chdir(cwd)
+        self.tree = types.SimpleNamespace(ngridmax=ngridmax, ncell=1 + 8 * ngridmax, nlevel=nlevelmax)
+        self.nlevelmax = nlevelmax
+
+    def run_state(self):
+        """(t, nstep, nstep_coarse, ifout, iout, used_mem, dtold[1:L], dtnew[1:L])."""
+        ip = ctypes.POINTER(ctypes.c_int)
+        L = self.nlevelmax
+        t = np.zeros(1)
+        ints = np.zeros(5, dtype=np.int32)
+        dto, dtn = np.zeros(L), np.zeros(L)
+        self.lib.ref_amr_get_run(L, _ptr(t), ints.ctypes.data_as(ip), _ptr(dto), _ptr(dtn))
+        return (float(t[0]),) + tuple(int(v) for v in ints) + (dto, dtn)
+
     def dump(self, directory, t, nstep=0, nstep_coarse=0):
         """dump_all's snapshot files into an existing directory."""
         self.lib.ref_amr_dump(directory.encode(), t, nstep, nstep_coarse)

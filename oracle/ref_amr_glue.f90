@@ -617,3 +617,54 @@ function ref_amr_numbf() bind(C, name="ref_amr_numbf") result(n)
   integer(c_int) :: n
   n = numbf
 end function ref_amr_numbf
+
+! The restart of amr/adaptive_loop.f90:32-34 for one domain: init_amr reads
+! output_NNNNN/amr_NNNNN.out00001 (amr/init_amr.f90:261-551: tree, level
+! lists, free list, times, level dts; build_comm per level) and init_hydro
+! reads hydro_NNNNN.out00001 (hydro/init_hydro.f90:56-250: primitive ->
+! conserved), both relative to the current directory.  The arrays the glue
+! or an earlier restart allocated are released first, since both routines
+! allocate them.  The caller checks the files first: the reference stops
+! the process on an incompatible file.
+subroutine ref_amr_restart(nlevelmax_in, ngridmax_in, nrestart_in) bind(C, name="ref_amr_restart")
+  use iso_c_binding
+  use amr_commons
+  use hydro_commons
+  implicit none
+  integer(c_int), value :: nlevelmax_in, ngridmax_in, nrestart_in
+  nx = 1; ny = 1; nz = 1
+  icoarse_min = 0; icoarse_max = 0
+  jcoarse_min = 0; jcoarse_max = 0
+  kcoarse_min = 0; kcoarse_max = 0
+  ncpu = 1; myid = 1
+  hydro = .true.
+  ordering = 'planar'
+  nlevelmax = nlevelmax_in
+  ngridmax = ngridmax_in
+  nrestart = nrestart_in
+  if (allocated(son)) then
+     deallocate(son, father, nbor, uold, unew, numbtot, active, reception)
+     deallocate(flag1, flag2, cpu_map, cpu_map2, hilbert_key, xg, next, prev)
+     deallocate(headl, taill, numbl, headb, tailb, numbb, boundary, bound_key, bound_key2)
+  end if
+  if (allocated(emission)) deallocate(emission)
+  call init_amr
+  call init_hydro
+  nrestart = 0
+end subroutine ref_amr_restart
+
+! Run state after a restart: t, nstep, nstep_coarse, ifout, iout, used_mem
+! and the level dts dtold/dtnew(1:n).
+subroutine ref_amr_get_run(n, tout_, ints, dtold_out, dtnew_out) bind(C, name="ref_amr_get_run")
+  use iso_c_binding
+  use amr_commons
+  implicit none
+  integer(c_int), value :: n
+  real(c_double) :: tout_
+  integer(c_int) :: ints(5)
+  real(c_double) :: dtold_out(n), dtnew_out(n)
+  tout_ = t
+  ints(1) = nstep; ints(2) = nstep_coarse; ints(3) = ifout; ints(4) = iout; ints(5) = used_mem
+  dtold_out(1:n) = dtold(1:n)
+  dtnew_out(1:n) = dtnew(1:n)
+end subroutine ref_amr_get_run

@@ -109,6 +109,25 @@ def configure_refinement(err_grad_d=-1.0, err_grad_u=-1.0, err_grad_p=-1.0, floo
     _check(lib.ramses_refine_config(ctypes.byref(p)), "ramses_refine_config")
 
 
+class SnapshotTree:
+    """A restarted mesh (snapshot.read_output) in the form AmrLevels reads
+    an OctTree: 0-based son/father/nbor/flag1 arrays, the level lists, grid
+    centres, the free list and used_mem."""
+
+    def __init__(self, mesh, levelmin):
+        self.levelmin = levelmin
+        self.levelmax = mesh.nlevelmax
+        self.nlevel = max([l for l, g in mesh.lists.items() if len(g)] + [1])
+        self.ngridmax = mesh.ngridmax
+        self.ncell = 1 + 8 * mesh.ngridmax
+        self.son, self.father, self.nbor = mesh.son, mesh.father, mesh.nbor
+        self.flag1 = mesh.flag1
+        self.active = {l: np.asarray(g, dtype=np.int32) for l, g in mesh.lists.items()}
+        self.xg = mesh.xg
+        self.free = mesh.free
+        self.used_mem = mesh.used_mem if mesh.used_mem >= 0 else mesh.ngridmax - len(mesh.free)
+
+
 class AmrLevels:
     def __init__(self, tree, u, device="cuda", stream=None, boxlen=1.0, nlevelmax=None):
         """tree: tree.OctTree; u: conserved state (5, ncell) over every cell
@@ -149,19 +168,27 @@ class AmrLevels:
         self.pos = torch.as_tensor(pos, device=device)
         # grid centres xg (amr/amr_commons.f90) in units of the coarse cell
         xg = np.zeros((ngm + 1, 3))
-        for (l, gx, gy, gz), i in tree.grid_at.items():
-            xg[i] = (np.array([gx, gy, gz]) + 0.5) * 0.5 ** (l - 1)
+        if getattr(tree, "xg", None) is not None:
+            xg[1:] = tree.xg
+        else:
+            for (l, gx, gy, gz), i in tree.grid_at.items():
+                xg[i] = (np.array([gx, gy, gz]) + 0.5) * 0.5 ** (l - 1)
         self.xg = torch.as_tensor(xg, device=device)
-        # free grids in increasing order, as the reference's free list after
-        # allocating grids 1..n in order (amr/init_amr.f90:245-256)
-        used = np.zeros(ngm + 1, dtype=bool)
-        for ids in tree.active.values():
-            used[np.asarray(ids)] = True
-        free = np.nonzero(~used[1:])[0].astype(np.int32) + 1
+        if getattr(tree, "free", None) is not None:
+            free = np.asarray(tree.free, dtype=np.int32)
+            used_mem = tree.used_mem
+        else:
+            # free grids in increasing order, as the reference's free list
+            # after allocating grids 1..n in order (amr/init_amr.f90:245-256)
+            used = np.zeros(ngm + 1, dtype=bool)
+            for ids in tree.active.values():
+                used[np.asarray(ids)] = True
+            free = np.nonzero(~used[1:])[0].astype(np.int32) + 1
+            used_mem = ngm - len(free)
         ring = np.zeros(ngm, dtype=np.int32)
         ring[:len(free)] = free
         self.ring = torch.as_tensor(ring, device=device)
-        self.free = ramses_amr_free(self.ring.data_ptr(), 0, len(free), ngm - len(free))
+        self.free = ramses_amr_free(self.ring.data_ptr(), 0, len(free), used_mem)
         if u is None:
             self.uold = torch.zeros((5, tree.ncell + 1), dtype=torch.float64, device=device)
         else:
@@ -170,6 +197,8 @@ class AmrLevels:
             self.uold = torch.as_tensor(full, device=device).contiguous()
         self.unew = self.uold.clone()
         self.flag1 = torch.zeros(tree.ncell + 1, dtype=torch.int32, device=device)
+        if getattr(tree, "flag1", None) is not None:
+            self.flag1[1:] = torch.as_tensor(np.asarray(tree.flag1, dtype=np.int32), device=device)
         self.flag2 = torch.zeros_like(self.flag1)
         self.work = torch.empty(0, dtype=torch.uint8, device=device)
         self.error = torch.zeros(1, dtype=torch.int32, device=device)
@@ -401,6 +430,30 @@ class AmrSolver(AmrLevels):
         self.nstep_coarse = 0
         self.cell_updates = 0
         self._mesh = ramses_amr_mesh()
+
+    @classmethod
+    def from_snapshot(cls, outdir, nrestart, levelmin, nlevelmax=None, ngridmax=None, smallr=1e-10, gamma=None,
+                      ordering=None, tout=None, **kw):
+        """The restart of amr/adaptive_loop.f90:32-34: the solver and run
+        state (snapshot.RunInfo) of output_NNNNN under outdir, with
+        NNNNN = nrestart.  The mesh, free list, refinement map, state, level
+        dts, t, nstep and nstep_coarse are those init_amr and init_hydro
+        rebuild (snapshot.read_output); kw as for AmrSolver."""
+        from . import snapshot
+        mesh, run = snapshot.read_output(outdir, nrestart, ngridmax=ngridmax, nlevelmax=nlevelmax, smallr=smallr,
+                                         gamma=gamma, ordering=ordering, tout=tout)
+        run.levelmin = levelmin
+        tree = SnapshotTree(mesh, levelmin)
+        s = cls(tree, mesh.uold, nlevelmax=mesh.nlevelmax, boxlen=run.boxlen, **kw)
+        n = len(run.dtnew)
+        dt = s.dt.cpu().numpy()
+        dto = s.dtold.cpu().numpy()
+        dt[1:n + 1], dto[1:n + 1] = run.dtnew, run.dtold
+        s.dt.copy_(s.dt.new_tensor(dt))
+        s.dtold.copy_(s.dtold.new_tensor(dto))
+        s.time.fill_(run.t)
+        s.fine_steps, s.nstep_coarse = run.nstep, run.nstep_coarse
+        return s, run
 
     def __del__(self):
         try:

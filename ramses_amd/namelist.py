@@ -144,6 +144,7 @@ class Params:
     nexpand: int = 1
     nsubcycle: list = field(default_factory=lambda: [2] * 100)
     nstepmax: int = 1000000
+    nrestart: int = 0
     gamma: float = 1.4
     courant_factor: float = 0.5
     smallr: float = 1e-10
@@ -185,6 +186,7 @@ class Params:
         nsub = _list(run.get("nsubcycle"))
         p.nsubcycle = [int(x) for x in nsub] + [2] * (100 - len(nsub))
         p.nstepmax = int(run.get("nstepmax", p.nstepmax))
+        p.nrestart = int(run.get("nrestart", 0))
         if str(hyd.get("scheme", "muscl")).strip().lower() != "muscl":
             raise ValueError("only scheme='muscl' is implemented")
         if float(hyd.get("difmag", 0.0)) > 0:

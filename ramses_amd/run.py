@@ -15,22 +15,30 @@ import sys
 import time
 
 
-def build(params, precision="fp64", device="cuda"):
-    """An AmrSolver with the mesh and state of the parameter file's
-    initial conditions (init_refine + init_refine_2)."""
+def _configure(p, precision):
     import ramses_amd as ra
-    from .amr import AmrSolver, configure_interpolation, configure_refinement
-    from .ics import region_condinit
-    from .tree import OctTree
-    p = params
+    from .amr import configure_interpolation, configure_refinement
     ra.configure(gamma=p.gamma, smallr=p.smallr, smallc=p.smallc, slope_type=p.slope_type,
                  slope_theta=p.slope_theta, riemann=p.riemann, courant_factor=p.courant_factor, precision=precision)
     configure_interpolation(p.interpol_type, p.interpol_var)
     configure_refinement(err_grad_d=p.err_grad_d, err_grad_u=p.err_grad_u, err_grad_p=p.err_grad_p,
                          floor_d=p.floor_d, floor_u=p.floor_u, floor_p=p.floor_p, nexpand=p.nexpand)
+
+
+def _ngridmax(p):
+    return max(p.ngridmax, sum(8 ** k for k in range(p.levelmin)) + 1)
+
+
+def build(params, precision="fp64", device="cuda"):
+    """An AmrSolver with the mesh and state of the parameter file's
+    initial conditions (init_refine + init_refine_2)."""
+    from .amr import AmrSolver
+    from .ics import region_condinit
+    from .tree import OctTree
+    p = params
+    _configure(p, precision)
     base = sum(8 ** k for k in range(p.levelmin))
-    ngridmax = max(p.ngridmax, base + 1)
-    tree = OctTree(p.levelmin, p.levelmin, extra_grids=ngridmax - base)
+    tree = OctTree(p.levelmin, p.levelmin, extra_grids=_ngridmax(p) - base)
     s = AmrSolver(tree, None, refine=p.levelmax > p.levelmin, nlevelmax=p.levelmax, nsubcycle=p.nsubcycle,
                   boxlen=p.boxlen, device=device)
     ic = lambda x, dx: region_condinit(x, dx, p.regions, gamma=p.gamma, smallr=p.smallr, smallc=p.smallc)  # noqa
@@ -41,25 +49,45 @@ def build(params, precision="fp64", device="cuda"):
     return s
 
 
+def restart(params, outdir, precision="fp64", device="cuda", ordering=None):
+    """An AmrSolver and snapshot.RunInfo restarted from
+    outdir/output_NNNNN with NNNNN = params.nrestart (init_amr and
+    init_hydro's restart, AmrSolver.from_snapshot)."""
+    from .amr import AmrSolver
+    p = params
+    _configure(p, precision)
+    return AmrSolver.from_snapshot(outdir, p.nrestart, p.levelmin, nlevelmax=p.levelmax, ngridmax=_ngridmax(p),
+                                   smallr=p.smallr, gamma=p.gamma, ordering=ordering, tout=p.tout,
+                                   refine=p.levelmax > p.levelmin, nsubcycle=p.nsubcycle, device=device)
+
+
 def run(path, outdir=".", precision="fp64", max_coarse=None, log=print, ordering="hilbert", bound_key=None):
     """ordering / bound_key: the domain decomposition the snapshot records
-    (one domain; the Hilbert key range by default)."""
+    (one domain; the Hilbert key range by default).  With nrestart > 0 in
+    &RUN_PARAMS the run continues from outdir/output_NNNNN instead, with
+    the decomposition that snapshot records."""
     from . import snapshot
     from .namelist import Params
     p = Params.from_namelist(path)
-    s = build(p, precision)
-    info = snapshot.RunInfo(levelmin=p.levelmin, gamma=p.gamma, smallr=p.smallr, boxlen=p.boxlen,
-                            noutput=len(p.tout), tout=tuple(p.tout), ordering=ordering, bound_key=bound_key)
+    if p.nrestart > 0:
+        s, info = restart(p, outdir, precision, ordering=None)
+    else:
+        s = build(p, precision)
+        info = snapshot.RunInfo(levelmin=p.levelmin, gamma=p.gamma, smallr=p.smallr, boxlen=p.boxlen,
+                                noutput=len(p.tout), tout=tuple(p.tout), ordering=ordering, bound_key=bound_key)
+    info.levelmin, info.gamma, info.smallr = p.levelmin, p.gamma, p.smallr
     t_end = p.tout[-1] if p.tout else float("inf")
-    nmax = p.nstepmax if max_coarse is None else min(p.nstepmax, max_coarse)
+    nmax = p.nstepmax
     t0 = time.perf_counter()
     outputs = []
-    while s.nstep_coarse < nmax:
+    steps = 0
+    while s.nstep_coarse < nmax and (max_coarse is None or steps < max_coarse):
         t = float(s.time.item())
         if t >= t_end:
             break
         s.coarse_step()
         s.check()
+        steps += 1
         t = float(s.time.item())
         octs = {l: s.noct(l) for l in range(p.levelmin, p.levelmax + 1) if s.noct(l)}
         log(f"coarse step {s.nstep_coarse:6d}  t={t:.6e}  fine steps {s.fine_steps}  octs {octs}")
@@ -76,7 +104,7 @@ def main(argv=None):
     ap.add_argument("namelist")
     ap.add_argument("--outdir", default=".")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
-    ap.add_argument("--max-coarse", type=int, default=None, help="stop after this many coarse steps")
+    ap.add_argument("--max-coarse", type=int, default=None, help="stop after this many coarse steps (of this run)")
     a = ap.parse_args(argv)
     run(a.namelist, a.outdir, a.precision, a.max_coarse)
 

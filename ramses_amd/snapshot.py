@@ -271,3 +271,182 @@ def write_output(mesh, run, outdir):
     write_amr(mesh, run, os.path.join(d, f"amr_{tag}.out00001"))
     write_hydro(mesh, run, os.path.join(d, f"hydro_{tag}.out00001"), os.path.join(d, "hydro_file_descriptor.txt"))
     return d
+
+
+# -- reading (restart) ------------------------------------------------------
+
+def _records(path):
+    """The payloads of a Fortran sequential unformatted file, in order."""
+    with open(path, "rb") as f:
+        data = f.read()
+    out, p = [], 0
+    while p < len(data):
+        if p + 4 > len(data):
+            raise ValueError(f"{path}: truncated record marker at byte {p}")
+        (n,) = struct.unpack_from("<i", data, p)
+        end = p + 4 + n
+        if n < 0 or end + 4 > len(data) or struct.unpack_from("<i", data, end)[0] != n:
+            raise ValueError(f"{path}: bad record at byte {p}")
+        out.append(data[p + 4:end])
+        p = end + 4
+    return out
+
+
+class _Reader:
+    def __init__(self, path):
+        self.path = path
+        self.recs = _records(path)
+        self.k = 0
+
+    def raw(self):
+        if self.k >= len(self.recs):
+            raise ValueError(f"{self.path}: fewer records than the header announces")
+        r = self.recs[self.k]
+        self.k += 1
+        return r
+
+    def i4(self):
+        return np.frombuffer(self.raw(), dtype="<i4").astype(np.int64)
+
+    def f8(self):
+        return np.frombuffer(self.raw(), dtype="<f8").copy()
+
+
+def output_files(outdir, nrestart):
+    """The amr and hydro file paths of output_NNNNN under outdir."""
+    tag = f"{nrestart:05d}"
+    d = os.path.join(outdir, f"output_{tag}")
+    return os.path.join(d, f"amr_{tag}.out00001"), os.path.join(d, f"hydro_{tag}.out00001")
+
+
+def read_output(outdir, nrestart, ngridmax=None, nlevelmax=None, smallr=1e-10, gamma=None, ordering=None,
+                tout=None):
+    """The restart of amr/adaptive_loop.f90:32-34 for one domain: the mesh
+    and run state of output_NNNNN (NNNNN = nrestart) under outdir, as
+    init_amr (amr/init_amr.f90:261-551) and init_hydro
+    (hydro/init_hydro.f90:56-250) rebuild them.
+
+    ngridmax / nlevelmax: the restarted run's sizes (default: the file's);
+    cell indices are renumbered when ngridmax differs (:440-449).  The free
+    list is every grid after the file's grids in increasing order (:386-391)
+    and used_mem is the recorded value.  ifout is the recorded one, at most
+    nrestart+1 (:331); iout is the first output time in `tout` (the
+    restarted run's) after t (:353-363).  The hydro file's primitives become
+    conserved variables as init_hydro converts them (density; velocity times
+    max(density, smallr); pressure/(gamma-1) plus the kinetic energy, or 0
+    where the density is not positive); the coarse cells are 0.  Raises
+    ValueError where the reference would stop (incompatible file)."""
+    amr_path, hydro_path = output_files(outdir, nrestart)
+    for p in (amr_path, hydro_path):
+        if not os.path.exists(p):
+            raise ValueError(f"Restart failed: file {p} not found")
+    r = _Reader(amr_path)
+    ncpu2, ndim2 = int(r.i4()[0]), int(r.i4()[0])
+    nx2, ny2, nz2 = (int(v) for v in r.i4())
+    L2, ngm2, nbound2, ngrid_current = (int(r.i4()[0]) for _ in range(4))
+    boxlen = float(r.f8()[0])
+    ngm = ngm2 if ngridmax is None else int(ngridmax)
+    L = L2 if nlevelmax is None else int(nlevelmax)
+    if ncpu2 != 1:
+        raise ValueError(f"Number of processes not compatible: the file has ncpu={ncpu2}, this build runs one domain")
+    if ndim2 != 3 or (nx2, ny2, nz2) != (1, 1, 1) or nbound2 != 0 or L2 > L or ngrid_current > ngm:
+        raise ValueError("file is not compatible with the run: ndim nx ny nz nlevelmax ngrid nboundary = "
+                         f"{ndim2} {nx2} {ny2} {nz2} {L2} {ngrid_current} {nbound2} (run: 3 1 1 1 {L} {ngm} 0)")
+    noutput2, iout2, ifout2 = (int(v) for v in r.i4())
+    tout2, aout2 = r.f8(), r.f8()
+    run = RunInfo(levelmin=1, boxlen=boxlen)
+    run.ifout = min(ifout2, nrestart + 1)
+    run.t = float(r.f8()[0])
+    run.dtold, run.dtnew = tuple(r.f8()), tuple(r.f8())
+    run.nstep, run.nstep_coarse = (int(v) for v in r.i4())
+    run.einit, run.mass_tot_0, run.rho_tot = (float(v) for v in r.f8())
+    (run.omega_m, run.omega_l, run.omega_k, run.omega_b, run.h0, run.aexp_ini,
+     run.boxlen_ini) = (float(v) for v in r.f8())
+    run.aexp, run.hexp, run.aexp_old, run.epot_tot_int, run.epot_tot_old = (float(v) for v in r.f8())
+    run.mass_sph = float(r.f8()[0])
+    huge = np.finfo(np.float64).max
+    run.tout = tuple(tout if tout is not None else tout2)
+    run.noutput = len(run.tout)
+    run.iout = 1
+    if run.tout and not all(x == huge for x in run.tout):
+        while run.iout <= len(run.tout) and run.tout[run.iout - 1] <= run.t:
+            run.iout += 1
+    headl, taill, numbl = r.i4(), r.i4(), r.i4()
+    r.raw()                                  # numbtot
+    _headf, _tailf, _numbf, used_mem, _used_tot = (int(v) for v in r.i4())
+    ordering2 = r.raw().decode("ascii", "replace").strip()
+    if ordering is not None and ordering2 != ordering:
+        raise ValueError(f"Ordering is incompatible: the file has {ordering2!r}, the run {ordering!r}")
+    run.ordering = ordering2
+    run.bound_key = tuple(r.f8())
+    ncell = NCOARSE + 8 * ngm
+    son = np.zeros(ncell, dtype=np.int32)
+    flag1 = np.zeros(ncell, dtype=np.int32)
+    father = np.zeros(ngm, dtype=np.int32)
+    nbor = np.zeros(6 * ngm, dtype=np.int32)
+    xg = np.zeros((ngm, 3))
+    son[:NCOARSE] = r.i4()
+    flag1[:NCOARSE] = r.i4()
+    r.raw()                                  # cpu_map of the coarse cells
+
+    def remap(cells, l):
+        if ngm == ngm2 or l <= 1:
+            return cells
+        pos = (cells - NCOARSE - 1) // ngm2
+        return NCOARSE + pos * ngm + (cells - NCOARSE - pos * ngm2)
+
+    lists = {}
+    for l in range(1, L2 + 1):
+        n = int(numbl[l - 1])
+        if n == 0:
+            continue
+        g = r.i4()
+        nxt, prv = r.i4(), r.i4()
+        if len(g) != n or int(g[0]) != int(headl[l - 1]) or int(g[-1]) != int(taill[l - 1]):
+            raise ValueError(f"{amr_path}: level {l} list does not match its head/tail/count")
+        if np.any(nxt[:-1] != g[1:]) or nxt[-1] != 0 or prv[0] != 0 or np.any(prv[1:] != g[:-1]):
+            raise ValueError(f"{amr_path}: level {l} next/prev links are not the list order")
+        if g.max() > ngm or g.min() < 1:
+            raise ValueError(f"{amr_path}: grid index beyond ngridmax at level {l}")
+        for d in range(3):
+            xg[g - 1, d] = r.f8()
+        father[g - 1] = remap(r.i4(), l)
+        for j in range(6):
+            nbor[j * ngm + g - 1] = remap(r.i4(), l)
+        for ind in range(1, 9):
+            son[NCOARSE + (ind - 1) * ngm + g - 1] = r.i4()
+        for ind in range(1, 9):
+            r.raw()                          # cpu_map
+        for ind in range(1, 9):
+            flag1[NCOARSE + (ind - 1) * ngm + g - 1] = r.i4()
+        lists[l] = g.astype(np.int32)
+    free = np.arange(ngrid_current + 1, ngm + 1, dtype=np.int32)
+
+    h = _Reader(hydro_path)
+    ncpu2, nvar2, ndim2, L2h, nbound2 = (int(h.i4()[0]) for _ in range(5))
+    gamma2 = float(h.f8()[0])
+    g_ = gamma2 if gamma is None else float(gamma)
+    if ncpu2 != 1 or nvar2 != 5 or ndim2 != 3 or nbound2 != 0 or L2h != L2:
+        raise ValueError(f"{hydro_path}: not a one-domain 3D hydro file with 5 variables and {L2} levels")
+    uold = np.zeros((5, ncell))
+    for l in range(1, L2 + 1):
+        l2, n2 = int(h.i4()[0]), int(h.i4()[0])
+        g = lists.get(l, np.zeros(0, dtype=np.int32)).astype(np.int64)
+        if l2 != l or n2 != len(g):
+            raise ValueError(f"{hydro_path}: level {l2} has {n2} grids, the amr file {len(g)}")
+        if n2 == 0:
+            continue
+        for ind in range(1, 9):
+            c = NCOARSE + (ind - 1) * ngm + g - 1
+            rho = h.f8()
+            uold[0, c] = rho
+            rmax = np.maximum(rho, smallr)
+            for d in range(1, 4):
+                uold[d, c] = h.f8() * rmax
+            e = h.f8() / (g_ - 1.0)
+            for d in range(1, 4):
+                e = e + (0.5 * (uold[d, c] * uold[d, c])) / rmax
+            uold[4, c] = np.where(rho > 0.0, e, 0.0)
+    mesh = Mesh(ngm, L, lists, son, father, nbor, flag1, xg, uold, free, used_mem)
+    run.gamma, run.smallr = g_, smallr
+    return mesh, run

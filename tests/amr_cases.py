@@ -59,13 +59,20 @@ def reference_coarse_step(A, tree, levelmin, dt, dtold, nsub=2):
 
 
 def reference_amr_step(A, levelmin, nlevelmax, dt, dtold, adaptive=True, smallc=1e-10, boxlen=1.0, nsub=None,
-                       stats=None):
+                       stats=None, clock=None):
     """One coarse step of amr_step (amr/amr_step.f90) driving the
     reference's compiled routines, with refinement (refine_fine at
     levelmin / second substeps, flag_fine after each level step).
     nsub: {level: 1 or 2} (default 2 everywhere); stats["cells"] counts
-    8 cells per oct per godunov_fine call."""
+    8 cells per oct per godunov_fine call; clock["t"] / clock["nstep"]
+    advance as update_time advances t and nstep (amr/update_time.f90:330-332)
+    where amr_step calls it (amr/amr_step.f90:357,363)."""
     ns = nsub or {}
+
+    def update_time(l):
+        if clock is not None:
+            clock["t"] = clock["t"] + dt[l]
+            clock["nstep"] = clock["nstep"] + 1
 
     def n(l):
         return len(A.grids(l)) if 1 <= l <= nlevelmax else 0
@@ -88,6 +95,9 @@ def reference_amr_step(A, levelmin, nlevelmax, dt, dtold, adaptive=True, smallc=
         elif l < nlevelmax:
             dtold[l + 1] = dt[l] / float(ns.get(l, 2))
             dt[l + 1] = dt[l] / float(ns.get(l, 2))
+            update_time(l)
+        else:
+            update_time(l)
         A.godunov(l, dt[l])
         if stats is not None:
             stats["cells"] = stats.get("cells", 0) + 8 * n(l)
