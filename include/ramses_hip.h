@@ -304,6 +304,19 @@ typedef struct ramses_amr_mesh {
   long long nstep, nstep_coarse, cell_updates;
   void *work;
   unsigned long work_bytes;
+  /* Captured mode (graph != 0): the whole coarse step is captured once as a
+   * hipGraph and replayed.  Every level kernel then reads its oct count from
+   * device memory and loops over a fixed grid, refine_fine decides on the
+   * device with no host synchronisation, and one copy in and one copy out
+   * per step carry noct, the free list and the counters.  The graph is
+   * captured again when a pointer, a size, nsubcycle or a configuration
+   * call changes.  Its workspace holds graph_capacity octs per level (0:
+   * ngridmax; about 13.4 KB per oct); a refinement that would take a level
+   * beyond it is refused and the step returns -10.  Results are identical to
+   * the direct mode. */
+  int graph;
+  int graph_capacity;
+  void *graph_state; /* library-owned: NULL before the first step */
 } ramses_amr_mesh;
 
 /* amr_step(levelmin, 1) (amr/amr_step.f90) followed by the end of a coarse
@@ -315,7 +328,8 @@ typedef struct ramses_amr_mesh {
  * nstep counts finest-level steps (amr/update_time.f90:332). */
 int ramses_amr_coarse_step(ramses_amr_mesh *m, void *stream);
 
-/* Frees the workspace a mesh's coarse steps allocated. */
+/* Frees the workspace (and the captured graph) a mesh's coarse steps
+ * allocated. */
 int ramses_amr_mesh_release(ramses_amr_mesh *m);
 
 /* ---------------------------------------------------------------------

@@ -62,7 +62,8 @@ class ramses_amr_mesh(ctypes.Structure):
                 ("levelmin", ctypes.c_int), ("nlevelmax", ctypes.c_int),
                 ("nsubcycle", ctypes.c_int * (MAX_LEVELS + 1)), ("adaptive", ctypes.c_int),
                 ("boxlen", ctypes.c_double), ("nstep", ctypes.c_longlong), ("nstep_coarse", ctypes.c_longlong),
-                ("cell_updates", ctypes.c_longlong), ("work", ctypes.c_void_p), ("work_bytes", ctypes.c_ulong)]
+                ("cell_updates", ctypes.c_longlong), ("work", ctypes.c_void_p), ("work_bytes", ctypes.c_ulong),
+                ("graph", ctypes.c_int), ("graph_capacity", ctypes.c_int), ("graph_state", ctypes.c_void_p)]
 
 
 def _bind(lib):
@@ -411,10 +412,15 @@ class AmrSolver(AmrLevels):
     refine=False the tree is static (the reference's `static` mode).
     """
 
-    def __init__(self, tree, u, nsubcycle=2, refine=False, **kw):
+    def __init__(self, tree, u, nsubcycle=2, refine=False, graph=False, graph_capacity=0, **kw):
         """nsubcycle: 1 or 2 for every level, or a list per level (index
-        0 is level 1), as &RUN_PARAMS nsubcycle (amr/amr_commons.f90:51)."""
+        0 is level 1), as &RUN_PARAMS nsubcycle (amr/amr_commons.f90:51).
+        graph: run each coarse step as the library's captured hipGraph (one
+        launch and one synchronisation per step; identical results), with a
+        workspace for graph_capacity octs per level (0: ngridmax)."""
         super().__init__(tree, u, **kw)
+        self.graph = bool(graph)
+        self.graph_capacity = int(graph_capacity)
         import torch
         nsub = [nsubcycle] * (self.nlevelmax + 1) if isinstance(nsubcycle, int) else list(nsubcycle)
         nsub = (nsub + [2] * (self.nlevelmax + 1))[:self.nlevelmax + 1]
@@ -522,6 +528,7 @@ class AmrSolver(AmrLevels):
         m.adaptive = 1 if self.adaptive else 0
         m.boxlen = self.boxlen
         m.nstep, m.nstep_coarse, m.cell_updates = self.fine_steps, self.nstep_coarse, self.cell_updates
+        m.graph, m.graph_capacity = (1 if self.graph else 0), self.graph_capacity
         rc = self.lib.ramses_amr_coarse_step(ctypes.byref(m), self._s())
         for l in range(1, self.nlevelmax + 1):
             self.nocts[l] = m.noct[l]

@@ -24,15 +24,15 @@ namespace ramses {
 
 namespace {
 
-// Thread = (batch, father neighbour j in 0..26, lane); workgroup = one batch.
-__global__ __launch_bounds__(1024) void amr_gather(AmrLevelArgs a) {
+// Thread = (batch, father neighbour j in 0..26, lane); workgroup = one batch
+// (grid-stride over the batches when the oct count is read from device memory).
+__device__ __forceinline__ void gather_task(const AmrLevelArgs& a, int noct, int b) {
   const int nv = a.nv;
   const int lane = threadIdx.x % nv;
   const int j = threadIdx.x / nv;
   if (j >= 27) return;
-  const int b = blockIdx.x;
   const int oct = b * nv + lane;
-  if (oct >= a.noct) return;
+  if (oct >= noct) return;
   const AmrTreeView& t = a.t;
   const int igrid = a.active[oct];
   const long f = t.father[igrid];
@@ -89,19 +89,20 @@ __global__ __launch_bounds__(1024) void amr_gather(AmrLevelArgs a) {
   }
 }
 
+__global__ __launch_bounds__(1024) void amr_gather(AmrLevelArgs a) {
+  const int noct = level_count(a.noct, a.dnoct);
+  const int nb = (noct + a.nv - 1) / a.nv;
+  for (int b = blockIdx.x; b < nb; b += gridDim.x) gather_task(a, noct, b);
+}
+
 __device__ __forceinline__ size_t fslot(int i3, int j3, int k3) { return (size_t)((i3 - 1) + 3 * ((j3 - 1) + 3 * (k3 - 1))); }
 __device__ __forceinline__ size_t sslot(int i3, int j3, int k3) {
   return (size_t)((i3 + 1) + 6 * ((j3 + 1) + 6 * (k3 + 1)));
 }
 
-// Flux reset at faces touching a refined cell, then the level update
-// unew += flux(i3) - flux(i3+1) per dimension.  Workgroup = one nvector
-// batch: first one thread per (face, lane) resets fluxes, then one thread
-// per (cell, variable, lane) accumulates the three dimensions in order.
-__global__ __launch_bounds__(256) void amr_update(AmrLevelArgs a) {
+__device__ __forceinline__ void update_batch(const AmrLevelArgs& a, int noct, int b) {
   const int nv = a.nv;
-  const int b = blockIdx.x;
-  const int nlane = min(nv, a.noct - b * nv);
+  const int nlane = min(nv, noct - b * nv);
   const AmrTreeView& t = a.t;
   const size_t fvs = (size_t)nv * 27;
   double* flux = a.flux + (size_t)b * fvs * 15;
@@ -139,6 +140,17 @@ __global__ __launch_bounds__(256) void amr_update(AmrLevelArgs a) {
   }
 }
 
+// Flux reset at faces touching a refined cell, then the level update
+// unew += flux(i3) - flux(i3+1) per dimension.  Workgroup = one nvector
+// batch at a time: first one thread per (face, lane) resets fluxes, then one
+// thread per (cell, variable, lane) accumulates the three dimensions in
+// order.
+__global__ __launch_bounds__(256) void amr_update(AmrLevelArgs a) {
+  const int noct = level_count(a.noct, a.dnoct);
+  const int nb = (noct + a.nv - 1) / a.nv;
+  for (int b = blockIdx.x; b < nb; b += gridDim.x) update_batch(a, noct, b);
+}
+
 // Conservative correction of the coarser leaf cells next to the level's
 // octs.  Thread = (one cell of a coarse-level oct, one variable): the tree
 // walk is repeated per variable, which keeps five times more threads in
@@ -149,9 +161,7 @@ __global__ __launch_bounds__(256) void amr_update(AmrLevelArgs a) {
 // loop order: batch of o, then dimension, then side, then face (k3, j3, i3);
 // the same order is rebuilt here from the position of o in the level's
 // active list.
-__global__ void amr_reflux(AmrLevelArgs a) {
-  const long tix = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tix >= 40L * a.noct_coarse) return;
+__device__ __forceinline__ void reflux_task(const AmrLevelArgs& a, long tix) {
   const long cidx = tix / 5;
   const int v = (int)(tix % 5);
   const AmrTreeView& t = a.t;
@@ -217,13 +227,23 @@ __global__ void amr_reflux(AmrLevelArgs a) {
   a.unew[(long)v * a.ustride + c] = s;
 }
 
+__global__ void amr_reflux(AmrLevelArgs a) {
+  const long n = 40L * level_count(a.noct_coarse, a.dnoct_coarse);
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long tix = (long)blockIdx.x * blockDim.x + threadIdx.x; tix < n; tix += stride) reflux_task(a, tix);
+}
+
 // set_unew / set_uold over the level's active cells (hydro/godunov_fine.f90:58-65, 193-197).
-__global__ void amr_copy_level(const int* active, int noct, AmrTreeView t, const double* src, double* dst,
-                               long ustride) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= 8L * noct) return;
-  const long c = cell_of(t, (int)(i / noct) + 1, active[i % noct]);
-  for (int v = 0; v < 5; ++v) dst[(long)v * ustride + c] = src[(long)v * ustride + c];
+// The per-cell kernels below take the oct count as a host value or, with
+// dn set, from device memory (grid-stride loops over a persistent grid).
+__global__ void amr_copy_level(const int* active, int noct_h, const int* dn, AmrTreeView t, const double* src,
+                               double* dst, long ustride) {
+  const int noct = level_count(noct_h, dn);
+  const long n = 8L * noct, stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const long c = cell_of(t, (int)(i / noct) + 1, active[i % noct]);
+    for (int v = 0; v < 5; ++v) dst[(long)v * ustride + c] = src[(long)v * ustride + c];
+  }
 }
 
 // upl for one refined cell c with son oct s: the average of its children
@@ -264,15 +284,14 @@ __device__ inline void upl_cell(const AmrTreeView& t, double* u, long ustride, l
 
 // upload_fine: refined cells of the level get the average of their children
 // (hydro/interpol_hydro.f90:5-67).
-__global__ void amr_upload(const int* active, int noct, AmrTreeView t, double* u, long ustride, double smallr,
-                           int ivar_mode) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= 8L * noct) return;
-  const int igrid = active[i / 8];
-  const long c = cell_of(t, (int)(i % 8) + 1, igrid);
-  const int s = t.son[c];
-  if (s <= 0) return;
-  upl_cell(t, u, ustride, c, s, smallr, ivar_mode);
+__global__ void amr_upload(const int* active, int noct_h, const int* dn, AmrTreeView t, double* u, long ustride,
+                           double smallr, int ivar_mode) {
+  const long n = 8L * level_count(noct_h, dn), stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const long c = cell_of(t, (int)(i % 8) + 1, active[i / 8]);
+    const int s = t.son[c];
+    if (s > 0) upl_cell(t, u, ustride, c, s, smallr, ivar_mode);
+  }
 }
 
 // set_uold followed by upload_fine of the level in one pass
@@ -280,32 +299,32 @@ __global__ void amr_upload(const int* active, int noct, AmrTreeView t, double* u
 // takes unew, a refined cell the average of its children's uold (set_uold
 // writes every cell and upload_fine then rewrites the refined ones, so the
 // result is the same).
-__global__ void amr_finish(const int* active, int noct, AmrTreeView t, const double* unew, double* u, long ustride,
-                           double smallr, int ivar_mode) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= 8L * noct) return;
-  const int igrid = active[i / 8];
-  const long c = cell_of(t, (int)(i % 8) + 1, igrid);
-  const int s = t.son[c];
-  if (s <= 0) {
+__global__ void amr_finish(const int* active, int noct_h, const int* dn, AmrTreeView t, const double* unew, double* u,
+                           long ustride, double smallr, int ivar_mode) {
+  const long n = 8L * level_count(noct_h, dn), stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const long c = cell_of(t, (int)(i % 8) + 1, active[i / 8]);
+    const int s = t.son[c];
+    if (s <= 0) {
 #pragma unroll
-    for (int v = 0; v < 5; ++v) u[(long)v * ustride + c] = unew[(long)v * ustride + c];
-    return;
+      for (int v = 0; v < 5; ++v) u[(long)v * ustride + c] = unew[(long)v * ustride + c];
+    } else {
+      upl_cell(t, u, ustride, c, s, smallr, ivar_mode);
+    }
   }
-  upl_cell(t, u, ustride, c, s, smallr, ivar_mode);
 }
 
 // courant_fine over the level's leaf cells (hydro/courant_fine.f90:44-131):
 // block max of the cmpdt wave speed, block dt limit min-combined (exact,
 // see cmpdt_w).
-__global__ void amr_courant(const int* active, int noct, AmrTreeView t, const double* u, long ustride, double dx,
-                            HydroParams hp, double* d_dt) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void amr_courant(const int* active, int noct_h, const int* dn, AmrTreeView t, const double* u,
+                            long ustride, double dx, HydroParams hp, double* d_dt) {
+  const long n = 8L * level_count(noct_h, dn), stride = (long)gridDim.x * blockDim.x;
   double w = 0.0;
-  if (i < 8L * noct) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const long c = cell_of(t, (int)(i % 8) + 1, active[i / 8]);
     if (t.son[c] == 0)
-      w = cmpdt_w(u[c], u[ustride + c], u[2 * ustride + c], u[3 * ustride + c], u[4 * ustride + c], hp);
+      w = fmax_f(w, cmpdt_w(u[c], u[ustride + c], u[2 * ustride + c], u[3 * ustride + c], u[4 * ustride + c], hp));
   }
   for (int off = 32; off > 0; off >>= 1) w = fmax_f(w, __shfl_xor(w, off, 64));
   __shared__ double red[4];
@@ -328,8 +347,18 @@ __global__ void amr_courant(const int* active, int noct, AmrTreeView t, const do
 //         t += dtnew(l) (:353-362, amr/update_time.f90:217,331)
 //   op 2  end of the step: dtnew(l-1) = dtnew(l) if nsubcycle(l-1) == 1, or
 //         dtold(l) + dtnew(l) after the second substep (:573-576)
+// With dn (the device oct counts, indexed by level) the operations are those
+// of a level step that runs only when level l has octs, and op 1 only when
+// no finer level does (amr/amr_step.f90:345-365); cnt, if set, counts
+// update_time calls (cnt[0], nstep) and cell updates (cnt[1]).
 __global__ void amr_dt_kernel(int op, double* dt, double* dtold, double* time, int l, int coarser, int nsub,
-                              int icount, double start) {
+                              int icount, double start, const int* dn, long long* cnt) {
+  if (dn) {
+    if (dn[l] == 0) return;
+    if (op == 1 && coarser && dn[l + 1] > 0) return;
+    if (cnt && op == 0) cnt[1] += 8LL * dn[l];
+    if (cnt && op == 1) cnt[0] += 1;
+  }
   if (op == 0) {
     dtold[l] = dt[l];
     double v = start;
@@ -347,6 +376,8 @@ __global__ void amr_dt_kernel(int op, double* dt, double* dtold, double* time, i
   }
 }
 
+__global__ void clear_bits(int* p, int mask) { p[0] &= ~mask; }
+
 __global__ void set_ints(int* p, int n, int v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -354,14 +385,24 @@ __global__ void set_ints(int* p, int n, int v) {
 
 }  // namespace
 
-hipError_t launch_unsplit(const UnsplitArgs& a, bool fp32, hipStream_t stream);
+hipError_t launch_unsplit(const UnsplitArgs& a, bool fp32, hipStream_t stream, long max_grid = 0);
+
+// Grid of a per-cell launch over n items: sized to n for a host count, a
+// fixed persistent grid for a count read from device memory.
+constexpr unsigned kPersistBlocks = 512;
+inline unsigned cell_grid(long n, const int* dn) {
+  return dn ? kPersistBlocks : (unsigned)((n + 255) / 256);
+}
 
 // gather + unsplit + update + reflux of one level; dt is read from d_dt.
+// With a.dnoct / a.dnoct_coarse the counts are read from device memory and
+// the grids are persistent (batch loops), so the launches can be captured.
 hipError_t launch_amr_godunov(const AmrLevelArgs& a, double dx, const double* d_dt, double* tmp, bool fp32,
                               hipStream_t s) {
-  if (a.noct <= 0) return hipSuccess;
+  const bool dev = a.dnoct != nullptr;
+  if (!dev && a.noct <= 0) return hipSuccess;
   const int nbatch = (a.noct + a.nv - 1) / a.nv;
-  hipLaunchKernelGGL(amr_gather, dim3(nbatch), dim3(27 * a.nv), 0, s, a);
+  hipLaunchKernelGGL(amr_gather, dim3(dev ? kPersistBlocks / 2 : (unsigned)nbatch), dim3(27 * a.nv), 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   UnsplitArgs u;
@@ -374,59 +415,62 @@ hipError_t launch_amr_godunov(const AmrLevelArgs& a, double dx, const double* d_
   u.d_dt = d_dt;
   u.ngroups = nbatch;
   u.ngrid_last = a.noct - (nbatch - 1) * a.nv;
+  u.d_noct = a.dnoct;
   u.nv = a.nv;
   u.hp = a.hp;
-  e = launch_unsplit(u, fp32, s);
+  e = launch_unsplit(u, fp32, s, kPersistBlocks);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(amr_update, dim3(nbatch), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(amr_update, dim3(dev ? kPersistBlocks : (unsigned)nbatch), dim3(256), 0, s, a);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  if (a.noct_coarse > 0 && a.active_coarse) {
+  if (a.active_coarse && (a.dnoct_coarse || a.noct_coarse > 0)) {
     const long n = 40L * a.noct_coarse;
-    hipLaunchKernelGGL(amr_reflux, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(amr_reflux, dim3(cell_grid(n, a.dnoct_coarse)), dim3(256), 0, s, a);
   }
   return hipGetLastError();
 }
 
 hipError_t launch_amr_copy(const int* active, int noct, const AmrTreeView& t, const double* src, double* dst,
-                           long ustride, hipStream_t s) {
-  if (noct <= 0) return hipSuccess;
-  const long n = 8L * noct;
-  hipLaunchKernelGGL(amr_copy_level, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, active, noct, t, src, dst,
+                           long ustride, hipStream_t s, const int* dn) {
+  if (!dn && noct <= 0) return hipSuccess;
+  hipLaunchKernelGGL(amr_copy_level, dim3(cell_grid(8L * noct, dn)), dim3(256), 0, s, active, noct, dn, t, src, dst,
                      ustride);
   return hipGetLastError();
 }
 
 hipError_t launch_amr_upload(const int* active, int noct, const AmrTreeView& t, double* u, long ustride,
-                             double smallr, int ivar_mode, hipStream_t s) {
-  if (noct <= 0) return hipSuccess;
-  const long n = 8L * noct;
-  hipLaunchKernelGGL(amr_upload, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, active, noct, t, u, ustride,
+                             double smallr, int ivar_mode, hipStream_t s, const int* dn) {
+  if (!dn && noct <= 0) return hipSuccess;
+  hipLaunchKernelGGL(amr_upload, dim3(cell_grid(8L * noct, dn)), dim3(256), 0, s, active, noct, dn, t, u, ustride,
                      smallr, ivar_mode);
   return hipGetLastError();
 }
 
 hipError_t launch_amr_courant(const int* active, int noct, const AmrTreeView& t, const double* u, long ustride,
-                              double dx, const HydroParams& hp, double* d_dt, hipStream_t s) {
-  if (noct <= 0) return hipSuccess;
-  const long n = 8L * noct;
-  hipLaunchKernelGGL(amr_courant, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, active, noct, t, u, ustride,
+                              double dx, const HydroParams& hp, double* d_dt, hipStream_t s, const int* dn) {
+  if (!dn && noct <= 0) return hipSuccess;
+  hipLaunchKernelGGL(amr_courant, dim3(cell_grid(8L * noct, dn)), dim3(256), 0, s, active, noct, dn, t, u, ustride,
                      dx, hp, d_dt);
   return hipGetLastError();
 }
 
 hipError_t launch_amr_finish(const int* active, int noct, const AmrTreeView& t, const double* unew, double* u,
-                             long ustride, double smallr, int ivar_mode, hipStream_t s) {
-  if (noct <= 0) return hipSuccess;
-  const long n = 8L * noct;
-  hipLaunchKernelGGL(amr_finish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, active, noct, t, unew, u,
+                             long ustride, double smallr, int ivar_mode, hipStream_t s, const int* dn) {
+  if (!dn && noct <= 0) return hipSuccess;
+  hipLaunchKernelGGL(amr_finish, dim3(cell_grid(8L * noct, dn)), dim3(256), 0, s, active, noct, dn, t, unew, u,
                      ustride, smallr, ivar_mode);
   return hipGetLastError();
 }
 
 hipError_t launch_amr_dt(int op, double* dt, double* dtold, double* time, int l, int coarser, int nsub, int icount,
-                         double start, hipStream_t s) {
-  hipLaunchKernelGGL(amr_dt_kernel, dim3(1), dim3(1), 0, s, op, dt, dtold, time, l, coarser, nsub, icount, start);
+                         double start, hipStream_t s, const int* dn, long long* cnt) {
+  hipLaunchKernelGGL(amr_dt_kernel, dim3(1), dim3(1), 0, s, op, dt, dtold, time, l, coarser, nsub, icount, start, dn,
+                     cnt);
+  return hipGetLastError();
+}
+
+hipError_t launch_clear_bits(int* p, int mask, hipStream_t s) {
+  hipLaunchKernelGGL(clear_bits, dim3(1), dim3(1), 0, s, p, mask);
   return hipGetLastError();
 }
 

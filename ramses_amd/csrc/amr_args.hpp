@@ -14,6 +14,11 @@ struct AmrTreeView {
   long ncell;          // ncoarse + 8*ngridmax
 };
 
+// A level's oct count: the host value, or (when d is set) the value in
+// device memory at kernel time, so that a launch sequence can be captured
+// once and replayed while the mesh changes.
+__device__ __forceinline__ int level_count(int n, const int* d) { return d ? *d : n; }
+
 struct AmrLevelArgs {
   AmrTreeView t;
   const double* uold;  // uold[v*(ncell+1) + icell]
@@ -21,6 +26,8 @@ struct AmrLevelArgs {
   long ustride;        // ncell + 1
   const int* active;   // active[a], a = 0..noct-1 (grid indices)
   int noct;
+  const int* dnoct;    // if set, noct is read from here
+  const int* dnoct_coarse;  // if set, noct_coarse is read from here
   int ilevel;
   const int* pos;      // pos[igrid] = position of the grid in its level's active list
   double* uin;         // batches (nbatch, 5, 6,6,6, nv)

@@ -5,8 +5,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 #include <string>
+#include <vector>
 #include <cstdint>
 
 #include "../../include/ramses_hip.h"
@@ -15,7 +17,7 @@
 
 namespace ramses {
 struct UnsplitArgs;
-hipError_t launch_unsplit(const UnsplitArgs& a, bool fp32, hipStream_t stream);
+hipError_t launch_unsplit(const UnsplitArgs& a, bool fp32, hipStream_t stream, long max_grid = 0);
 hipError_t launch_cmpdt(const double* uu, const double* gg, int nv, int ncell, double dx,
                         const HydroParams& hp, double* d_dt, hipStream_t stream);
 hipError_t launch_set_double(double* p, double v, hipStream_t s);
@@ -28,16 +30,17 @@ hipError_t launch_init_slot(double* state, double t_end, double start, hipStream
 hipError_t launch_amr_godunov(const AmrLevelArgs& a, double dx, const double* d_dt, double* tmp, bool fp32,
                               hipStream_t s);
 hipError_t launch_amr_copy(const int* active, int noct, const AmrTreeView& t, const double* src, double* dst,
-                           long ustride, hipStream_t s);
+                           long ustride, hipStream_t s, const int* dn = nullptr);
 hipError_t launch_amr_upload(const int* active, int noct, const AmrTreeView& t, double* u, long ustride,
-                             double smallr, int ivar_mode, hipStream_t s);
+                             double smallr, int ivar_mode, hipStream_t s, const int* dn = nullptr);
 hipError_t launch_amr_courant(const int* active, int noct, const AmrTreeView& t, const double* u, long ustride,
-                              double dx, const HydroParams& hp, double* d_dt, hipStream_t s);
+                              double dx, const HydroParams& hp, double* d_dt, hipStream_t s, const int* dn = nullptr);
 hipError_t launch_amr_dt(int op, double* dt, double* dtold, double* time, int l, int coarser, int nsub, int icount,
-                         double start, hipStream_t s);
+                         double start, hipStream_t s, const int* dn = nullptr, long long* cnt = nullptr);
 hipError_t launch_set_ints(int* p, int n, int v, hipStream_t s);
+hipError_t launch_clear_bits(int* p, int mask, hipStream_t s);
 hipError_t launch_amr_finish(const int* active, int noct, const AmrTreeView& t, const double* unew, double* u,
-                             long ustride, double smallr, int ivar_mode, hipStream_t s);
+                             long ustride, double smallr, int ivar_mode, hipStream_t s, const int* dn = nullptr);
 struct RefineParams {
   double err_grad_d, err_grad_u, err_grad_p;
   double floor_d, floor_u, floor_p;
@@ -50,8 +53,13 @@ struct TreeMut {
 };
 hipError_t launch_flag_fine(const int* active, int noct, const AmrTreeView& t, int ilevel, int icount, int levelmin,
                             int nlevelmax, const double* u, long us, int* flag1, int* flag2, const RefineParams& rp,
-                            const HydroParams& hp, hipStream_t s);
+                            const HydroParams& hp, hipStream_t s, const int* dn = nullptr);
 size_t refine_work_bytes(int noct, int noct_fine);
+hipError_t launch_refine_fine_dev(const int* active, const int* dn_l, int ilevel, const AmrTreeView& t,
+                                  const TreeMut& m, int* active_fine, int* dn_f, int* pos, int* ring, int* dfree,
+                                  int* ctrl, int* flag1, int* flag2, double* xg, double* uold, double* unew,
+                                  int interpol_type, int interpol_var, double smallr, void* work, int* d_error,
+                                  int capacity, hipStream_t s);
 hipError_t launch_refine_fine(const int* active, int noct, int ilevel, const AmrTreeView& t, const TreeMut& m,
                               int* active_fine, int* noct_fine, int* pos, int* ring, int* head, int* numbf,
                               int* used_mem, int* flag1, int* flag2, double* xg, double* uold, double* unew,
@@ -79,6 +87,9 @@ struct Config {
 
 std::mutex g_cfg_mu;
 Config g_cfg;
+// Bumped by every configuration call: a captured coarse step holds the
+// configuration of its capture and is captured again after a change.
+std::atomic<long> g_config_gen{0};
 
 int fail(int code, const std::string& msg) {
   g_err = msg;
@@ -150,6 +161,7 @@ int ramses_hydro_config(const ramses_hydro_params* p) {
   c.set = true;
   std::lock_guard<std::mutex> lk(g_cfg_mu);
   g_cfg = c;
+  ++g_config_gen;
   return 0;
 }
 
@@ -176,6 +188,7 @@ int ramses_unsplit_device(const double* uin, const double* gravin, double* flux,
   a.d_dt = nullptr;
   a.ngroups = ngroups;
   a.ngrid_last = ngrid_last;
+  a.d_noct = nullptr;
   a.nv = c.nvector;
   a.hp = c.hp;
   hipError_t e = ramses::launch_unsplit(a, c.precision == RAMSES_FP32_MIXED, (hipStream_t)stream);
@@ -280,6 +293,7 @@ int ramses_amr_config(int interpol_type, int interpol_var) {
   if (interpol_var < 0 || interpol_var > 2) return fail(-2, "interpol_var must be 0..2");
   g_interpol_type = interpol_type;
   g_interpol_var = interpol_var;
+  ++g_config_gen;
   return 0;
 }
 
@@ -307,6 +321,8 @@ int ramses_amr_godunov(const ramses_amr_tree* t, const ramses_amr_level* lv, con
   a.ustride = v.ncell + 1;
   a.active = lv->active;
   a.noct = lv->noct;
+  a.dnoct = nullptr;
+  a.dnoct_coarse = nullptr;
   a.ilevel = lv->ilevel;
   a.pos = lv->pos;
   const int nv = c.nvector;
@@ -375,6 +391,7 @@ static ramses::RefineParams g_refine = kRefineDefaults;
 }  // namespace
 
 int ramses_refine_config(const ramses_refine_params* p) {
+  ++g_config_gen;
   if (!p) {
     g_refine = kRefineDefaults;
     return 0;
@@ -531,6 +548,281 @@ int amr_step_rec(StepCtx& c, int l, int icount) {
   return 0;
 }
 
+// --- the captured coarse step -------------------------------------------
+
+// What a captured step reads and writes besides the mesh arrays: the level
+// counts, the free list, the refinement decision and the counters.  One
+// copy in before the replay and one copy out after it.
+struct AmrDevState {
+  int noct[RAMSES_MAX_LEVELS + 2];
+  int free[4];       // head, numbf, used_mem
+  int ctrl[4];       // refine decision: ncreate, nkill, ok
+  int error;         // *m->error after the step
+  int pad;
+  long long cnt[2];  // nstep, cell_updates
+};
+
+struct GraphState {
+  hipStream_t cap = nullptr;  // capture stream (default priority)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  AmrDevState* dev = nullptr;
+  AmrDevState* host = nullptr;  // pinned
+  void* work = nullptr;
+  size_t work_bytes = 0;
+  int capacity = 0;
+  std::vector<long> key;
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  void release() {
+    drop_graph();
+    if (work) (void)hipFree(work);
+    if (dev) (void)hipFree(dev);
+    if (host) (void)hipHostFree(host);
+    if (cap) (void)hipStreamDestroy(cap);
+    work = nullptr;
+    dev = nullptr;
+    host = nullptr;
+    cap = nullptr;
+  }
+};
+
+std::vector<long> graph_key(const ramses_amr_mesh* m, int capacity) {
+  std::vector<long> k;
+  auto P = [&](const void* q) { k.push_back((long)(uintptr_t)q); };
+  P(m->tree.son);
+  P(m->tree.father);
+  P(m->tree.nbor);
+  for (int v : {m->tree.ncoarse, m->tree.ngridmax, m->tree.nx, m->tree.ny, m->tree.nz}) k.push_back(v);
+  for (int l = 1; l <= m->nlevelmax; ++l) {
+    P(m->active[l]);
+    k.push_back(m->nsubcycle[l]);
+  }
+  for (const void* q : {(const void*)m->pos, (const void*)m->xg, (const void*)m->flag1, (const void*)m->flag2,
+                        (const void*)m->free.ring, (const void*)m->uold, (const void*)m->unew, (const void*)m->dt,
+                        (const void*)m->dtold, (const void*)m->time, (const void*)m->error})
+    P(q);
+  long bl;
+  std::memcpy(&bl, &m->boxlen, sizeof(bl));
+  for (long v : {(long)m->levelmin, (long)m->nlevelmax, (long)m->adaptive, bl, (long)capacity,
+                 (long)g_config_gen.load()})
+    k.push_back(v);
+  return k;
+}
+
+struct GCtx {
+  ramses_amr_mesh* m;
+  hipStream_t s;
+  Config c;
+  AmrDevState* dev;
+  void* work;
+  int capacity;
+  ramses::AmrTreeView v;
+  ramses::TreeMut tm;
+};
+
+// amr_step(l, icount) as a fixed launch sequence: every level step down to
+// nlevelmax is emitted, and each kernel of a level step does nothing when
+// that level has no octs at run time, which is when the reference does not
+// call it (amr/amr_step.f90:345-365).  The counts a step's kernels read do
+// not change during that step: refine_fine(i) changes only level i+1, and a
+// level's step refines only its own and finer levels.
+hipError_t gstep(GCtx& g, int l, int icount) {
+  ramses_amr_mesh* m = g.m;
+  int* dn = g.dev->noct;
+  const long us = g.v.ncell + 1;
+  hipError_t e;
+  if (m->adaptive && m->levelmin < m->nlevelmax && (l == m->levelmin || icount > 1)) {
+    for (int i = l; i < m->nlevelmax; ++i) {
+      e = ramses::launch_refine_fine_dev(m->active[i], dn + i, i, g.v, g.tm, m->active[i + 1], dn + i + 1, m->pos,
+                                         m->free.ring, g.dev->free, g.dev->ctrl, m->flag1, m->flag2, m->xg, m->uold,
+                                         m->unew, g_interpol_type, g_interpol_var, g.c.hp.smallr, g.work, m->error,
+                                         g.capacity, g.s);
+      if (e != hipSuccess) return e;
+    }
+  }
+  const double dx = __builtin_ldexp(1.0, -l) * m->boxlen;
+  double start = g.c.hp.courant_factor * dx / g.c.hp.smallc;
+  const double capdt = m->boxlen / g.c.hp.smallc;
+  if (capdt < start) start = capdt;
+  const int coarser = l > m->levelmin;
+  e = ramses::launch_amr_dt(0, m->dt, m->dtold, m->time, l, coarser, coarser ? m->nsubcycle[l - 1] : 1, 0, start, g.s,
+                            dn, g.dev->cnt);
+  if (e != hipSuccess) return e;
+  e = ramses::launch_amr_courant(m->active[l], 0, g.v, m->uold, us, dx, g.c.hp, m->dt + l, g.s, dn + l);
+  if (e != hipSuccess) return e;
+  e = ramses::launch_amr_copy(m->active[l], 0, g.v, m->uold, m->unew, us, g.s, dn + l);  // set_unew
+  if (e != hipSuccess) return e;
+  if (l < m->nlevelmax) {
+    if ((e = gstep(g, l + 1, 1)) != hipSuccess) return e;
+    if (m->nsubcycle[l] == 2 && (e = gstep(g, l + 1, 2)) != hipSuccess) return e;
+  }
+  e = ramses::launch_amr_dt(1, m->dt, m->dtold, m->time, l, l < m->nlevelmax, m->nsubcycle[l], 0, 0.0, g.s, dn,
+                            g.dev->cnt);
+  if (e != hipSuccess) return e;
+  ramses::AmrLevelArgs a;
+  a.t = g.v;
+  a.uold = m->uold;
+  a.unew = m->unew;
+  a.ustride = us;
+  a.active = m->active[l];
+  a.noct = 0;
+  a.dnoct = dn + l;
+  a.dnoct_coarse = l > 1 ? dn + l - 1 : nullptr;
+  a.ilevel = l;
+  a.pos = m->pos;
+  const int nv = g.c.nvector;
+  const size_t nb = (size_t)((g.capacity + nv - 1) / nv);
+  char* w = (char*)(((uintptr_t)g.work + 255) & ~(uintptr_t)255);
+  a.uin = (double*)w;
+  a.flux = a.uin + nb * nv * 216 * 5;
+  double* tmp = a.flux + nb * nv * 27 * 15;
+  a.ok = (unsigned char*)(tmp + nb * nv * 27 * 6);
+  a.nv = nv;
+  a.interpol_type = g_interpol_type;
+  a.interpol_var = g_interpol_var;
+  a.hp = g.c.hp;
+  a.active_coarse = l > 1 ? m->active[l - 1] : nullptr;
+  a.noct_coarse = 0;
+  a.error = m->error;
+  e = ramses::launch_amr_godunov(a, dx, m->dt + l, tmp, g.c.precision == RAMSES_FP32_MIXED, g.s);
+  if (e != hipSuccess) return e;
+  e = ramses::launch_amr_finish(m->active[l], 0, g.v, m->unew, m->uold, us, g.c.hp.smallr, g_interpol_var, g.s,
+                                dn + l);
+  if (e != hipSuccess) return e;
+  if (m->adaptive) {
+    const int ic = (l <= m->levelmin || m->nsubcycle[l - 1] == 2) ? icount : 2;
+    e = ramses::launch_flag_fine(m->active[l], 0, g.v, l, ic, m->levelmin, m->nlevelmax, m->uold, us, m->flag1,
+                                 m->flag2, g_refine, g.c.hp, g.s, dn + l);
+    if (e != hipSuccess) return e;
+  }
+  if (l > m->levelmin) {
+    e = ramses::launch_amr_dt(2, m->dt, m->dtold, m->time, l, 1, m->nsubcycle[l - 1], icount, 0.0, g.s, dn,
+                              g.dev->cnt);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// The whole coarse step (amr_step(levelmin, 1) and the end of adaptive_loop's
+// step) as launches on g.s, for capture.
+hipError_t gcoarse(GCtx& g) {
+  ramses_amr_mesh* m = g.m;
+  // the refusal bits (4: no free grid, 8: capacity) describe this step only
+  hipError_t e = ramses::launch_clear_bits(m->error, 4 | 8, g.s);
+  if (e != hipSuccess) return e;
+  e = gstep(g, m->levelmin, 1);
+  if (e != hipSuccess) return e;
+  if (m->adaptive && m->levelmin < m->nlevelmax) {
+    const long us = g.v.ncell + 1;
+    for (int l = m->levelmin - 1; l >= 1; --l)
+      if ((e = ramses::launch_amr_upload(m->active[l], 0, g.v, m->uold, us, g.c.hp.smallr, g_interpol_var, g.s,
+                                         g.dev->noct + l)) != hipSuccess)
+        return e;
+    for (int l = m->levelmin - 1; l >= 1; --l)
+      if ((e = ramses::launch_flag_fine(m->active[l], 0, g.v, l, 2, m->levelmin, m->nlevelmax, m->uold, us, m->flag1,
+                                        m->flag2, g_refine, g.c.hp, g.s, g.dev->noct + l)) != hipSuccess)
+        return e;
+    if ((e = ramses::launch_set_ints(m->flag1 + 1, m->tree.ncoarse, 1, g.s)) != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// Level steps a captured coarse step emits: sum over levels of the product
+// of the coarser levels' nsubcycle.
+long graph_level_steps(const ramses_amr_mesh* m) {
+  long tot = 0, mult = 1;
+  for (int l = m->levelmin; l <= m->nlevelmax; ++l) {
+    tot += mult;
+    mult *= m->nsubcycle[l];
+  }
+  return tot;
+}
+
+constexpr long kMaxGraphLevelSteps = 1023;
+
+int coarse_step_graph(ramses_amr_mesh* m, hipStream_t s) {
+  const Config c = current();
+  if (c.hp.slope_type == 3) return fail(-4, "slope_type 3 is not supported on the AMR path");
+  if (c.nvector * 27 > 1024) return fail(-8, "nvector too large for the AMR gather (max 37)");
+  if (graph_level_steps(m) > kMaxGraphLevelSteps)
+    return fail(-1, "captured mode: too many level steps per coarse step (reduce nlevelmax-levelmin)");
+  const int capacity = (m->graph_capacity > 0 && m->graph_capacity < m->tree.ngridmax) ? m->graph_capacity
+                                                                                      : m->tree.ngridmax;
+  for (int l = 1; l <= m->nlevelmax; ++l)
+    if (m->noct[l] > capacity) return fail(-10, "captured mode: a level holds more octs than graph_capacity");
+  ramses::AmrTreeView v;
+  int rc = amr_view(&m->tree, v);
+  if (rc) return rc;
+  GraphState* gs = static_cast<GraphState*>(m->graph_state);
+  hipError_t e;
+  if (!gs) {
+    gs = new GraphState();
+    m->graph_state = gs;
+    if ((e = hipStreamCreate(&gs->cap)) != hipSuccess) return hip_fail(e, "capture stream");
+    if ((e = hipMalloc(&gs->dev, sizeof(AmrDevState))) != hipSuccess) return hip_fail(e, "graph state");
+    if ((e = hipHostMalloc(&gs->host, sizeof(AmrDevState))) != hipSuccess) return hip_fail(e, "graph staging");
+  }
+  if (gs->capacity != capacity || !gs->work) {
+    if (gs->work) {
+      if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "graph workspace sync");
+      (void)hipFree(gs->work);
+      gs->work = nullptr;
+    }
+    size_t need = amr_work(capacity, c.nvector);
+    const size_t rneed = ramses::refine_work_bytes(capacity, capacity);
+    if (rneed > need) need = rneed;
+    if ((e = hipMalloc(&gs->work, need)) != hipSuccess) return hip_fail(e, "graph workspace");
+    gs->work_bytes = need;
+    gs->capacity = capacity;
+    gs->drop_graph();
+  }
+  std::vector<long> key = graph_key(m, capacity);
+  if (!gs->exec || key != gs->key) {
+    gs->drop_graph();
+    GCtx g{m, gs->cap, c, gs->dev, gs->work, capacity, v, ramses::TreeMut{m->tree.son, m->tree.father, m->tree.nbor}};
+    if ((e = hipStreamBeginCapture(gs->cap, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+      return hip_fail(e, "begin capture");
+    hipError_t ge = gcoarse(g);
+    e = hipStreamEndCapture(gs->cap, &gs->graph);
+    if (ge != hipSuccess) return hip_fail(ge, "captured coarse step launch");
+    if (e != hipSuccess) return hip_fail(e, "end capture");
+    if ((e = hipGraphInstantiate(&gs->exec, gs->graph, nullptr, nullptr, 0)) != hipSuccess)
+      return hip_fail(e, "graph instantiate");
+    gs->key = key;
+  }
+  AmrDevState* h = gs->host;
+  std::memset(h, 0, sizeof(AmrDevState));
+  for (int l = 1; l <= m->nlevelmax; ++l) h->noct[l] = m->noct[l];
+  h->free[0] = m->free.head;
+  h->free[1] = m->free.numbf;
+  h->free[2] = m->free.used_mem;
+  h->cnt[0] = m->nstep;
+  h->cnt[1] = m->cell_updates;
+  if ((e = hipMemcpyAsync(gs->dev, h, sizeof(AmrDevState), hipMemcpyHostToDevice, s)) != hipSuccess)
+    return hip_fail(e, "graph state in");
+  if ((e = hipGraphLaunch(gs->exec, s)) != hipSuccess) return hip_fail(e, "graph launch");
+  if ((e = hipMemcpyAsync(&gs->dev->error, m->error, sizeof(int), hipMemcpyDeviceToDevice, s)) != hipSuccess)
+    return hip_fail(e, "graph error copy");
+  if ((e = hipMemcpyAsync(h, gs->dev, sizeof(AmrDevState), hipMemcpyDeviceToHost, s)) != hipSuccess)
+    return hip_fail(e, "graph state out");
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "graph step sync");
+  for (int l = 1; l <= m->nlevelmax; ++l) m->noct[l] = h->noct[l];
+  m->free.head = h->free[0];
+  m->free.numbf = h->free[1];
+  m->free.used_mem = h->free[2];
+  m->nstep = h->cnt[0];
+  m->cell_updates = h->cnt[1];
+  m->nstep_coarse += 1;
+  if (h->error & 4) return fail(-9, "No more free memory: increase ngridmax");
+  if (h->error & 8) return fail(-10, "captured mode: a refinement exceeded graph_capacity");
+  return 0;
+}
+
 }  // namespace
 
 int ramses_amr_coarse_step(ramses_amr_mesh* m, void* stream) {
@@ -545,6 +837,7 @@ int ramses_amr_coarse_step(ramses_amr_mesh* m, void* stream) {
     if (m->nsubcycle[l] != 1 && m->nsubcycle[l] != 2) return fail(-2, "nsubcycle must be 1 or 2");
   }
   if (!(m->boxlen > 0)) return fail(-1, "boxlen must be > 0");
+  if (m->graph) return coarse_step_graph(m, (hipStream_t)stream);
   StepCtx c{m, (hipStream_t)stream, current().hp};
   int rc = amr_step_rec(c, m->levelmin, 1);
   if (rc) return rc;
@@ -567,6 +860,12 @@ int ramses_amr_coarse_step(ramses_amr_mesh* m, void* stream) {
 
 int ramses_amr_mesh_release(ramses_amr_mesh* m) {
   if (!m) return 0;
+  if (m->graph_state) {
+    GraphState* gs = static_cast<GraphState*>(m->graph_state);
+    gs->release();
+    delete gs;
+    m->graph_state = nullptr;
+  }
   if (m->work) {
     hipError_t e = hipFree(m->work);
     m->work = nullptr;
@@ -578,6 +877,7 @@ int ramses_amr_mesh_release(ramses_amr_mesh* m) {
 
 int ramses_tune(int knob, int value) {
   ramses::Tuning& t = ramses::tuning();
+  ++g_config_gen;
   switch (knob) {
     case RAMSES_TUNE_UNSPLIT_BLOCK:
       if (value < 0 || value > 768 || value % 64) return fail(-2, "unsplit block must be a multiple of 64 in [64, 768]");

@@ -49,33 +49,39 @@ namespace {
 
 // --- flag_fine ----------------------------------------------------------
 
-// Cells of the level in cell-major order: i -> (ind = i / noct + 1, a = i % noct).
-__device__ __forceinline__ bool level_cell(const int* active, int noct, const AmrTreeView& t, long i, int& g,
+// Cells of the level in cell-major order: i -> (ind = i / noct + 1, a = i %
+// noct).  The flag kernels loop over them grid-stride; the oct count is the
+// host value or, with dn set, the one in device memory.
+__device__ __forceinline__ void level_cell(const int* active, int noct, const AmrTreeView& t, long i, int& g,
                                            int& ind, long& c) {
-  if (i >= 8L * noct) return false;
   ind = (int)(i / noct) + 1;
   g = active[i % noct];
   c = cell_of(t, ind, g);
-  return true;
 }
 
-__global__ void flag_init(const int* active, int noct, AmrTreeView t, int* flag1, int all) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  int g, ind;
-  long c;
-  if (!level_cell(active, noct, t, i, g, ind, c)) return;
-  if (all) {  // below levelmin every cell is flagged (init_flag :150-157)
-    flag1[c] = 1;
-    return;
-  }
-  const int s = t.son[c];
-  int f = 0;
-  if (s > 0)
-    for (int k = 1; k <= 8; ++k) {
-      const long cc = cell_of(t, k, s);
-      if (t.son[cc] > 0 || flag1[cc] == 1) f = 1;
+#define FOR_LEVEL_CELLS(noct_h, dn)                                                             \
+  const int noct = level_count(noct_h, dn);                                                     \
+  const long ncells_ = 8L * noct, stride_ = (long)gridDim.x * blockDim.x;                       \
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < ncells_; i += stride_)
+
+__global__ void flag_init(const int* active, int noct_h, const int* dn, AmrTreeView t, int* flag1, int all) {
+  FOR_LEVEL_CELLS(noct_h, dn) {
+    int g, ind;
+    long c;
+    level_cell(active, noct, t, i, g, ind, c);
+    if (all) {  // below levelmin every cell is flagged (init_flag :150-157)
+      flag1[c] = 1;
+      continue;
     }
-  flag1[c] = f;
+    const int s = t.son[c];
+    int f = 0;
+    if (s > 0)
+      for (int k = 1; k <= 8; ++k) {
+        const long cc = cell_of(t, k, s);
+        if (t.son[cc] > 0 || flag1[cc] == 1) f = 1;
+      }
+    flag1[c] = f;
+  }
 }
 
 // One pass of smooth_fine's dilation (amr/flag_utils.f90:555-677): a cell
@@ -83,24 +89,27 @@ __global__ void flag_init(const int* active, int noct, AmrTreeView t, int* flag1
 // the level were (flag1[0] == 0 stands for a missing neighbour).  The
 // reference counts into flag2 and merges into flag1; reading `in` and
 // writing `out` gives the same Jacobi pass in one launch.
-__global__ void flag_smooth(const int* active, int noct, AmrTreeView t, const int* in, int* out, int nthr) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  int g, ind;
-  long c;
-  if (!level_cell(active, noct, t, i, g, ind, c)) return;
-  int n = 0;
+__global__ void flag_smooth(const int* active, int noct_h, const int* dn, AmrTreeView t, const int* in, int* out,
+                            int nthr) {
+  FOR_LEVEL_CELLS(noct_h, dn) {
+    int g, ind;
+    long c;
+    level_cell(active, noct, t, i, g, ind, c);
+    int n = 0;
 #pragma unroll
-  for (int nb = 1; nb <= 6; ++nb) n += in[nbor_cell(t, g, ind, nb)];
-  out[c] = (in[c] == 1 || n >= nthr) ? 1 : 0;
+    for (int nb = 1; nb <= 6; ++nb) n += in[nbor_cell(t, g, ind, nb)];
+    out[c] = (in[c] == 1 || n >= nthr) ? 1 : 0;
+  }
 }
 
 // Copy of the level's flags from one buffer to the other.
-__global__ void flag_copy(const int* active, int noct, AmrTreeView t, const int* in, int* out) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  int g, ind;
-  long c;
-  if (!level_cell(active, noct, t, i, g, ind, c)) return;
-  out[c] = in[c];
+__global__ void flag_copy(const int* active, int noct_h, const int* dn, AmrTreeView t, const int* in, int* out) {
+  FOR_LEVEL_CELLS(noct_h, dn) {
+    int g, ind;
+    long c;
+    level_cell(active, noct, t, i, g, ind, c);
+    out[c] = in[c];
+  }
 }
 
 struct Prim3 {
@@ -122,12 +131,11 @@ __device__ __forceinline__ double grad_err(double g, double m, double d, double 
   return 2.0 * fmax_f(__builtin_fabs((d - m) / ((d + m) + floor_x)), __builtin_fabs((m - g) / ((m + g) + floor_x)));
 }
 
-__global__ void flag_hydro(const int* active, int noct, AmrTreeView t, const double* u, long us, int* flag1,
-                           RefineParams rp, HydroParams hp) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ void flag_hydro_cell(const int* active, int noct, const AmrTreeView& t, const double* u, long us,
+                                int* flag1, const RefineParams& rp, const HydroParams& hp, long i) {
   int g, ind;
   long c;
-  if (!level_cell(active, noct, t, i, g, ind, c)) return;
+  level_cell(active, noct, t, i, g, ind, c);
   bool ok = false;
   const Prim3 m = refine_prim(u, us, c, hp);
   for (int idim = 0; idim < 3; ++idim) {
@@ -154,16 +162,22 @@ __global__ void flag_hydro(const int* active, int noct, AmrTreeView t, const dou
   if (ok) flag1[c] = 1;
 }
 
+__global__ void flag_hydro(const int* active, int noct_h, const int* dn, AmrTreeView t, const double* u, long us,
+                           int* flag1, RefineParams rp, HydroParams hp) {
+  FOR_LEVEL_CELLS(noct_h, dn) flag_hydro_cell(active, noct, t, u, us, flag1, rp, hp, i);
+}
+
 // Thread = (grid, father neighbour j of 27): a missing or unrefined
 // neighbour unflags all cells of the grid (every such thread writes the
 // same zeros), as ensure_ref_rules does for a grid with any such neighbour.
-__global__ void flag_ensure(const int* active, int noct, AmrTreeView t, int ilevel, int* flag1) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= 27L * noct) return;
-  const int g = active[i / 27], j = (int)(i % 27);
-  const long nf = nbor_father_cell(t, t.father[g], j % 3, (j / 3) % 3, j / 9, ilevel);
-  if (nf == 0 || t.son[nf] == 0)
-    for (int ind = 1; ind <= 8; ++ind) flag1[cell_of(t, ind, g)] = 0;
+__global__ void flag_ensure(const int* active, int noct_h, const int* dn, AmrTreeView t, int ilevel, int* flag1) {
+  const long n = 27L * level_count(noct_h, dn), stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int g = active[i / 27], j = (int)(i % 27);
+    const long nf = nbor_father_cell(t, t.father[g], j % 3, (j / 3) % 3, j / 9, ilevel);
+    if (nf == 0 || t.son[nf] == 0)
+      for (int ind = 1; ind <= 8; ++ind) flag1[cell_of(t, ind, g)] = 0;
+  }
 }
 
 // --- exclusive scan of 64-bit counters ----------------------------------
@@ -193,37 +207,56 @@ __device__ long long block_excl_scan(long long x, long long* sh, long long& tota
   return base + v - x;
 }
 
-// out[i] = sum of in[0..i) within the tile; sums[tile] = tile total.
+// Length of a scanned array: a host value, or derived from a device oct count
+// (kind 1: the count itself; kind 2: the refine slots of that many octs).
+struct DevLen {
+  long n;
+  const int* d;
+  int kind;
+  __device__ __forceinline__ long get() const {
+    if (!d) return n;
+    const long c = *d;
+    return kind == 2 ? ((c + 31) / 32) * 256 : c;
+  }
+};
+
+// out[i] = sum of in[0..i) within the tile; sums[tile] = tile total.  Block b
+// scans tiles b, b + gridDim.x, ...
 __global__ __launch_bounds__(kScanThreads) void scan_tiles(const long long* in, long long* out, long long* sums,
-                                                            long n) {
+                                                            DevLen len) {
   __shared__ long long sh[kScanThreads / 64];
-  const long base = (long)blockIdx.x * kScanTile + (long)threadIdx.x * kScanItems;
-  long long v[kScanItems], s = 0;
-  for (int k = 0; k < kScanItems; ++k) {
-    v[k] = base + k < n ? in[base + k] : 0;
-    s += v[k];
-  }
-  long long total;
-  long long run = block_excl_scan(s, sh, total);
-  for (int k = 0; k < kScanItems; ++k) {
-    if (base + k < n) out[base + k] = run;
-    run += v[k];
-  }
-  if (threadIdx.x == 0) {
-    sums[blockIdx.x] = total;
-    if (gridDim.x == 1) {  // one tile: its exclusive offsets are final
-      sums[0] = 0;
-      sums[1] = total;
+  const long n = len.get();
+  const long ntiles = (n + kScanTile - 1) / kScanTile;
+  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const long base = tile * kScanTile + (long)threadIdx.x * kScanItems;
+    long long v[kScanItems], s = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+      v[k] = base + k < n ? in[base + k] : 0;
+      s += v[k];
+    }
+    long long total;
+    long long run = block_excl_scan(s, sh, total);
+    for (int k = 0; k < kScanItems; ++k) {
+      if (base + k < n) out[base + k] = run;
+      run += v[k];
+    }
+    if (threadIdx.x == 0) {
+      sums[tile] = total;
+      if (gridDim.x == 1 && ntiles == 1) {  // one tile: its exclusive offsets are final
+        sums[0] = 0;
+        sums[1] = total;
+      }
     }
   }
 }
 
 // One block: exclusive scan of the tile totals; sums[ntiles] = grand total.
-__global__ __launch_bounds__(1024) void scan_sums(long long* sums, int ntiles) {
+__global__ __launch_bounds__(1024) void scan_sums(long long* sums, DevLen len) {
   __shared__ long long sh[1024 / 64];
+  const long ntiles = (len.get() + kScanTile - 1) / kScanTile;
   long long carry = 0;
-  for (int b0 = 0; b0 < ntiles; b0 += 1024) {
-    const int i = b0 + threadIdx.x;
+  for (long b0 = 0; b0 < ntiles; b0 += 1024) {
+    const long i = b0 + threadIdx.x;
     const long long x = i < ntiles ? sums[i] : 0;
     long long total;
     const long long e = block_excl_scan(x, sh, total);
@@ -233,30 +266,32 @@ __global__ __launch_bounds__(1024) void scan_sums(long long* sums, int ntiles) {
   if (threadIdx.x == 0) sums[ntiles] = carry;
 }
 
-__global__ void scan_add(long long* out, const long long* sums, long n) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] += sums[i / kScanTile];
+__global__ void scan_add(long long* out, const long long* sums, DevLen len) {
+  const long n = len.get(), stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) out[i] += sums[i / kScanTile];
 }
 
 // --- refine_fine ----------------------------------------------------------
 
 // Slot p = batch*256 + (ind-1)*32 + lane over the level's active list in
 // nvector=32 batches.  mark = create (low 32 bits) | kill << 32.
-__global__ void refine_mark(const int* active, int noct, AmrTreeView t, const int* flag1, long long* mark,
-                            long nslot) {
-  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= nslot) return;
-  const int b = (int)(p >> 8), r = (int)(p & 255);
-  const int ind = (r >> 5) + 1, lane = r & 31;
-  const int a = b * 32 + lane;
-  long long m = 0;
-  if (a < noct) {
-    const long c = cell_of(t, ind, active[a]);
-    const int s = t.son[c], f = flag1[c];
-    if (f == 1 && s == 0) m = 1;
-    if (f == 0 && s > 0) m = 1LL << 32;
+__global__ void refine_mark(const int* active, int noct_h, const int* dn, AmrTreeView t, const int* flag1,
+                            long long* mark) {
+  const int noct = level_count(noct_h, dn);
+  const long nslot = (long)((noct + 31) / 32) * 256, stride = (long)gridDim.x * blockDim.x;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < nslot; p += stride) {
+    const int b = (int)(p >> 8), r = (int)(p & 255);
+    const int ind = (r >> 5) + 1, lane = r & 31;
+    const int a = b * 32 + lane;
+    long long m = 0;
+    if (a < noct) {
+      const long c = cell_of(t, ind, active[a]);
+      const int s = t.son[c], f = flag1[c];
+      if (f == 1 && s == 0) m = 1;
+      if (f == 0 && s > 0) m = 1LL << 32;
+    }
+    mark[p] = m;
   }
-  mark[p] = m;
 }
 
 struct RefineRun {
@@ -282,17 +317,27 @@ struct RefineRun {
   int interpol_type, interpol_var;
   double smallr;
   int* error;
+  // device-decided refinement (no host synchronisation): the level's oct
+  // count, the free list {head, numbf} and the decision {ncreate, nkill, ok}
+  // are read from device memory when these are set
+  const int* dnoct;
+  const int* dfree;
+  const int* ctrl;
+  __device__ __forceinline__ long slots() const {
+    return dnoct ? (long)((*dnoct + 31) / 32) * 256 : nslot;
+  }
+  __device__ __forceinline__ int head_() const { return dfree ? dfree[0] : head; }
+  __device__ __forceinline__ int numbf_() const { return dfree ? dfree[1] : numbf; }
 };
 
-__global__ void refine_make(RefineRun r) {
-  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= r.nslot || (r.mark[p] & 0xffffffffLL) == 0) return;
+__device__ void make_slot(const RefineRun& r, long p) {
+  if ((r.mark[p] & 0xffffffffLL) == 0) return;
   const int b = (int)(p >> 8), q = (int)(p & 255);
   const int ind = (q >> 5) + 1, lane = q & 31;
   const int igrid = r.active[b * 32 + lane];
   const long c = cell_of(r.t, ind, igrid);
   const int k = (int)(r.off[p] & 0xffffffffLL);
-  const int g = r.ring[(int)(((long)r.head + k) % r.cap)];
+  const int g = r.ring[(int)(((long)r.head_() + k) % r.cap)];
   r.newg[k] = g;
   r.m.son[c] = g;
   r.m.father[g] = (int)c;
@@ -320,9 +365,14 @@ __global__ void refine_make(RefineRun r) {
   }
 }
 
-__global__ void refine_kill(RefineRun r) {
-  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= r.nslot || (r.mark[p] >> 32) == 0) return;
+__global__ void refine_make(RefineRun r) {
+  if (r.ctrl && !r.ctrl[2]) return;
+  const long n = r.slots(), stride = (long)gridDim.x * blockDim.x;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) make_slot(r, p);
+}
+
+__device__ void kill_slot(const RefineRun& r, long p) {
+  if ((r.mark[p] >> 32) == 0) return;
   const int b = (int)(p >> 8), q = (int)(p & 255);
   const int ind = (q >> 5) + 1, lane = q & 31;
   const long c = cell_of(r.t, ind, r.active[b * 32 + lane]);
@@ -343,7 +393,38 @@ __global__ void refine_kill(RefineRun r) {
       r.unew[(long)v * r.us + cc] = 0.0;
     }
   }
-  r.ring[(int)(((long)r.head + r.numbf + k) % r.cap)] = g;
+  r.ring[(int)(((long)r.head_() + r.numbf_() + k) % r.cap)] = g;
+}
+
+__global__ void refine_kill(RefineRun r) {
+  if (r.ctrl && !r.ctrl[2]) return;
+  const long n = r.slots(), stride = (long)gridDim.x * blockDim.x;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) kill_slot(r, p);
+}
+
+// The refinement decision on the device, from the scanned marks: ctrl =
+// {ncreate, nkill, ok}; ok is 0 when there is nothing to do or when the
+// free list cannot hold the new octs ("No more free memory",
+// amr/refine_utils.f90:415-423: error bit 4, nothing is modified).
+// A refinement that would take the next level beyond `capacity` octs (the
+// workspace of a captured step) is refused the same way (error bit 8).
+__global__ void refine_decide(const long long* sums, DevLen len, const int* dfree, const int* dn_f, int capacity,
+                              int* ctrl, int* error) {
+  const long ntiles = (len.get() + kScanTile - 1) / kScanTile;
+  const long long tot = sums[ntiles];
+  const int nc = (int)(tot & 0xffffffffLL), nk = (int)(tot >> 32);
+  int ok = (nc > 0 || nk > 0) ? 1 : 0;
+  if (ok && nc >= dfree[1]) {
+    ok = 0;
+    atomicOr(error, 4);
+  }
+  if (ok && (long)(nk > 0 ? *dn_f - nk : *dn_f) + nc > capacity) {
+    ok = 0;
+    atomicOr(error, 8);
+  }
+  ctrl[0] = nc;
+  ctrl[1] = nk;
+  ctrl[2] = ok;
 }
 
 // Survivors of the next level's list (father still set) in list order.
@@ -367,29 +448,82 @@ __global__ void list_assemble(int* list, const int* tmp, int nalive, const int* 
   pos[g] = i;
 }
 
+// The same list maintenance with the counts and the decision in device
+// memory: dnf = the next level's count before this refinement.
+__global__ void list_alive_d(const int* list, const int* dnf, const int* father, long long* mark, const int* ctrl) {
+  if (!ctrl[2] || ctrl[1] == 0) return;  // nothing killed: the list keeps its order
+  const long n = *dnf, stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    mark[i] = father[list[i]] != 0 ? 1 : 0;
+}
+
+__global__ void list_compact_d(const int* list, const int* dnf, const long long* mark, const long long* off, int* tmp,
+                               const int* ctrl) {
+  if (!ctrl[2] || ctrl[1] == 0) return;
+  const long n = *dnf, stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    if (mark[i]) tmp[off[i]] = list[i];
+}
+
+__global__ void list_assemble_d(int* list, const int* tmp, const int* dnf, const int* newg, int* pos,
+                                const int* ctrl) {
+  if (!ctrl[2]) return;
+  const int nf = *dnf, nc = ctrl[0], nk = ctrl[1];
+  const int nalive = nk > 0 ? nf - nk : nf;
+  const long n = (long)nalive + nc, stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    if (i < nalive && nk == 0) continue;  // survivors already in place
+    const int g = i < nalive ? tmp[i] : newg[i - nalive];
+    list[i] = g;
+    pos[g] = (int)i;
+  }
+}
+
+// The new counts: next level, free-list head / numbf, used_mem.
+__global__ void refine_commit(int* dnf, int* dfree, const int* ctrl, int cap) {
+  if (!ctrl[2]) return;
+  const int nf = *dnf, nc = ctrl[0], nk = ctrl[1];
+  *dnf = (nk > 0 ? nf - nk : nf) + nc;
+  const int numbf = dfree[1];
+  dfree[0] = (int)(((long)dfree[0] + nc) % cap);
+  if (nc > 0) dfree[2] = cap - (numbf - nc);  // make_grid_fine's last used_mem
+  dfree[1] = numbf - nc + nk;
+}
+
 inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
 
 hipError_t scan64(const long long* in, long long* out, long long* sums, long n, hipStream_t s) {
   const long nt = (n + kScanTile - 1) / kScanTile;
   if (nt == 0) return hipSuccess;
-  hipLaunchKernelGGL(scan_tiles, dim3((unsigned)nt), dim3(kScanThreads), 0, s, in, out, sums, n);
+  const DevLen len{n, nullptr, 0};
+  hipLaunchKernelGGL(scan_tiles, dim3((unsigned)nt), dim3(kScanThreads), 0, s, in, out, sums, len);
   if (nt > 1) {
-    hipLaunchKernelGGL(scan_sums, dim3(1), dim3(1024), 0, s, sums, (int)nt);
-    hipLaunchKernelGGL(scan_add, dim3(nblk(n, 256)), dim3(256), 0, s, out, (const long long*)sums, n);
+    hipLaunchKernelGGL(scan_sums, dim3(1), dim3(1024), 0, s, sums, len);
+    hipLaunchKernelGGL(scan_add, dim3(nblk(n, 256)), dim3(256), 0, s, out, (const long long*)sums, len);
   }
+  return hipGetLastError();
+}
+
+// The scan with its length in device memory: persistent grids, and the
+// tile totals always scanned so that sums[ntiles] is the grand total.
+constexpr unsigned kPersist = 512;
+hipError_t scan64_dev(const long long* in, long long* out, long long* sums, DevLen len, hipStream_t s) {
+  hipLaunchKernelGGL(scan_tiles, dim3(kPersist / 2), dim3(kScanThreads), 0, s, in, out, sums, len);
+  hipLaunchKernelGGL(scan_sums, dim3(1), dim3(1024), 0, s, sums, len);
+  hipLaunchKernelGGL(scan_add, dim3(kPersist), dim3(256), 0, s, out, (const long long*)sums, len);
   return hipGetLastError();
 }
 
 }  // namespace
 
-// flag_fine(ilevel, icount) for one level (amr/flag_utils.f90:57-106).
+// flag_fine(ilevel, icount) for one level (amr/flag_utils.f90:57-106).  With
+// dn the oct count is read from device memory (persistent grids).
 hipError_t launch_flag_fine(const int* active, int noct, const AmrTreeView& t, int ilevel, int icount, int levelmin,
                             int nlevelmax, const double* u, long us, int* flag1, int* flag2, const RefineParams& rp,
-                            const HydroParams& hp, hipStream_t s) {
-  if (ilevel == nlevelmax || noct <= 0) return hipSuccess;
-  const long n = 8L * noct;
-  const unsigned nb = nblk(n, 256);
-  hipLaunchKernelGGL(flag_init, dim3(nb), dim3(256), 0, s, active, noct, t, flag1, ilevel < levelmin ? 1 : 0);
+                            const HydroParams& hp, hipStream_t s, const int* dn) {
+  if (ilevel == nlevelmax || (!dn && noct <= 0)) return hipSuccess;
+  const unsigned nb = dn ? kPersist : nblk(8L * noct, 256);
+  hipLaunchKernelGGL(flag_init, dim3(nb), dim3(256), 0, s, active, noct, dn, t, flag1, ilevel < levelmin ? 1 : 0);
   if (ilevel < levelmin) return hipGetLastError();
   // the flags ping-pong between flag1 and flag2, one launch per pass
   int* cur = flag1;
@@ -397,7 +531,8 @@ hipError_t launch_flag_fine(const int* active, int noct, const AmrTreeView& t, i
   auto smooth = [&]() {
     const int nthr[3] = {1, 2, 2};
     for (int k = 0; k < 3; ++k) {
-      hipLaunchKernelGGL(flag_smooth, dim3(nb), dim3(256), 0, s, active, noct, t, (const int*)cur, other, nthr[k]);
+      hipLaunchKernelGGL(flag_smooth, dim3(nb), dim3(256), 0, s, active, noct, dn, t, (const int*)cur, other,
+                         nthr[k]);
       int* tmp = cur;
       cur = other;
       other = tmp;
@@ -405,12 +540,13 @@ hipError_t launch_flag_fine(const int* active, int noct, const AmrTreeView& t, i
   };
   smooth();
   if (rp.err_grad_d >= 0.0 || rp.err_grad_p >= 0.0 || rp.err_grad_u >= 0.0)
-    hipLaunchKernelGGL(flag_hydro, dim3(nb), dim3(256), 0, s, active, noct, t, u, us, cur, rp, hp);
+    hipLaunchKernelGGL(flag_hydro, dim3(nb), dim3(256), 0, s, active, noct, dn, t, u, us, cur, rp, hp);
   for (int e = 0; e < rp.nexpand; ++e) smooth();
   if (cur != flag1)
-    hipLaunchKernelGGL(flag_copy, dim3(nb), dim3(256), 0, s, active, noct, t, (const int*)cur, flag1);
+    hipLaunchKernelGGL(flag_copy, dim3(nb), dim3(256), 0, s, active, noct, dn, t, (const int*)cur, flag1);
   if (ilevel > levelmin && icount < 2)
-    hipLaunchKernelGGL(flag_ensure, dim3(nblk(27L * noct, 256)), dim3(256), 0, s, active, noct, t, ilevel, flag1);
+    hipLaunchKernelGGL(flag_ensure, dim3(dn ? kPersist : nblk(27L * noct, 256)), dim3(256), 0, s, active, noct, dn,
+                       t, ilevel, flag1);
   return hipGetLastError();
 }
 
@@ -445,8 +581,8 @@ hipError_t launch_refine_fine(const int* active, int noct, int ilevel, const Amr
   long long* sums = off + nmax;  // nt entries
   int* newg = (int*)(sums + nt);
   int* tmp = newg + nslot;
-  hipLaunchKernelGGL(refine_mark, dim3(nblk(nslot, 256)), dim3(256), 0, s, active, noct, t, (const int*)flag1, mark,
-                     nslot);
+  hipLaunchKernelGGL(refine_mark, dim3(nblk(nslot, 256)), dim3(256), 0, s, active, noct, (const int*)nullptr, t,
+                     (const int*)flag1, mark);
   hipError_t e = scan64(mark, off, sums, nslot, s);
   if (e != hipSuccess) return e;
   const long ntiles = (nslot + kScanTile - 1) / kScanTile;
@@ -483,6 +619,9 @@ hipError_t launch_refine_fine(const int* active, int noct, int ilevel, const Amr
   r.interpol_var = interpol_var;
   r.smallr = smallr;
   r.error = d_error;
+  r.dnoct = nullptr;
+  r.dfree = nullptr;
+  r.ctrl = nullptr;
   if (ncreate > 0) hipLaunchKernelGGL(refine_make, dim3(nblk(nslot, 128)), dim3(128), 0, s, r);
   if (nkill > 0) hipLaunchKernelGGL(refine_kill, dim3(nblk(nslot, 256)), dim3(256), 0, s, r);
   if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -508,6 +647,74 @@ hipError_t launch_refine_fine(const int* active, int noct, int ilevel, const Amr
   if (ncreate > 0) *used_mem = t.ngridmax - (*numbf - ncreate);  // make_grid_fine's last value
   *numbf = *numbf - ncreate + nkill;
   return hipSuccess;
+}
+
+// refine_fine(ilevel) with no host synchronisation, for a captured launch
+// sequence: the level counts dn_l (level ilevel) and dn_f (level ilevel+1),
+// the free list dfree = {head, numbf, used_mem} and the decision ctrl[3] live
+// in device memory; work holds refine_work_bytes(capacity, capacity), every
+// level holding at most `capacity` octs.  A refinement that would exhaust
+// the free list sets error bit 4, one that would exceed the capacity error
+// bit 8; either changes nothing.
+hipError_t launch_refine_fine_dev(const int* active, const int* dn_l, int ilevel, const AmrTreeView& t,
+                                  const TreeMut& m, int* active_fine, int* dn_f, int* pos, int* ring, int* dfree,
+                                  int* ctrl, int* flag1, int* flag2, double* xg, double* uold, double* unew,
+                                  int interpol_type, int interpol_var, double smallr, void* work, int* d_error,
+                                  int capacity, hipStream_t s) {
+  const int cap = t.ngridmax;
+  const long nslot = (long)((capacity + 31) / 32) * 256;
+  const long nmax = nslot > capacity ? nslot : capacity;
+  const long nt = (nmax + kScanTile - 1) / kScanTile + 1;
+  char* w = (char*)(((uintptr_t)work + 255) & ~(uintptr_t)255);
+  long long* mark = (long long*)w;
+  long long* off = mark + nmax;
+  long long* sums = off + nmax;
+  int* newg = (int*)(sums + nt);
+  int* tmp = newg + nslot;
+  hipLaunchKernelGGL(refine_mark, dim3(kPersist), dim3(256), 0, s, active, 0, dn_l, t, (const int*)flag1, mark);
+  const DevLen slots{0, dn_l, 2};
+  hipError_t e = scan64_dev(mark, off, sums, slots, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(refine_decide, dim3(1), dim3(1), 0, s, (const long long*)sums, slots, (const int*)dfree,
+                     (const int*)dn_f, capacity, ctrl, d_error);
+  RefineRun r;
+  r.t = t;
+  r.m = m;
+  r.active = active;
+  r.noct = 0;
+  r.mark = mark;
+  r.off = off;
+  r.nslot = 0;
+  r.ring = ring;
+  r.cap = cap;
+  r.head = 0;
+  r.numbf = 0;
+  r.newg = newg;
+  r.uold = uold;
+  r.unew = unew;
+  r.us = t.ncell + 1;
+  r.flag1 = flag1;
+  r.flag2 = flag2;
+  r.xg = xg;
+  r.dx = __builtin_ldexp(1.0, -ilevel);
+  r.interpol_type = interpol_type;
+  r.interpol_var = interpol_var;
+  r.smallr = smallr;
+  r.error = d_error;
+  r.dnoct = dn_l;
+  r.dfree = dfree;
+  r.ctrl = ctrl;
+  hipLaunchKernelGGL(refine_make, dim3(kPersist), dim3(128), 0, s, r);
+  hipLaunchKernelGGL(refine_kill, dim3(kPersist), dim3(256), 0, s, r);
+  hipLaunchKernelGGL(list_alive_d, dim3(kPersist), dim3(256), 0, s, (const int*)active_fine, (const int*)dn_f,
+                     (const int*)m.father, mark, (const int*)ctrl);
+  if ((e = scan64_dev(mark, off, sums, DevLen{0, dn_f, 1}, s)) != hipSuccess) return e;
+  hipLaunchKernelGGL(list_compact_d, dim3(kPersist), dim3(256), 0, s, (const int*)active_fine, (const int*)dn_f,
+                     (const long long*)mark, (const long long*)off, tmp, (const int*)ctrl);
+  hipLaunchKernelGGL(list_assemble_d, dim3(kPersist), dim3(256), 0, s, active_fine, (const int*)tmp,
+                     (const int*)dn_f, (const int*)newg, pos, (const int*)ctrl);
+  hipLaunchKernelGGL(refine_commit, dim3(1), dim3(1), 0, s, dn_f, dfree, (const int*)ctrl, cap);
+  return hipGetLastError();
 }
 
 }  // namespace ramses

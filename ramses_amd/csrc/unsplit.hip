@@ -179,17 +179,16 @@ __device__ __forceinline__ void row_faces(const RowCtx<R>& c, const SolverConsts
   }
 }
 
+// One (group, row, lane) task of the flat decomposition.
 template <typename R, int RS, bool S27>
-__global__ __launch_bounds__(768) void unsplit_rows(UnsplitArgs a) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void unsplit_task(const UnsplitArgs& a, long t, long ngroups, int ngrid_last) {
   const int nv = a.nv;
   const long per_group = 12L * nv;
   const long gidx = t / per_group;
-  if (gidx >= a.ngroups) return;
   const int rem = (int)(t - gidx * per_group);
   const int row = rem / nv;
   const int l = rem - row * nv;
-  const int ngrid = (gidx == a.ngroups - 1) ? a.ngrid_last : nv;
+  const int ngrid = (gidx == ngroups - 1) ? ngrid_last : nv;
   if (l >= ngrid) return;
 
   const int d = row >> 2;               // direction of the faces
@@ -224,6 +223,23 @@ __global__ __launch_bounds__(768) void unsplit_rows(UnsplitArgs a) {
     row_faces<R, 2, RS, S27>(c, sc, a1, b1, (R)(dt / a.dx[2]), flux, tmp, fvs, make_scale(dt, a.dx[2]));
 }
 
+// Grid-stride over the (group, row, lane) tasks; with a.d_noct the group
+// count follows from the oct count in device memory at kernel time.
+template <typename R, int RS, bool S27>
+__global__ __launch_bounds__(768) void unsplit_rows(UnsplitArgs a) {
+  long ngroups = a.ngroups;
+  int ngrid_last = a.ngrid_last;
+  if (a.d_noct) {
+    const int n = *a.d_noct;
+    ngroups = (n + a.nv - 1) / a.nv;
+    ngrid_last = n - (int)(ngroups - 1) * a.nv;
+  }
+  const long total = ngroups * 12L * a.nv;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += stride)
+    unsplit_task<R, RS, S27>(a, t, ngroups, ngrid_last);
+}
+
 }  // namespace
 
 template <typename R, int RS, bool S27>
@@ -241,15 +257,16 @@ static void launch_r(const UnsplitArgs& a, long grid, int block, hipStream_t str
   }
 }
 
-hipError_t launch_unsplit(const UnsplitArgs& a, bool fp32, hipStream_t stream) {
+hipError_t launch_unsplit(const UnsplitArgs& a, bool fp32, hipStream_t stream, long max_grid) {
   const long threads = a.ngroups * 12L * a.nv;
-  if (threads <= 0) return hipSuccess;
+  if (threads <= 0 && !a.d_noct) return hipSuccess;
   // Any multiple of 64 works with the flat (batch, row, lane) mapping.  384 =
   // 12 rows x 32 lanes puts exactly one nvector=32 batch in a workgroup, so
   // re-reads of its stencil cells stay in one XCD's L2; 256 spreads a batch
   // over two workgroups.  The autotuner picks per device.
   const int block = tuning().unsplit_block ? tuning().unsplit_block : 256;
-  const long grid = (threads + block - 1) / block;
+  // with a device oct count the grid is a fixed, persistent size
+  const long grid = a.d_noct ? max_grid : (threads + block - 1) / block;
   if (fp32)
     launch_r<float>(a, grid, block, stream);
   else

@@ -64,11 +64,13 @@ def test_mesh_struct_layout_matches_header(tmp_path):
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     src = tmp_path / "m.c"
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "include/ramses_hip.h"\n'
-                   'int main(){printf("%zu %zu %zu %zu %zu\\n", sizeof(ramses_amr_mesh), offsetof(ramses_amr_mesh, free),'
-                   ' offsetof(ramses_amr_mesh, levelmin), offsetof(ramses_amr_mesh, boxlen),'
-                   ' offsetof(ramses_amr_mesh, work));return 0;}\n')
+                   'int main(){printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(ramses_amr_mesh),'
+                   ' offsetof(ramses_amr_mesh, free), offsetof(ramses_amr_mesh, levelmin),'
+                   ' offsetof(ramses_amr_mesh, boxlen), offsetof(ramses_amr_mesh, work),'
+                   ' offsetof(ramses_amr_mesh, graph), offsetof(ramses_amr_mesh, graph_state));return 0;}\n')
     exe = tmp_path / "m"
     subprocess.run(["gcc", "-I", root, str(src), "-o", str(exe)], check=True)
     got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     M = amr.ramses_amr_mesh
-    assert got == [ctypes.sizeof(M), M.free.offset, M.levelmin.offset, M.boxlen.offset, M.work.offset]
+    assert got == [ctypes.sizeof(M), M.free.offset, M.levelmin.offset, M.boxlen.offset, M.work.offset,
+                   M.graph.offset, M.graph_state.offset]

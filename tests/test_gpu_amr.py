@@ -144,7 +144,8 @@ def test_amr_uniform_tree_equals_uniform_level(gpu):
 
 
 @needs_ref
-def test_amr_subcycled_steps_vs_reference(gpu):
+@pytest.mark.parametrize("graph", [False, True])
+def test_amr_subcycled_steps_vs_reference(gpu, graph):
     """Three coarse steps of the subcycled multi-level step (levels 3-5,
     4 fine steps per coarse step) against the reference's own routines
     driven in amr_step's order: every cell and every level dt, bitwise."""
@@ -162,7 +163,7 @@ def test_amr_subcycled_steps_vs_reference(gpu):
     configure_interpolation(1, 0)
     A.put(0, u)
     A.put(1, u)
-    s = AmrSolver(tree, u)
+    s = AmrSolver(tree, u, graph=graph)
     dt = [0.0] * (tree.nlevel + 2)
     dtold = [0.0] * (tree.nlevel + 2)
     for k in range(3):
@@ -240,7 +241,8 @@ def test_flag_and_refine_vs_reference(gpu, egd, egp):
 
 
 @needs_ref
-def test_amr_adaptive_steps_vs_reference(gpu):
+@pytest.mark.parametrize("graph", [False, True])
+def test_amr_adaptive_steps_vs_reference(gpu, graph):
     """Three coarse steps of the adaptive multi-level step (refine at the
     start of the levelmin step and of every second substep, flag after
     every level step) against the reference's routines driven in
@@ -248,6 +250,7 @@ def test_amr_adaptive_steps_vs_reference(gpu):
     import ramses_amd as ra
     from amr_cases import reference_amr_step
     A, s, tree = _refine_setup(0.12, 0.25)
+    s.graph = graph
     for l in range(6, 0, -1):
         if s.noct(l):
             A.flag_fine(l, 2)
@@ -309,7 +312,8 @@ def test_adaptive_mesh_golden(gpu):
 
 
 @needs_ref
-def test_sedov_amr_init_and_steps_vs_reference(gpu):
+@pytest.mark.parametrize("graph", [False, True])
+def test_sedov_amr_init_and_steps_vs_reference(gpu, graph):
     """The adaptive Sedov set-up (namelist/sedov3d.nml regions, gamma 1.4,
     hllc, minmod, CFL 0.8, err_grad_d = err_grad_p = 0.1, interpol_type 1)
     at levelmin 3, levelmax 6: the initial mesh built by init_refine /
@@ -332,7 +336,7 @@ def test_sedov_amr_init_and_steps_vs_reference(gpu):
     A.set_refine(lmin, err_grad_d=0.1, err_grad_p=0.1)
     configure_interpolation(1, 0)
     configure_refinement(err_grad_d=0.1, err_grad_p=0.1)
-    s = AmrSolver(tree, None, refine=True, nlevelmax=L)
+    s = AmrSolver(tree, None, refine=True, nlevelmax=L, graph=graph)
     A.set_xg(s.xg[1:].cpu().numpy())
     A.set_sedov(1.0, 1e-5, 0.4, (0.5, 0.5, 0.5))
     reference_init_refine(A, lmin, L)
@@ -423,7 +427,8 @@ def test_snapshot_after_adaptive_steps_vs_reference(gpu):
 
 
 @needs_ref
-def test_sedov_amr_no_subcycling_vs_reference(gpu):
+@pytest.mark.parametrize("graph", [False, True])
+def test_sedov_amr_no_subcycling_vs_reference(gpu, graph):
     """nsubcycle=1 on every level (as namelist/sedov3d.nml sets it): one
     step per level per coarse step, the coarse dt taken from the finest
     level; init_refine and three coarse steps bitwise."""
@@ -444,7 +449,7 @@ def test_sedov_amr_no_subcycling_vs_reference(gpu):
     A.set_nsubcycle([1] * 10)
     configure_interpolation(1, 0)
     configure_refinement(err_grad_d=0.1, err_grad_p=0.1)
-    s = AmrSolver(tree, None, refine=True, nlevelmax=L, nsubcycle=1)
+    s = AmrSolver(tree, None, refine=True, nlevelmax=L, nsubcycle=1, graph=graph)
     A.set_xg(s.xg[1:].cpu().numpy())
     A.set_sedov(1.0, 1e-5, 0.4, (0.5, 0.5, 0.5))
     reference_init_refine(A, lmin, L)
@@ -653,7 +658,8 @@ def test_refine_out_of_grids_is_an_error_and_changes_nothing(gpu):
     ra.configure()
 
 
-def test_adaptive_fp32_mixed_close_to_fp64(gpu):
+@pytest.mark.parametrize("graph", [False, True])
+def test_adaptive_fp32_mixed_close_to_fp64(gpu, graph):
     """FP32-mixed adaptive coarse step: same refinement decisions here and
     a relative L1 difference on the leaves below 1e-5 per variable."""
     import ramses_amd as ra
@@ -665,7 +671,7 @@ def test_adaptive_fp32_mixed_close_to_fp64(gpu):
         ra.configure(gamma=5 / 3, riemann="hllc", slope_type=1, courant_factor=0.5, precision=prec)
         configure_interpolation(1, 0)
         configure_refinement(err_grad_d=0.12, err_grad_p=0.25)
-        s = AmrSolver(tree, u, refine=True, nlevelmax=6)
+        s = AmrSolver(tree, u, refine=True, nlevelmax=6, graph=graph)
         s.flag_all()
         s.coarse_step()
         s.check()
@@ -677,5 +683,63 @@ def test_adaptive_fp32_mixed_close_to_fp64(gpu):
     for v in range(5):
         a, b = u32[v][leaf], u64[v][leaf]
         assert np.abs(a - b).sum() / np.abs(b).sum() < 1e-5, v
+    configure_refinement()
+    ra.configure()
+
+
+def test_captured_step_recaptures_after_configuration_change(gpu):
+    """A captured coarse step holds the configuration it was captured with:
+    after a configuration call it is captured again, so three adaptive
+    steps with a change of courant_factor and of the refinement criteria
+    between them give the direct mode's mesh, state and dts bit for bit."""
+    import ramses_amd as ra
+    from ramses_amd.amr import AmrSolver, configure_interpolation, configure_refinement
+    tree = sphere_tree(extra_grids=60000)
+    u = smooth_state(tree, seed=3)
+    out = {}
+    for graph in (False, True):
+        ra.configure(gamma=5 / 3, riemann="hllc", slope_type=1, courant_factor=0.5)
+        configure_interpolation(1, 0)
+        configure_refinement(err_grad_d=0.12, err_grad_p=0.25)
+        s = AmrSolver(tree, u, refine=True, nlevelmax=6, graph=graph)
+        s.flag_all()
+        s.coarse_step()
+        ra.configure(gamma=5 / 3, riemann="hllc", slope_type=1, courant_factor=0.4)
+        s.coarse_step()
+        configure_refinement(err_grad_d=0.3, err_grad_p=0.5)
+        s.coarse_step()
+        s.check()
+        out[graph] = (s.tree_arrays(), {l: s.grids(l) for l in range(1, 7)}, s.free_list(), s.state("old"),
+                      s.dt.cpu().numpy(), float(s.time.item()), s.fine_steps, s.cell_updates, s.free.used_mem)
+        del s
+    a, b = out[False], out[True]
+    assert all(np.array_equal(x, y) for x, y in zip(a[0], b[0]))
+    assert all(np.array_equal(a[1][l], b[1][l]) for l in range(1, 7))
+    for x, y in zip(a[2:], b[2:]):
+        assert np.array_equal(x, y)
+    configure_refinement()
+    ra.configure()
+
+
+def test_captured_step_errors(gpu):
+    """Captured mode refuses a level larger than its workspace before it
+    runs, and a step whose refinement would exhaust the free list raises
+    the direct mode's error."""
+    import ramses_amd as ra
+    from ramses_amd.amr import AmrSolver, configure_interpolation, configure_refinement
+    tree = sphere_tree(extra_grids=20000)
+    u = smooth_state(tree, seed=3)
+    ra.configure(gamma=5 / 3, riemann="hllc", slope_type=1, courant_factor=0.5)
+    configure_interpolation(1, 0)
+    configure_refinement(err_grad_d=0.12, err_grad_p=0.25)
+    s = AmrSolver(tree, u, refine=True, nlevelmax=6, graph=True, graph_capacity=8)
+    s.flag_all()
+    with pytest.raises(ra.RamsesError, match="graph_capacity"):
+        s.coarse_step()
+    small = sphere_tree(extra_grids=50)
+    s = AmrSolver(small, smooth_state(small, seed=3), refine=True, nlevelmax=6, graph=True)
+    s.flag_all()
+    with pytest.raises(ra.RamsesError, match="free memory"):
+        s.coarse_step()
     configure_refinement()
     ra.configure()
