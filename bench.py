@@ -225,9 +225,10 @@ def amr_bench(torch, ra, args):
         configure_interpolation(1, 0)
         configure_refinement(err_grad_d=w["err_grad_d"], err_grad_p=w["err_grad_p"])
         for warm in (True, False):
-            s = AmrSolver(tree, None, refine=True, nlevelmax=lmax)
+            s = AmrSolver(tree, None, refine=True, nlevelmax=lmax, graph=args.amr_mode == "graph")
             s.init_refine(ic)
             init_octs = {l: s.noct(l) for l in range(lmin, lmax + 1)}
+            s.capture()  # graph mode: the one-time capture stays outside the timed region
             torch.cuda.synchronize()
             if warm:
                 for _ in range(max(1, args.warmup // 3)):
@@ -273,7 +274,8 @@ def amr_bench(torch, ra, args):
         "warmup": args.warmup, "ms_per_step": head["ms_per_coarse_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (Sedov initial conditions)",
         "config": {"workload": w["name"], "levelmin": lmin, "levelmax": lmax, "fine_steps": nfine,
-                   "parallelism": "1 GPU"},
+                   "parallelism": "1 GPU",
+                   "mode": "captured hipGraph per coarse step" if args.amr_mode == "graph" else "direct launches"},
         "roofline": head["roofline"], "variants": out, "fp32_same_mesh": same_mesh,
         "fp32_rel_l1_vs_fp64": err, "t_end": {k: v[2] for k, v in finals.items()},
         "fp32_over_fp64": out["fp32"]["cell_updates_per_s"] / head["cell_updates_per_s"],
@@ -399,6 +401,8 @@ def main():
                          "adaptive Sedov run")
     ap.add_argument("--fine-steps", type=int, default=0, help="sedov_amr: finest-level steps (default 50)")
     ap.add_argument("--levelmax", type=int, default=0, help="sedov_amr: deepest level (default 10)")
+    ap.add_argument("--amr-mode", default="graph", choices=["graph", "direct"],
+                    help="sedov_amr: coarse steps replayed from a captured hipGraph, or launched directly")
     ap.add_argument("--requests", type=int, default=2000, help="serve: number of solves")
     ap.add_argument("--load", default="1.2,0.8", help="serve: offered loads / measured capacity (comma list)")
     ap.add_argument("--steps-per-tick", type=int, default=8, help="serve: level steps per scheduler tick")

@@ -313,7 +313,8 @@ typedef struct ramses_amr_mesh {
    * call changes.  Its workspace holds graph_capacity octs per level (0:
    * ngridmax; about 13.4 KB per oct); a refinement that would take a level
    * beyond it is refused and the step returns -10.  Results are identical to
-   * the direct mode. */
+   * the direct mode.  graph == 2 captures (if needed) without running a step,
+   * so that a timed loop replays only. */
   int graph;
   int graph_capacity;
   void *graph_state; /* library-owned: NULL before the first step */

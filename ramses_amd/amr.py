@@ -507,6 +507,17 @@ class AmrSolver(AmrLevels):
             for l in range(L, 0, -1):
                 self.flag(l, 2, lmin)
 
+    def capture(self):
+        """Capture the coarse step's hipGraph now (graph mode), without
+        running a step; the next coarse_step only replays it."""
+        if not self.graph:
+            return
+        self.graph = 2
+        try:
+            self.coarse_step()
+        finally:
+            self.graph = True
+
     def coarse_step(self):
         """One coarse step of adaptive_loop (amr/adaptive_loop.f90:86-191):
         amr_step(levelmin, 1) and, with refinement on, upload_fine and
@@ -528,7 +539,7 @@ class AmrSolver(AmrLevels):
         m.adaptive = 1 if self.adaptive else 0
         m.boxlen = self.boxlen
         m.nstep, m.nstep_coarse, m.cell_updates = self.fine_steps, self.nstep_coarse, self.cell_updates
-        m.graph, m.graph_capacity = (1 if self.graph else 0), self.graph_capacity
+        m.graph, m.graph_capacity = int(self.graph), self.graph_capacity
         rc = self.lib.ramses_amr_coarse_step(ctypes.byref(m), self._s())
         for l in range(1, self.nlevelmax + 1):
             self.nocts[l] = m.noct[l]

@@ -795,6 +795,7 @@ int coarse_step_graph(ramses_amr_mesh* m, hipStream_t s) {
       return hip_fail(e, "graph instantiate");
     gs->key = key;
   }
+  if (m->graph == 2) return 0;  // capture only
   AmrDevState* h = gs->host;
   std::memset(h, 0, sizeof(AmrDevState));
   for (int l = 1; l <= m->nlevelmax; ++l) h->noct[l] = m->noct[l];
