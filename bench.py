@@ -219,7 +219,7 @@ def amr_bench(torch, ra, args):
     lmin, lmax = w["levelmin"], args.levelmax or w["levelmax"]
     tree = OctTree(lmin, lmin, extra_grids=w["ngridmax"] - sum(8 ** k for k in range(lmin)))
     ic = lambda x, dx: sedov_regions(x, dx, gamma=w["gamma"])  # noqa: E731
-    out, finals = {}, {}
+    out, finals, tuned = {}, {}, {}
     for prec in ("fp64", "fp32"):
         ra.configure(gamma=w["gamma"], riemann="hllc", slope_type=1, courant_factor=w["courant"], precision=prec)
         configure_interpolation(1, 0)
@@ -231,6 +231,9 @@ def amr_bench(torch, ra, args):
             s.capture()  # graph mode: the one-time capture stays outside the timed region
             torch.cuda.synchronize()
             if warm:
+                if s.graph and not args.no_tune:
+                    from ramses_amd.autotune import tune_amr_graph
+                    tuned[prec] = tune_amr_graph(s, ("amr_graph", lmin, lmax, prec))[0]
                 for _ in range(max(1, args.warmup // 3)):
                     s.coarse_step()
                 torch.cuda.synchronize()
@@ -277,6 +280,7 @@ def amr_bench(torch, ra, args):
                    "parallelism": "1 GPU",
                    "mode": "captured hipGraph per coarse step" if args.amr_mode == "graph" else "direct launches"},
         "roofline": head["roofline"], "variants": out, "fp32_same_mesh": same_mesh,
+        "autotuned_graph_blocks": tuned or None,
         "fp32_rel_l1_vs_fp64": err, "t_end": {k: v[2] for k, v in finals.items()},
         "fp32_over_fp64": out["fp32"]["cell_updates_per_s"] / head["cell_updates_per_s"],
         **({"cpu_baseline": cpu} if cpu else {}),

@@ -2,17 +2,20 @@
 
 The reference tunes one thing, the compile-time vector length nvector
 (amr/amr_parameters.f90:49-53).  Here the equivalent choices are the unsplit
-workgroup size and the resident-level tile / z-chunk, and they depend on the
-device and on the problem size, so they are measured: every candidate is
-timed with HIP events on the caller's stream (median of a few launches) on
-the real inputs, the fastest is installed with ``ramses_tune`` and the
-choice is remembered per (kernel, size, precision, solver) for the process.
+workgroup size, the resident-level tile / z-chunk and the fixed grid of a
+captured oct-tree coarse step, and they depend on the device and on the
+problem size, so they are measured: every candidate is timed with HIP
+events on the caller's stream (median of a few launches) on the real
+inputs, the fastest is installed with ``ramses_tune`` and the choice is
+remembered per (kernel, size, precision, solver) for the process.  (The
+oct-group size nvector itself stays 32 on the oct tree: it fixes the order
+of the coarse-cell refluxes, so changing it would change the results.)
 """
 import ctypes
 
 from . import _check, load_library
 
-UNSPLIT_BLOCK, LEVEL_TILE, LEVEL_TZ = 1, 2, 3
+UNSPLIT_BLOCK, LEVEL_TILE, LEVEL_TZ, PERSIST_BLOCKS = 1, 2, 3, 4
 _cache = {}
 
 
@@ -33,7 +36,7 @@ def _time(fn, reps=5):
         b.record()
         torch.cuda.synchronize()
         ts.append(a.elapsed_time(b))
-    ts = sorted(ts[1:])
+    ts = sorted(ts[1:]) if reps > 0 else ts
     return ts[len(ts) // 2]
 
 
@@ -80,6 +83,32 @@ def tune_level(run, n, key, verbose=False):
     return best
 
 
+def tune_amr_graph(solver, key, blocks=(256, 512, 1024, 2048), rounds=2, verbose=False):
+    """The fixed-grid size of a captured coarse step (AmrSolver with
+    graph=True), timed on the solver's own coarse steps: each candidate is
+    captured (untimed) and one step replayed, `rounds` times over the
+    candidates, keeping each one's fastest step.  The solver advances by
+    len(blocks) * rounds coarse steps.  Returns (blocks, ms)."""
+    import torch
+    if key in _cache:
+        _set(PERSIST_BLOCKS, _cache[key][0])
+        return _cache[key]
+    best = {}
+    for _ in range(rounds):
+        for b in blocks:
+            _set(PERSIST_BLOCKS, b)
+            solver.capture()
+            torch.cuda.synchronize()
+            ms = _time(solver.coarse_step, reps=0)
+            if verbose:
+                print(f"autotune amr graph blocks={b}: {ms:.3f} ms")
+            best[b] = min(best.get(b, ms), ms)
+    b = min(best, key=best.get)
+    _set(PERSIST_BLOCKS, b)
+    _cache[key] = (b, best[b])
+    return _cache[key]
+
+
 def reset():
-    for k in (UNSPLIT_BLOCK, LEVEL_TILE, LEVEL_TZ):
+    for k in (UNSPLIT_BLOCK, LEVEL_TILE, LEVEL_TZ, PERSIST_BLOCKS):
         _set(k, 0)

@@ -337,6 +337,36 @@ __global__ void amr_courant(const int* active, int noct_h, const int* dn, AmrTre
   }
 }
 
+// set_unew and courant_fine of a level in one pass over its cells
+// (hydro/godunov_fine.f90:40-130, hydro/courant_fine.f90:44-131): every
+// cell copies uold to unew, every leaf joins the block max of the cmpdt wave
+// speed, and the block's dt limit is min-combined into *d_dt (exact: the dt
+// limit is non-increasing in the wave speed).
+__global__ void amr_begin(const int* active, int noct_h, const int* dn, AmrTreeView t, const double* u, double* unew,
+                          long ustride, double dx, HydroParams hp, double* d_dt) {
+  const int noct = level_count(noct_h, dn);
+  const long n = 8L * noct, stride = (long)gridDim.x * blockDim.x;
+  double w = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const long c = cell_of(t, (int)(i / noct) + 1, active[i % noct]);
+    double q[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) q[v] = u[(long)v * ustride + c];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) unew[(long)v * ustride + c] = q[v];
+    if (t.son[c] == 0) w = fmax_f(w, cmpdt_w(q[0], q[1], q[2], q[3], q[4], hp));
+  }
+  for (int off = 32; off > 0; off >>= 1) w = fmax_f(w, __shfl_xor(w, off, 64));
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) m = fmax_f(m, red[k]);
+    if (m > 0.0) atomicMin(reinterpret_cast<unsigned long long*>(d_dt), dt_bits(cmpdt_from_w(m, dx, hp)));
+  }
+}
+
 // The level time-step bookkeeping of amr_step (amr/amr_step.f90), one
 // thread, so a coarse step keeps every dt on the device:
 //   op 0  newdt_fine prologue: dtold(l) = dtnew(l); dtnew(l) = start, capped
@@ -389,9 +419,8 @@ hipError_t launch_unsplit(const UnsplitArgs& a, bool fp32, hipStream_t stream, l
 
 // Grid of a per-cell launch over n items: sized to n for a host count, a
 // fixed persistent grid for a count read from device memory.
-constexpr unsigned kPersistBlocks = 512;
 inline unsigned cell_grid(long n, const int* dn) {
-  return dn ? kPersistBlocks : (unsigned)((n + 255) / 256);
+  return dn ? persist_blocks() : (unsigned)((n + 255) / 256);
 }
 
 // gather + unsplit + update + reflux of one level; dt is read from d_dt.
@@ -402,7 +431,7 @@ hipError_t launch_amr_godunov(const AmrLevelArgs& a, double dx, const double* d_
   const bool dev = a.dnoct != nullptr;
   if (!dev && a.noct <= 0) return hipSuccess;
   const int nbatch = (a.noct + a.nv - 1) / a.nv;
-  hipLaunchKernelGGL(amr_gather, dim3(dev ? kPersistBlocks / 2 : (unsigned)nbatch), dim3(27 * a.nv), 0, s, a);
+  hipLaunchKernelGGL(amr_gather, dim3(dev ? persist_blocks() / 2 : (unsigned)nbatch), dim3(27 * a.nv), 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   UnsplitArgs u;
@@ -418,9 +447,9 @@ hipError_t launch_amr_godunov(const AmrLevelArgs& a, double dx, const double* d_
   u.d_noct = a.dnoct;
   u.nv = a.nv;
   u.hp = a.hp;
-  e = launch_unsplit(u, fp32, s, kPersistBlocks);
+  e = launch_unsplit(u, fp32, s, persist_blocks());
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(amr_update, dim3(dev ? kPersistBlocks : (unsigned)nbatch), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(amr_update, dim3(dev ? persist_blocks() : (unsigned)nbatch), dim3(256), 0, s, a);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (a.active_coarse && (a.dnoct_coarse || a.noct_coarse > 0)) {
@@ -451,6 +480,15 @@ hipError_t launch_amr_courant(const int* active, int noct, const AmrTreeView& t,
   if (!dn && noct <= 0) return hipSuccess;
   hipLaunchKernelGGL(amr_courant, dim3(cell_grid(8L * noct, dn)), dim3(256), 0, s, active, noct, dn, t, u, ustride,
                      dx, hp, d_dt);
+  return hipGetLastError();
+}
+
+hipError_t launch_amr_begin(const int* active, int noct, const AmrTreeView& t, const double* u, double* unew,
+                            long ustride, double dx, const HydroParams& hp, double* d_dt, hipStream_t s,
+                            const int* dn) {
+  if (!dn && noct <= 0) return hipSuccess;
+  hipLaunchKernelGGL(amr_begin, dim3(cell_grid(8L * noct, dn)), dim3(256), 0, s, active, noct, dn, t, u, unew,
+                     ustride, dx, hp, d_dt);
   return hipGetLastError();
 }
 

@@ -39,6 +39,9 @@ hipError_t launch_amr_dt(int op, double* dt, double* dtold, double* time, int l,
                          double start, hipStream_t s, const int* dn = nullptr, long long* cnt = nullptr);
 hipError_t launch_set_ints(int* p, int n, int v, hipStream_t s);
 hipError_t launch_clear_bits(int* p, int mask, hipStream_t s);
+hipError_t launch_amr_begin(const int* active, int noct, const AmrTreeView& t, const double* u, double* unew,
+                            long ustride, double dx, const HydroParams& hp, double* d_dt, hipStream_t s,
+                            const int* dn = nullptr);
 hipError_t launch_amr_finish(const int* active, int noct, const AmrTreeView& t, const double* unew, double* u,
                              long ustride, double smallr, int ivar_mode, hipStream_t s, const int* dn = nullptr);
 struct RefineParams {
@@ -66,7 +69,7 @@ hipError_t launch_refine_fine(const int* active, int noct, int ilevel, const Amr
                               int interpol_type, int interpol_var, double smallr, void* work, int* d_error,
                               int* nofree, hipStream_t s);
 Tuning& tuning() {
-  static Tuning t{0, 0, 0};
+  static Tuning t{0, 0, 0, 0};
   return t;
 }
 }  // namespace ramses
@@ -516,9 +519,9 @@ int amr_step_rec(StepCtx& c, int l, int icount) {
   hipError_t e = ramses::launch_amr_dt(0, m->dt, m->dtold, m->time, l, coarser, coarser ? m->nsubcycle[l - 1] : 1,
                                        0, start, c.s);
   if (e != hipSuccess) return hip_fail(e, "dt launch");
-  e = ramses::launch_amr_courant(lv.active, lv.noct, v, m->uold, v.ncell + 1, lv.dx, c.hp, m->dt + l, c.s);
-  if (e != hipSuccess) return hip_fail(e, "courant launch");
-  if ((rc = ramses_amr_copy(&m->tree, &lv, m->uold, m->unew, c.s))) return rc;  // set_unew
+  // courant_fine and set_unew in one pass
+  e = ramses::launch_amr_begin(lv.active, lv.noct, v, m->uold, m->unew, v.ncell + 1, lv.dx, c.hp, m->dt + l, c.s);
+  if (e != hipSuccess) return hip_fail(e, "courant / set_unew launch");
   if (l < m->nlevelmax && m->noct[l + 1] > 0) {
     if ((rc = amr_step_rec(c, l + 1, 1))) return rc;
     if (m->nsubcycle[l] == 2 && (rc = amr_step_rec(c, l + 1, 2))) return rc;
@@ -653,10 +656,8 @@ hipError_t gstep(GCtx& g, int l, int icount) {
   e = ramses::launch_amr_dt(0, m->dt, m->dtold, m->time, l, coarser, coarser ? m->nsubcycle[l - 1] : 1, 0, start, g.s,
                             dn, g.dev->cnt);
   if (e != hipSuccess) return e;
-  e = ramses::launch_amr_courant(m->active[l], 0, g.v, m->uold, us, dx, g.c.hp, m->dt + l, g.s, dn + l);
-  if (e != hipSuccess) return e;
-  e = ramses::launch_amr_copy(m->active[l], 0, g.v, m->uold, m->unew, us, g.s, dn + l);  // set_unew
-  if (e != hipSuccess) return e;
+  e = ramses::launch_amr_begin(m->active[l], 0, g.v, m->uold, m->unew, us, dx, g.c.hp, m->dt + l, g.s, dn + l);
+  if (e != hipSuccess) return e;  // courant_fine + set_unew
   if (l < m->nlevelmax) {
     if ((e = gstep(g, l + 1, 1)) != hipSuccess) return e;
     if (m->nsubcycle[l] == 2 && (e = gstep(g, l + 1, 2)) != hipSuccess) return e;
@@ -892,6 +893,10 @@ int ramses_tune(int knob, int value) {
       if (value != 0 && value < 4) return fail(-2, "level tz must be 0 or >= 4");
       t.level_tz = value;
       return 0;
+    case RAMSES_TUNE_PERSIST_BLOCKS:
+      if (value != 0 && (value < 32 || value > 8192)) return fail(-2, "persist blocks must be 0 or in [32, 8192]");
+      t.persist_blocks = value;
+      return 0;
     default:
       return fail(-1, "unknown tuning knob");
   }
@@ -903,6 +908,7 @@ int ramses_tune_get(int knob) {
     case RAMSES_TUNE_UNSPLIT_BLOCK: return t.unsplit_block;
     case RAMSES_TUNE_LEVEL_TILE: return t.level_tile;
     case RAMSES_TUNE_LEVEL_TZ: return t.level_tz;
+    case RAMSES_TUNE_PERSIST_BLOCKS: return (int)ramses::persist_blocks();
     default: return fail(-1, "unknown tuning knob");
   }
 }

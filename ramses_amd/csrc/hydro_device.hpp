@@ -33,8 +33,11 @@ struct Tuning {
   int unsplit_block;
   int level_tile;
   int level_tz;
+  int persist_blocks;  // workgroups of a captured AMR step's fixed-grid kernels (0: default)
 };
 Tuning& tuning();
+// Workgroups of the fixed grids of a captured AMR coarse step.
+inline unsigned persist_blocks() { return tuning().persist_blocks > 0 ? (unsigned)tuning().persist_blocks : 1024u; }
 
 // Fortran MAX/MIN of two reals.  The hardware min/max (IEEE maxNum/minNum)
 // returns the same value as the reference for every non-NaN pair; the only

@@ -506,11 +506,10 @@ hipError_t scan64(const long long* in, long long* out, long long* sums, long n, 
 
 // The scan with its length in device memory: persistent grids, and the
 // tile totals always scanned so that sums[ntiles] is the grand total.
-constexpr unsigned kPersist = 512;
 hipError_t scan64_dev(const long long* in, long long* out, long long* sums, DevLen len, hipStream_t s) {
-  hipLaunchKernelGGL(scan_tiles, dim3(kPersist / 2), dim3(kScanThreads), 0, s, in, out, sums, len);
+  hipLaunchKernelGGL(scan_tiles, dim3(persist_blocks() / 2), dim3(kScanThreads), 0, s, in, out, sums, len);
   hipLaunchKernelGGL(scan_sums, dim3(1), dim3(1024), 0, s, sums, len);
-  hipLaunchKernelGGL(scan_add, dim3(kPersist), dim3(256), 0, s, out, (const long long*)sums, len);
+  hipLaunchKernelGGL(scan_add, dim3(persist_blocks()), dim3(256), 0, s, out, (const long long*)sums, len);
   return hipGetLastError();
 }
 
@@ -522,7 +521,7 @@ hipError_t launch_flag_fine(const int* active, int noct, const AmrTreeView& t, i
                             int nlevelmax, const double* u, long us, int* flag1, int* flag2, const RefineParams& rp,
                             const HydroParams& hp, hipStream_t s, const int* dn) {
   if (ilevel == nlevelmax || (!dn && noct <= 0)) return hipSuccess;
-  const unsigned nb = dn ? kPersist : nblk(8L * noct, 256);
+  const unsigned nb = dn ? persist_blocks() : nblk(8L * noct, 256);
   hipLaunchKernelGGL(flag_init, dim3(nb), dim3(256), 0, s, active, noct, dn, t, flag1, ilevel < levelmin ? 1 : 0);
   if (ilevel < levelmin) return hipGetLastError();
   // the flags ping-pong between flag1 and flag2, one launch per pass
@@ -545,7 +544,7 @@ hipError_t launch_flag_fine(const int* active, int noct, const AmrTreeView& t, i
   if (cur != flag1)
     hipLaunchKernelGGL(flag_copy, dim3(nb), dim3(256), 0, s, active, noct, dn, t, (const int*)cur, flag1);
   if (ilevel > levelmin && icount < 2)
-    hipLaunchKernelGGL(flag_ensure, dim3(dn ? kPersist : nblk(27L * noct, 256)), dim3(256), 0, s, active, noct, dn,
+    hipLaunchKernelGGL(flag_ensure, dim3(dn ? persist_blocks() : nblk(27L * noct, 256)), dim3(256), 0, s, active, noct, dn,
                        t, ilevel, flag1);
   return hipGetLastError();
 }
@@ -671,7 +670,7 @@ hipError_t launch_refine_fine_dev(const int* active, const int* dn_l, int ilevel
   long long* sums = off + nmax;
   int* newg = (int*)(sums + nt);
   int* tmp = newg + nslot;
-  hipLaunchKernelGGL(refine_mark, dim3(kPersist), dim3(256), 0, s, active, 0, dn_l, t, (const int*)flag1, mark);
+  hipLaunchKernelGGL(refine_mark, dim3(persist_blocks()), dim3(256), 0, s, active, 0, dn_l, t, (const int*)flag1, mark);
   const DevLen slots{0, dn_l, 2};
   hipError_t e = scan64_dev(mark, off, sums, slots, s);
   if (e != hipSuccess) return e;
@@ -704,14 +703,14 @@ hipError_t launch_refine_fine_dev(const int* active, const int* dn_l, int ilevel
   r.dnoct = dn_l;
   r.dfree = dfree;
   r.ctrl = ctrl;
-  hipLaunchKernelGGL(refine_make, dim3(kPersist), dim3(128), 0, s, r);
-  hipLaunchKernelGGL(refine_kill, dim3(kPersist), dim3(256), 0, s, r);
-  hipLaunchKernelGGL(list_alive_d, dim3(kPersist), dim3(256), 0, s, (const int*)active_fine, (const int*)dn_f,
+  hipLaunchKernelGGL(refine_make, dim3(persist_blocks()), dim3(128), 0, s, r);
+  hipLaunchKernelGGL(refine_kill, dim3(persist_blocks()), dim3(256), 0, s, r);
+  hipLaunchKernelGGL(list_alive_d, dim3(persist_blocks()), dim3(256), 0, s, (const int*)active_fine, (const int*)dn_f,
                      (const int*)m.father, mark, (const int*)ctrl);
   if ((e = scan64_dev(mark, off, sums, DevLen{0, dn_f, 1}, s)) != hipSuccess) return e;
-  hipLaunchKernelGGL(list_compact_d, dim3(kPersist), dim3(256), 0, s, (const int*)active_fine, (const int*)dn_f,
+  hipLaunchKernelGGL(list_compact_d, dim3(persist_blocks()), dim3(256), 0, s, (const int*)active_fine, (const int*)dn_f,
                      (const long long*)mark, (const long long*)off, tmp, (const int*)ctrl);
-  hipLaunchKernelGGL(list_assemble_d, dim3(kPersist), dim3(256), 0, s, active_fine, (const int*)tmp,
+  hipLaunchKernelGGL(list_assemble_d, dim3(persist_blocks()), dim3(256), 0, s, active_fine, (const int*)tmp,
                      (const int*)dn_f, (const int*)newg, pos, (const int*)ctrl);
   hipLaunchKernelGGL(refine_commit, dim3(1), dim3(1), 0, s, dn_f, dfree, (const int*)ctrl, cap);
   return hipGetLastError();
