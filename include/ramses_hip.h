@@ -31,6 +31,8 @@ extern "C" {
 #define RAMSES_RIEMANN_LLF 0
 #define RAMSES_RIEMANN_HLLC 1
 #define RAMSES_RIEMANN_HLL 2
+#define RAMSES_RIEMANN_ACOUSTIC 3 /* riemann_acoustic, hydro/godunov_utils.f90:500 */
+#define RAMSES_RIEMANN_EXACT 4    /* riemann_approx ('exact'), hydro/godunov_utils.f90:268 */
 
 #define RAMSES_FP64 0        /* every operation in FP64, bitwise the reference */
 #define RAMSES_FP32_MIXED 1  /* FP32 primitives/slopes/trace/Riemann; FP64 state,
@@ -51,6 +53,7 @@ typedef struct ramses_hydro_params {
   int riemann;           /* RAMSES_RIEMANN_* */
   int precision;         /* RAMSES_FP64 or RAMSES_FP32_MIXED */
   int nvector;           /* leading dimension of the Fortran arrays (32 in the reference build) */
+  int niter_riemann;     /* Newton iterations of the exact solver (reference default 10) */
 } ramses_hydro_params;
 
 /* Validate and install the configuration used by every later call.

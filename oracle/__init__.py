@@ -28,7 +28,7 @@ ORACLE_LIB = os.path.join(_HERE, "libhydro_oracle.so")
 REF_LIB = os.path.join(_HERE, "_ref", "libramses_ref.so")
 
 NVAR, NDIM = 5, 3
-RIEMANN = {"llf": 0, "hllc": 1, "hll": 2}
+RIEMANN = {"llf": 0, "hllc": 1, "hll": 2, "acoustic": 3, "exact": 4}
 
 _dp = ctypes.POINTER(ctypes.c_double)
 
@@ -44,7 +44,7 @@ class _Params(ctypes.Structure):
     _fields_ = [("gamma", ctypes.c_double), ("smallr", ctypes.c_double),
                 ("smallc", ctypes.c_double), ("slope_theta", ctypes.c_double),
                 ("courant_factor", ctypes.c_double), ("difmag", ctypes.c_double),
-                ("slope_type", ctypes.c_int), ("riemann", ctypes.c_int)]
+                ("slope_type", ctypes.c_int), ("riemann", ctypes.c_int), ("niter_riemann", ctypes.c_int)]
 
 
 def stencil_shape(nv):
@@ -91,9 +91,9 @@ class Oracle:
         self.lib = lib
 
     def set_params(self, gamma=1.4, smallr=1e-10, smallc=1e-10, slope_type=1, slope_theta=1.5,
-                   riemann="llf", courant_factor=0.5, difmag=0.0):
+                   riemann="llf", courant_factor=0.5, difmag=0.0, niter_riemann=10):
         p = _Params(gamma, smallr, smallc, slope_theta, courant_factor, difmag, slope_type,
-                    RIEMANN[riemann])
+                    RIEMANN[riemann], niter_riemann)
         rc = self.lib.hr_set_params(ctypes.byref(p))
         if rc != 0:
             raise ValueError(f"oracle rejected the configuration (code {rc})")
@@ -163,9 +163,11 @@ class Reference:
         self._grav0 = np.zeros(grav_shape(self.nvector))
 
     def set_params(self, gamma=1.4, smallr=1e-10, smallc=1e-10, slope_type=1, slope_theta=1.5,
-                   riemann="llf", courant_factor=0.5, difmag=0.0):
+                   riemann="llf", courant_factor=0.5, difmag=0.0, niter_riemann=10):
         self.lib.ref_set_hydro_params(gamma, smallr, smallc, slope_type, slope_theta,
                                       RIEMANN[riemann], courant_factor, difmag)
+        self.lib.ref_set_niter_riemann.argtypes = [ctypes.c_int]
+        self.lib.ref_set_niter_riemann(niter_riemann)
 
     def unsplit(self, uin, gravin, flux, tmp, dx, dt, ngrid, dy=None, dz=None):
         assert uin.shape == stencil_shape(self.nvector)

@@ -23,7 +23,7 @@
 #define NC6 216 /* 6x6x6 stencil cells */
 #define NF3 27  /* 3x3x3 face slots */
 
-static hr_params P = {1.4, 1e-10, 1e-10, 1.5, 0.5, 0.0, 1, HR_RIEMANN_LLF};
+static hr_params P = {1.4, 1e-10, 1e-10, 1.5, 0.5, 0.0, 1, HR_RIEMANN_LLF, 10};
 
 static const double zero = 0.0, one = 1.0, two = 2.0, half = 0.5;
 
@@ -36,7 +36,8 @@ int hr_set_params(const hr_params *p) {
   if (!(p->slope_type == 0 || p->slope_type == 1 || p->slope_type == 2 || p->slope_type == 3 ||
         p->slope_type == 7 || p->slope_type == 8))
     return -2; /* hydro/umuscl.f90:1474-1477 stops on other 3D slope types */
-  if (p->riemann < 0 || p->riemann > 2) return -3;
+  if (p->riemann < 0 || p->riemann > 4) return -3;
+  if (p->niter_riemann < 0) return -6;
   if (p->difmag != 0.0) return -4; /* cmpdivu/consup not restated */
   if (!(p->gamma > 1.0)) return -5;
   P = *p;
@@ -402,10 +403,144 @@ static void riemann_hllc(const double *ql, const double *qr, double *fg) {
   fg[5] = uo * eo;
 }
 
+/* The flux of a sampled state q[0..5] = (rho, u_n, p, u_t1, u_t2, e_int)
+ * shared by riemann_acoustic and riemann_approx
+ * (hydro/godunov_utils.f90:468-492, 631-652). */
+static void sampled_flux(const double *q, double *fg) {
+  const double entho = one / (P.gamma - one);
+  fg[0] = q[0] * q[1];
+  fg[1] = q[2] + q[0] * (q[1] * q[1]);
+  double etot = q[2] * entho + half * q[0] * (q[1] * q[1]);
+  etot = etot + half * q[0] * (q[3] * q[3]);
+  etot = etot + half * q[0] * (q[4] * q[4]);
+  fg[2] = q[1] * (etot + q[2]);
+  for (int n = 3; n < 6; ++n) fg[n] = fg[0] * q[n];
+}
+
+/* riemann_acoustic: hydro/godunov_utils.f90:500-655 */
+static void riemann_acoustic(const double *ql, const double *qr, double *fg) {
+  const double gamma = P.gamma;
+  const double smallp = P.smallc * P.smallc / gamma; /* :528 */
+  const double entho = one / (gamma - one);
+  const double rl = fmaxr(ql[0], P.smallr), ul = ql[1], pl = fmaxr(ql[2], rl * smallp);
+  const double rr = fmaxr(qr[0], P.smallr), ur = qr[1], pr = fmaxr(qr[2], rr * smallp);
+  const double cl = sqrt(gamma * pl / rl), cr = sqrt(gamma * pr / rr); /* :542-553 */
+  const double wl = cl * rl, wr = cr * rr;
+  const double pstar = ((wr * pl + wl * pr) + wl * wr * (ul - ur)) / (wl + wr);
+  const double ustar = ((wr * ur + wl * ul) + (pl - pr)) / (wl + wr);
+  const double sgnm = copysign(one, ustar); /* :557-559 */
+  double ro, uo, po, co;
+  if (sgnm == one) {
+    ro = rl; uo = ul; po = pl; co = cl;
+  } else {
+    ro = rr; uo = ur; po = pr; co = cr;
+  }
+  double rstar = ro + (pstar - po) / (co * co); /* :579-584 */
+  rstar = fmaxr(rstar, P.smallr);
+  double cstar = sqrt(fabs(gamma * pstar / rstar));
+  cstar = fmaxr(cstar, P.smallc);
+  double spout = co - sgnm * uo, spin = cstar - sgnm * ustar; /* :587-590 */
+  double ushock = half * (spin + spout);
+  ushock = fmaxr(ushock, -sgnm * ustar);
+  if (pstar >= po) {
+    spout = ushock;
+    spin = spout;
+  }
+  double q[6];
+  if (spout < zero) { /* :605-621 */
+    q[0] = ro; q[1] = uo; q[2] = po;
+  } else if (spin >= zero) {
+    q[0] = rstar; q[1] = ustar; q[2] = pstar;
+  } else {
+    const double frac = spout / (spout - spin);
+    q[0] = frac * rstar + (one - frac) * ro;
+    q[1] = frac * ustar + (one - frac) * uo;
+    q[2] = frac * pstar + (one - frac) * po;
+  }
+  q[3] = sgnm == one ? ql[3] : qr[3]; /* :624-634 */
+  q[4] = sgnm == one ? ql[4] : qr[4];
+  q[5] = po / ro * entho;
+  sampled_flux(q, fg);
+}
+
+/* riemann_approx ('exact'): hydro/godunov_utils.f90:268-495.  The
+ * reference iterates all faces together and drops a face from the Newton
+ * loop once its relative pressure change is at most 1e-6; per face that is
+ * "update, then stop if converged", up to niter_riemann updates. */
+static void riemann_approx(const double *ql, const double *qr, double *fg) {
+  const double gamma = P.gamma;
+  const double smallp = P.smallc * P.smallc / gamma; /* :295-298 */
+  const double smallpp = P.smallr * smallp;
+  const double gamma6 = (gamma + one) / (two * gamma);
+  const double entho = one / (gamma - one);
+  const double rl = fmaxr(ql[0], P.smallr), ul = ql[1], pl = fmaxr(ql[2], rl * smallp);
+  const double rr = fmaxr(qr[0], P.smallr), ur = qr[1], pr = fmaxr(qr[2], rr * smallp);
+  const double cl = gamma * pl * rl, cr = gamma * pr * rr; /* :311-314 */
+  double wl = sqrt(cl), wr = sqrt(cr);
+  double pstar = ((wr * pl + wl * pr) + wl * wr * (ul - ur)) / (wl + wr);
+  pstar = fmaxr(pstar, 0.0);
+  double pold = pstar;
+  for (int iter = 1; iter <= P.niter_riemann; ++iter) { /* :328-363 */
+    const double wwl = sqrt(cl * (one + gamma6 * (pold - pl) / pl));
+    const double wwr = sqrt(cr * (one + gamma6 * (pold - pr) / pr));
+    const double ql_ = two * (wwl * wwl * wwl) / (wwl * wwl + cl);
+    const double qr_ = two * (wwr * wwr * wwr) / (wwr * wwr + cr);
+    const double usl = ul - (pold - pl) / wwl;
+    const double usr = ur + (pold - pr) / wwr;
+    const double delp = fmaxr(qr_ * ql_ / (qr_ + ql_) * (usl - usr), -pold);
+    pold = pold + delp;
+    const double uo = fabs(delp / (pold + smallpp));
+    if (!(uo > 1e-06)) break;
+  }
+  pstar = pold; /* :367-374 */
+  wl = sqrt(cl * (one + gamma6 * (pstar - pl) / pl));
+  wr = sqrt(cr * (one + gamma6 * (pstar - pr) / pr));
+  const double ustar = half * (ul + (pl - pstar) / wl + ur - (pr - pstar) / wr); /* :378-381 */
+  const double sgnm = copysign(one, ustar);
+  double ro, uo, po, wo;
+  if (sgnm == one) {
+    ro = rl; uo = ul; po = pl; wo = wl;
+  } else {
+    ro = rr; uo = ur; po = pr; wo = wr;
+  }
+  const double co = fmaxr(P.smallc, sqrt(fabs(gamma * po / ro))); /* :403-405 */
+  double rstar;
+  if (pstar >= po)
+    rstar = ro / (one + ro * (po - pstar) / (wo * wo)); /* shock */
+  else
+    rstar = ro * pow(pstar / po, one / gamma); /* rarefaction */
+  rstar = fmaxr(rstar, P.smallr); /* :419-430 */
+  double cstar = sqrt(fabs(gamma * pstar / rstar));
+  cstar = fmaxr(cstar, P.smallc);
+  double spout = co - sgnm * uo, spin = cstar - sgnm * ustar;
+  const double ushock = wo / ro - sgnm * uo;
+  if (pstar >= po) {
+    spout = ushock;
+    spin = spout;
+  }
+  double q[6];
+  if (spout <= zero) { /* :440-455 */
+    q[0] = ro; q[1] = uo; q[2] = po;
+  } else if (spin >= zero) {
+    q[0] = rstar; q[1] = ustar; q[2] = pstar;
+  } else {
+    const double frac = spout / (spout - spin);
+    q[1] = frac * ustar + (one - frac) * uo;
+    q[2] = frac * pstar + (one - frac) * po;
+    q[0] = ro * pow(q[2] / po, one / gamma);
+  }
+  q[3] = sgnm == one ? ql[3] : qr[3]; /* :458-468 */
+  q[4] = sgnm == one ? ql[4] : qr[4];
+  q[5] = po / ro * entho;
+  sampled_flux(q, fg);
+}
+
 static inline void riemann(const double *ql, const double *qr, double *fg) {
   switch (P.riemann) { /* hydro/umuscl.f90:791-804 */
   case HR_RIEMANN_HLLC: riemann_hllc(ql, qr, fg); break;
   case HR_RIEMANN_HLL: riemann_hll(ql, qr, fg); break;
+  case HR_RIEMANN_ACOUSTIC: riemann_acoustic(ql, qr, fg); break;
+  case HR_RIEMANN_EXACT: riemann_approx(ql, qr, fg); break;
   default: riemann_llf(ql, qr, fg); break;
   }
 }

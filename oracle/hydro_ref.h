@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-enum { HR_RIEMANN_LLF = 0, HR_RIEMANN_HLLC = 1, HR_RIEMANN_HLL = 2 };
+enum { HR_RIEMANN_LLF = 0, HR_RIEMANN_HLLC = 1, HR_RIEMANN_HLL = 2, HR_RIEMANN_ACOUSTIC = 3, HR_RIEMANN_EXACT = 4 };
 
 /* hydro/hydro_parameters.f90:74-85 (module globals of the reference) */
 typedef struct {
@@ -32,6 +32,7 @@ typedef struct {
   double difmag;
   int slope_type;
   int riemann;
+  int niter_riemann; /* Newton iterations of the 'exact' solver (default 10) */
 } hr_params;
 
 /* returns 0 on success, <0 if the configuration is not supported */

@@ -31,10 +31,24 @@ subroutine ref_set_hydro_params(gamma_in, smallr_in, smallc_in, slope_type_in, &
      riemann = 'hllc'
   case (2)
      riemann = 'hll'
+  case (3)
+     riemann = 'acoustic'
+  case (4)
+     riemann = 'exact'
   case default
      riemann = 'llf'
   end select
 end subroutine ref_set_hydro_params
+
+! niter_riemann (hydro/hydro_parameters.f90:75): Newton iterations of the
+! 'exact' solver.
+subroutine ref_set_niter_riemann(n) bind(C, name="ref_set_niter_riemann")
+  use iso_c_binding
+  use hydro_parameters
+  implicit none
+  integer(c_int), value :: n
+  niter_riemann = n
+end subroutine ref_set_niter_riemann
 
 ! Shapes of the compiled reference (compile-time constants).
 subroutine ref_shape(nvector_out, nvar_out, ndim_out) bind(C, name="ref_shape")

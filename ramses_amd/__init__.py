@@ -28,7 +28,7 @@ LIB_PATH = os.path.join(HERE, "libramses_hip.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "ramses_hip.h")
 
 NVAR, NDIM = 5, 3
-RIEMANN = {"llf": 0, "hllc": 1, "hll": 2}
+RIEMANN = {"llf": 0, "hllc": 1, "hll": 2, "acoustic": 3, "exact": 4}
 PRECISION = {"fp64": 0, "fp32": 1, "fp32_mixed": 1}
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -38,7 +38,7 @@ class ramses_hydro_params(ctypes.Structure):
     _fields_ = [("gamma", ctypes.c_double), ("smallr", ctypes.c_double), ("smallc", ctypes.c_double),
                 ("slope_theta", ctypes.c_double), ("courant_factor", ctypes.c_double),
                 ("difmag", ctypes.c_double), ("slope_type", ctypes.c_int), ("riemann", ctypes.c_int),
-                ("precision", ctypes.c_int), ("nvector", ctypes.c_int)]
+                ("precision", ctypes.c_int), ("nvector", ctypes.c_int), ("niter_riemann", ctypes.c_int)]
 
 
 class RamsesError(RuntimeError):
@@ -125,16 +125,17 @@ _config_cache = {}
 
 
 def configure(gamma=1.4, smallr=1e-10, smallc=1e-10, slope_type=1, slope_theta=1.5, riemann="llf",
-              courant_factor=0.5, difmag=0.0, precision="fp64", nvector=32):
-    """Install the solver configuration (the reference's hydro module globals)."""
+              courant_factor=0.5, difmag=0.0, precision="fp64", nvector=32, niter_riemann=10):
+    """Install the solver configuration (the reference's hydro module globals,
+    hydro/hydro_parameters.f90:74-85, with the reference's defaults)."""
     lib = load_library()
     p = ramses_hydro_params(gamma, smallr, smallc, slope_theta, courant_factor, difmag, slope_type,
-                            RIEMANN[riemann], PRECISION[precision], nvector)
+                            RIEMANN[riemann], PRECISION[precision], nvector, niter_riemann)
     _check(lib.ramses_hydro_config(ctypes.byref(p)), "ramses_hydro_config")
     _config_cache.clear()
     _config_cache.update(gamma=gamma, smallr=smallr, smallc=smallc, slope_type=slope_type,
                          slope_theta=slope_theta, riemann=riemann, courant_factor=courant_factor,
-                         precision=precision, nvector=nvector)
+                         precision=precision, nvector=nvector, niter_riemann=niter_riemann)
 
 
 def stencil_shape(nv=32):

@@ -82,7 +82,7 @@ namespace {
 thread_local std::string g_err;
 
 struct Config {
-  ramses::HydroParams hp{1.4, 1e-10, 1e-10, 1.5, 0.5, 1, ramses::kLLF};
+  ramses::HydroParams hp{1.4, 1e-10, 1e-10, 1.5, 0.5, 1, ramses::kLLF, 10};
   int precision = RAMSES_FP64;
   int nvector = 32;
   bool set = false;
@@ -145,8 +145,9 @@ int ramses_hydro_config(const ramses_hydro_params* p) {
   const int st = p->slope_type;
   if (!(st == 0 || st == 1 || st == 2 || st == 3 || st == 7 || st == 8))
     return fail(-4, "unsupported slope_type for NDIM=3 (supported: 0,1,2,3,7,8)");
-  if (p->riemann < RAMSES_RIEMANN_LLF || p->riemann > RAMSES_RIEMANN_HLL)
-    return fail(-5, "unsupported riemann solver (llf, hllc, hll)");
+  if (p->riemann < RAMSES_RIEMANN_LLF || p->riemann > RAMSES_RIEMANN_EXACT)
+    return fail(-5, "unsupported riemann solver (llf, hllc, hll, acoustic, exact)");
+  if (p->niter_riemann < 0 || p->niter_riemann > 1000) return fail(-10, "niter_riemann out of [0, 1000]");
   if (p->difmag != 0.0) return fail(-6, "difmag > 0 (artificial diffusion) is not supported");
   if (p->precision != RAMSES_FP64 && p->precision != RAMSES_FP32_MIXED) return fail(-7, "bad precision");
   if (p->nvector <= 0 || p->nvector > 4096) return fail(-8, "nvector out of range");
@@ -159,6 +160,7 @@ int ramses_hydro_config(const ramses_hydro_params* p) {
   c.hp.courant_factor = p->courant_factor;
   c.hp.slope_type = p->slope_type;
   c.hp.riemann = p->riemann;
+  c.hp.niter_riemann = p->niter_riemann;
   c.precision = p->precision;
   c.nvector = p->nvector;
   c.set = true;
@@ -313,7 +315,6 @@ int ramses_amr_godunov(const ramses_amr_tree* t, const ramses_amr_level* lv, con
   if (lv->noct > 0 && (!lv->active || !lv->pos)) return fail(-1, "null active/pos");
   if (lv->ilevel < 1) return fail(-1, "ilevel must be >= 1");
   if (!(lv->dx > 0)) return fail(-1, "dx must be > 0");
-  if (c.hp.slope_type == 3) return fail(-4, "slope_type 3 is not supported on the AMR path");
   if (c.nvector * 27 > 1024) return fail(-8, "nvector too large for the AMR gather (max 37)");
   const size_t need = amr_work(lv->noct, c.nvector);
   if (!work || work_bytes < need) return fail(-1, "work buffer too small (see ramses_amr_work_bytes)");
@@ -748,7 +749,6 @@ constexpr long kMaxGraphLevelSteps = 1023;
 
 int coarse_step_graph(ramses_amr_mesh* m, hipStream_t s) {
   const Config c = current();
-  if (c.hp.slope_type == 3) return fail(-4, "slope_type 3 is not supported on the AMR path");
   if (c.nvector * 27 > 1024) return fail(-8, "nvector too large for the AMR gather (max 37)");
   if (graph_level_steps(m) > kMaxGraphLevelSteps)
     return fail(-1, "captured mode: too many level steps per coarse step (reduce nlevelmax-levelmin)");

@@ -15,7 +15,10 @@
 
 namespace ramses {
 
-enum Riemann : int { kLLF = 0, kHLLC = 1, kHLL = 2 };
+enum Riemann : int { kLLF = 0, kHLLC = 1, kHLL = 2, kACOUSTIC = 3, kEXACT = 4 };
+// Template value of kernels that pick the solver at run time (the
+// acoustic and exact solvers share one instantiation per kernel).
+constexpr int kDynamicRiemann = 99;
 
 // hydro/hydro_parameters.f90:74-85 resolved once on the host.
 struct HydroParams {
@@ -26,6 +29,7 @@ struct HydroParams {
   double courant_factor;
   int slope_type;  // 0,1,2,3,7,8
   int riemann;     // Riemann enum
+  int niter_riemann;  // Newton iterations of the exact solver
 };
 
 // Launch-shape knobs set through ramses_tune (0 = built-in default).
@@ -58,6 +62,8 @@ __device__ __forceinline__ double div_w(double a, double b) { return a / b; }
 __device__ __forceinline__ float div_w(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ double sqrt_w(double a) { return __builtin_sqrt(a); }
 __device__ __forceinline__ float sqrt_w(float a) { return __builtin_amdgcn_sqrtf(a); }
+__device__ __forceinline__ double pow_w(double a, double b) { return pow(a, b); }
+__device__ __forceinline__ float pow_w(float a, float b) { return powf(a, b); }
 __device__ __forceinline__ double abs_w(double a) { return __builtin_fabs(a); }
 __device__ __forceinline__ float abs_w(float a) { return __builtin_fabsf(a); }
 __device__ __forceinline__ double copysign_w(double a, double b) { return __builtin_copysign(a, b); }
@@ -391,6 +397,130 @@ __device__ __forceinline__ Flux6<R> riemann_hllc(const Face<R>& L, const Face<R>
   return g;
 }
 
+// Flux of a sampled state q = (rho, u_n, p, u_t1, u_t2, e_int), shared by
+// the acoustic and exact solvers (hydro/godunov_utils.f90:468-492, 631-652).
+template <typename R>
+__device__ __forceinline__ Flux6<R> sampled_flux(const R q[6], R entho) {
+  const R half = (R)0.5;
+  Flux6<R> g;
+  g.f[0] = q[0] * q[1];
+  g.f[1] = q[2] + q[0] * (q[1] * q[1]);
+  R etot = q[2] * entho + half * q[0] * (q[1] * q[1]);
+  etot = etot + half * q[0] * (q[3] * q[3]);
+  etot = etot + half * q[0] * (q[4] * q[4]);
+  g.f[2] = q[1] * (etot + q[2]);
+#pragma unroll
+  for (int n = 3; n < 6; ++n) g.f[n] = g.f[0] * q[n];
+  return g;
+}
+
+// riemann_acoustic: hydro/godunov_utils.f90:500-655 (linearised star state,
+// rarefaction fan interpolated linearly).
+template <typename R>
+__device__ __forceinline__ Flux6<R> riemann_acoustic(const Face<R>& L, const Face<R>& Rt, R gamma, R smallr,
+                                                     R smallp, R entho, R smallc) {
+  const R half = (R)0.5, zero = (R)0, one = (R)1;
+  const R rl = fmax_f(L.r, smallr), ul = L.un, pl = fmax_f(L.p, rl * smallp);
+  const R rr = fmax_f(Rt.r, smallr), ur = Rt.un, pr = fmax_f(Rt.p, rr * smallp);
+  const R cl = sqrt_w(div_w(gamma * pl, rl)), cr = sqrt_w(div_w(gamma * pr, rr));
+  const R wl = cl * rl, wr = cr * rr;
+  const R pstar = div_w((wr * pl + wl * pr) + wl * wr * (ul - ur), wl + wr);
+  const R ustar = div_w((wr * ur + wl * ul) + (pl - pr), wl + wr);
+  const R sgnm = copysign_w(one, ustar);
+  const bool left = sgnm == one;
+  const R ro = left ? rl : rr, uo = left ? ul : ur, po = left ? pl : pr, co = left ? cl : cr;
+  R rstar = ro + div_w(pstar - po, co * co);
+  rstar = fmax_f(rstar, smallr);
+  R cstar = sqrt_w(abs_w(div_w(gamma * pstar, rstar)));
+  cstar = fmax_f(cstar, smallc);
+  R spout = co - sgnm * uo, spin = cstar - sgnm * ustar;
+  R ushock = half * (spin + spout);
+  ushock = fmax_f(ushock, -sgnm * ustar);
+  if (pstar >= po) {
+    spout = ushock;
+    spin = spout;
+  }
+  R q[6];
+  if (spout < zero) {
+    q[0] = ro; q[1] = uo; q[2] = po;
+  } else if (spin >= zero) {
+    q[0] = rstar; q[1] = ustar; q[2] = pstar;
+  } else {
+    const R frac = div_w(spout, spout - spin);
+    q[0] = frac * rstar + (one - frac) * ro;
+    q[1] = frac * ustar + (one - frac) * uo;
+    q[2] = frac * pstar + (one - frac) * po;
+  }
+  q[3] = left ? L.ut1 : Rt.ut1;
+  q[4] = left ? L.ut2 : Rt.ut2;
+  q[5] = div_w(po, ro) * entho;
+  return sampled_flux(q, entho);
+}
+
+// riemann_approx ('exact'): hydro/godunov_utils.f90:268-495.  Newton
+// iterations on the two-shock star pressure; the reference drops a face from
+// the iteration once |delta p / (p + smallr*smallp)| <= 1e-6, i.e. per face:
+// update, then stop if converged, at most niter updates.  The rarefaction
+// branches use pow (libm's on the reference side, the device library's
+// here: those branches can differ in the last bit).
+template <typename R>
+__device__ __forceinline__ Flux6<R> riemann_exact(const Face<R>& L, const Face<R>& Rt, R gamma, R smallr, R smallp,
+                                                  R entho, R smallc, int niter) {
+  const R half = (R)0.5, zero = (R)0, one = (R)1, two = (R)2;
+  const R smallpp = smallr * smallp;
+  const R gamma6 = div_w(gamma + one, two * gamma);
+  const R rl = fmax_f(L.r, smallr), ul = L.un, pl = fmax_f(L.p, rl * smallp);
+  const R rr = fmax_f(Rt.r, smallr), ur = Rt.un, pr = fmax_f(Rt.p, rr * smallp);
+  const R cl = gamma * pl * rl, cr = gamma * pr * rr;
+  R wl = sqrt_w(cl), wr = sqrt_w(cr);
+  R pold = fmax_f(div_w((wr * pl + wl * pr) + wl * wr * (ul - ur), wl + wr), zero);
+  for (int iter = 1; iter <= niter; ++iter) {
+    const R wwl = sqrt_w(cl * (one + div_w(gamma6 * (pold - pl), pl)));
+    const R wwr = sqrt_w(cr * (one + div_w(gamma6 * (pold - pr), pr)));
+    const R ql = div_w(two * (wwl * wwl * wwl), wwl * wwl + cl);
+    const R qr = div_w(two * (wwr * wwr * wwr), wwr * wwr + cr);
+    const R usl = ul - div_w(pold - pl, wwl);
+    const R usr = ur + div_w(pold - pr, wwr);
+    const R delp = fmax_f(div_w(qr * ql, qr + ql) * (usl - usr), -pold);
+    pold = pold + delp;
+    if (!(abs_w(div_w(delp, pold + smallpp)) > (R)1e-06)) break;
+  }
+  const R pstar = pold;
+  wl = sqrt_w(cl * (one + div_w(gamma6 * (pstar - pl), pl)));
+  wr = sqrt_w(cr * (one + div_w(gamma6 * (pstar - pr), pr)));
+  const R ustar = half * (ul + div_w(pl - pstar, wl) + ur - div_w(pr - pstar, wr));
+  const R sgnm = copysign_w(one, ustar);
+  const bool left = sgnm == one;
+  const R ro = left ? rl : rr, uo = left ? ul : ur, po = left ? pl : pr, wo = left ? wl : wr;
+  const R co = fmax_f(smallc, sqrt_w(abs_w(div_w(gamma * po, ro))));
+  const R inv_gamma = div_w(one, gamma);
+  R rstar = pstar >= po ? div_w(ro, one + div_w(ro * (po - pstar), wo * wo)) : ro * pow_w(div_w(pstar, po), inv_gamma);
+  rstar = fmax_f(rstar, smallr);
+  R cstar = sqrt_w(abs_w(div_w(gamma * pstar, rstar)));
+  cstar = fmax_f(cstar, smallc);
+  R spout = co - sgnm * uo, spin = cstar - sgnm * ustar;
+  const R ushock = div_w(wo, ro) - sgnm * uo;
+  if (pstar >= po) {
+    spout = ushock;
+    spin = spout;
+  }
+  R q[6];
+  if (spout <= zero) {
+    q[0] = ro; q[1] = uo; q[2] = po;
+  } else if (spin >= zero) {
+    q[0] = rstar; q[1] = ustar; q[2] = pstar;
+  } else {
+    const R frac = div_w(spout, spout - spin);
+    q[1] = frac * ustar + (one - frac) * uo;
+    q[2] = frac * pstar + (one - frac) * po;
+    q[0] = ro * pow_w(div_w(q[2], po), inv_gamma);
+  }
+  q[3] = left ? L.ut1 : Rt.ut1;
+  q[4] = left ? L.ut2 : Rt.ut2;
+  q[5] = div_w(po, ro) * entho;
+  return sampled_flux(q, entho);
+}
+
 // Scaling of a face flux by dt/dx as the reference writes it, (f*dt)/dx
 // (hydro/umuscl.f90:106,129,153).  When dx is a power of two the division is
 // the exact multiplication by 1/dx (inv_dx != 0), which the FP64 path uses
@@ -424,8 +554,8 @@ __device__ __forceinline__ double scale_flux(R f, const FluxScale& s) {
 // Solver constants in working precision.
 template <typename R>
 struct SolverConsts {
-  R gamma, smallr, smallp, entho, smallc2, theta;
-  int riemann, slope_type;
+  R gamma, smallr, smallp, entho, smallc2, theta, smallc;
+  int riemann, slope_type, niter;
 };
 
 template <typename R>
@@ -439,6 +569,8 @@ __host__ __device__ inline SolverConsts<R> make_consts(const HydroParams& hp) {
   c.theta = (R)hp.slope_theta;
   c.riemann = hp.riemann;
   c.slope_type = hp.slope_type;
+  c.smallc = (R)hp.smallc;
+  c.niter = hp.niter_riemann;
   return c;
 }
 
@@ -446,6 +578,8 @@ template <typename R>
 __device__ __forceinline__ Flux6<R> riemann_solve(const Face<R>& L, const Face<R>& Rt, const SolverConsts<R>& c) {
   if (c.riemann == kHLLC) return riemann_hllc(L, Rt, c.gamma, c.smallr, c.smallp, c.entho, c.smallc2);
   if (c.riemann == kHLL) return riemann_hll(L, Rt, c.gamma, c.smallr, c.smallp, c.entho);
+  if (c.riemann == kACOUSTIC) return riemann_acoustic(L, Rt, c.gamma, c.smallr, c.smallp, c.entho, c.smallc);
+  if (c.riemann == kEXACT) return riemann_exact(L, Rt, c.gamma, c.smallr, c.smallp, c.entho, c.smallc, c.niter);
   return riemann_llf(L, Rt, c.gamma, c.smallr, c.smallp, c.entho);
 }
 
@@ -453,6 +587,7 @@ template <typename R, int RS>
 __device__ __forceinline__ Flux6<R> riemann_fixed(const Face<R>& L, const Face<R>& Rt, const SolverConsts<R>& c) {
   if constexpr (RS == kHLLC) return riemann_hllc(L, Rt, c.gamma, c.smallr, c.smallp, c.entho, c.smallc2);
   else if constexpr (RS == kHLL) return riemann_hll(L, Rt, c.gamma, c.smallr, c.smallp, c.entho);
+  else if constexpr (RS == kDynamicRiemann) return riemann_solve(L, Rt, c);
   else return riemann_llf(L, Rt, c.gamma, c.smallr, c.smallp, c.entho);
 }
 

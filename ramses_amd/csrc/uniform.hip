@@ -376,6 +376,10 @@ hipError_t launch_tile(const UniformArgs& a, int TZ, hipStream_t s) {
   switch (a.hp.riemann) {
     case kHLLC: hipLaunchKernelGGL((uniform_godunov_kernel<R, TX, TY, kHLLC>), grid, dim3(NT), 0, s, a, TZ); break;
     case kHLL: hipLaunchKernelGGL((uniform_godunov_kernel<R, TX, TY, kHLL>), grid, dim3(NT), 0, s, a, TZ); break;
+    case kACOUSTIC:
+    case kEXACT:
+      hipLaunchKernelGGL((uniform_godunov_kernel<R, TX, TY, kDynamicRiemann>), grid, dim3(NT), 0, s, a, TZ);
+      break;
     default: hipLaunchKernelGGL((uniform_godunov_kernel<R, TX, TY, kLLF>), grid, dim3(NT), 0, s, a, TZ); break;
   }
   return hipGetLastError();
@@ -415,10 +419,12 @@ hipError_t launch_level_batch(double* const* buf0, double* const* buf1, double* 
   if (fp32) {
     if (hp.riemann == kHLLC) launch_batch_t<float, kHLLC>(a, blocks, s);
     else if (hp.riemann == kHLL) launch_batch_t<float, kHLL>(a, blocks, s);
+    else if (hp.riemann == kACOUSTIC || hp.riemann == kEXACT) launch_batch_t<float, kDynamicRiemann>(a, blocks, s);
     else launch_batch_t<float, kLLF>(a, blocks, s);
   } else {
     if (hp.riemann == kHLLC) launch_batch_t<double, kHLLC>(a, blocks, s);
     else if (hp.riemann == kHLL) launch_batch_t<double, kHLL>(a, blocks, s);
+    else if (hp.riemann == kACOUSTIC || hp.riemann == kEXACT) launch_batch_t<double, kDynamicRiemann>(a, blocks, s);
     else launch_batch_t<double, kLLF>(a, blocks, s);
   }
   return hipGetLastError();

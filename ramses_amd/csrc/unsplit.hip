@@ -253,6 +253,11 @@ static void launch_r(const UnsplitArgs& a, long grid, int block, hipStream_t str
   switch (a.hp.riemann) {
     case kHLLC: s27 ? launch_t<R, kHLLC, true>(a, grid, block, stream) : launch_t<R, kHLLC, false>(a, grid, block, stream); break;
     case kHLL: s27 ? launch_t<R, kHLL, true>(a, grid, block, stream) : launch_t<R, kHLL, false>(a, grid, block, stream); break;
+    case kACOUSTIC:
+    case kEXACT:
+      s27 ? launch_t<R, kDynamicRiemann, true>(a, grid, block, stream)
+          : launch_t<R, kDynamicRiemann, false>(a, grid, block, stream);
+      break;
     default: s27 ? launch_t<R, kLLF, true>(a, grid, block, stream) : launch_t<R, kLLF, false>(a, grid, block, stream); break;
   }
 }

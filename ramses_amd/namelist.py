@@ -152,6 +152,7 @@ class Params:
     slope_type: int = 1
     slope_theta: float = 1.5
     riemann: str = "llf"
+    niter_riemann: int = 10
     interpol_type: int = 1
     interpol_var: int = 0
     err_grad_d: float = -1.0
@@ -198,6 +199,9 @@ class Params:
         p.slope_type = int(hyd.get("slope_type", p.slope_type))
         p.slope_theta = float(hyd.get("slope_theta", p.slope_theta))
         p.riemann = str(hyd.get("riemann", p.riemann)).strip().lower()
+        if p.riemann not in ("llf", "hllc", "hll", "acoustic", "exact"):
+            raise ValueError(f"unknown riemann solver {p.riemann!r}")
+        p.niter_riemann = int(hyd.get("niter_riemann", p.niter_riemann))
         for k in ("interpol_type", "interpol_var"):
             setattr(p, k, int(ref.get(k, getattr(p, k))))
         for k in ("err_grad_d", "err_grad_u", "err_grad_p", "floor_d", "floor_u", "floor_p"):

@@ -19,7 +19,8 @@ def _configure(p, precision):
     import ramses_amd as ra
     from .amr import configure_interpolation, configure_refinement
     ra.configure(gamma=p.gamma, smallr=p.smallr, smallc=p.smallc, slope_type=p.slope_type,
-                 slope_theta=p.slope_theta, riemann=p.riemann, courant_factor=p.courant_factor, precision=precision)
+                 slope_theta=p.slope_theta, riemann=p.riemann, courant_factor=p.courant_factor, precision=precision,
+                 niter_riemann=p.niter_riemann)
     configure_interpolation(p.interpol_type, p.interpol_var)
     configure_refinement(err_grad_d=p.err_grad_d, err_grad_u=p.err_grad_u, err_grad_p=p.err_grad_p,
                          floor_d=p.floor_d, floor_u=p.floor_u, floor_p=p.floor_p, nexpand=p.nexpand)

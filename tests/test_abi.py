@@ -27,10 +27,11 @@ def test_header_cites_reference():
 
 
 @pytest.mark.parametrize("bad", [dict(slope_type=4), dict(slope_type=5), dict(riemann_code=7), dict(difmag=0.1),
-                                 dict(gamma=1.0), dict(nvector=0), dict(precision_code=9)])
+                                 dict(gamma=1.0), dict(nvector=0), dict(precision_code=9),
+                                 dict(niter_riemann=-1)])
 def test_config_rejects(bad):
     lib = ramses_amd.load_library()
-    p = ramses_amd.ramses_hydro_params(1.4, 1e-10, 1e-10, 1.5, 0.5, 0.0, 1, 0, 0, 32)
+    p = ramses_amd.ramses_hydro_params(1.4, 1e-10, 1e-10, 1.5, 0.5, 0.0, 1, 0, 0, 32, 10)
     for k, v in bad.items():
         setattr(p, {"riemann_code": "riemann", "precision_code": "precision"}.get(k, k), v)
     assert lib.ramses_hydro_config(ctypes.byref(p)) < 0
@@ -39,6 +40,8 @@ def test_config_rejects(bad):
 
 def test_config_accepts_and_python_mirror():
     ramses_amd.configure(gamma=5 / 3, riemann="hllc", slope_type=1, precision="fp32")
+    ramses_amd.configure(riemann="acoustic")
+    ramses_amd.configure(riemann="exact", niter_riemann=3)
     ramses_amd.configure()  # defaults of hydro/hydro_parameters.f90:74-85
     with pytest.raises(ramses_amd.RamsesError):
         ramses_amd.configure(slope_type=6)

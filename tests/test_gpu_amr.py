@@ -57,7 +57,8 @@ def test_amr_sweeps_golden(gpu):
 
 
 @needs_ref
-@pytest.mark.parametrize("riemann,slope", [("hllc", 1), ("llf", 2), ("hll", 8)])
+@pytest.mark.parametrize("riemann,slope", [("hllc", 1), ("llf", 2), ("hll", 8), ("hllc", 3), ("llf", 7),
+                                           ("acoustic", 1), ("exact", 2)])
 def test_amr_sweeps_vs_reference(gpu, riemann, slope):
     import ramses_amd as ra
     from ramses_amd.amr import configure_interpolation
@@ -80,7 +81,10 @@ def test_amr_sweeps_vs_reference(gpu, riemann, slope):
         A.godunov(l, dt)
         ref = A.get(1)
         out = _gpu_sweep(tree, u, l, dt)
-        assert np.array_equal(out, ref), (riemann, slope, l)
+        if riemann == "exact":  # pow in the rarefaction branches: last-bit differences allowed
+            np.testing.assert_allclose(out, ref, rtol=1e-12, atol=1e-300)
+        else:
+            assert np.array_equal(out, ref), (riemann, slope, l)
     ra.configure()
 
 

@@ -66,7 +66,7 @@ def test_oracle_sedov16_golden(oracle_lib, golden_sedov16):
 
 
 @pytest.mark.skipif(not oracle.reference_available(), reason="reference build (oracle/_ref) not present")
-@pytest.mark.parametrize("riemann", ["llf", "hllc", "hll"])
+@pytest.mark.parametrize("riemann", ["llf", "hllc", "hll", "acoustic", "exact"])
 @pytest.mark.parametrize("slope", [0, 1, 2, 3, 7, 8])
 def test_oracle_vs_compiled_reference(oracle_lib, riemann, slope):
     R = oracle.Reference()
@@ -94,7 +94,7 @@ def euler_flux_x(rho, u, v, w, p, gamma):
     return np.array([rho * u, rho * u * u + p, rho * u * v, rho * u * w, (E + p) * u])
 
 
-@pytest.mark.parametrize("riemann", ["llf", "hllc", "hll"])
+@pytest.mark.parametrize("riemann", ["llf", "hllc", "hll", "acoustic", "exact"])
 def test_uniform_state_gives_physical_flux(oracle_lib, riemann):
     """Equal states on both sides of every face: the Godunov flux is the
     physical Euler flux F(U), and the divergence of the update vanishes."""
