@@ -1,5 +1,5 @@
 // One level of an oct-tree (AMR) godunov sweep on the GPU:
-// godfine1 (hydro/godunov_fine.f90:486-911) split into four launches over
+// godfine1 (hydro/godunov_fine.f90:486-911) split into three launches over
 // the level's active octs, all reading the tree (son / father / nbor) and
 // the state (uold / unew) resident in HBM in the reference's memory model:
 //   amr_gather   6x6x6 stencil of every oct in nvector-strided batches
@@ -7,8 +7,9 @@
 //                missing neighbour oct is a buffer oct interpolated from its
 //                father's 7-point neighbourhood: getnborfather :404-525 and
 //                interpol_hydro, hydro/interpol_hydro.f90:268-637);
-//   unsplit      the batched kernel of unsplit.hip on those batches;
-//   amr_update   flux reset at refined faces and the level update (:720-792);
+//   amr_solve    per batch, unsplit (the device code of unsplit.hip), then
+//                the flux reset at refined faces and the level update
+//                (:720-792);
 //   amr_reflux   the conservative correction of coarser leaf neighbours
 //                (:798-908), gathered per coarse cell and summed in the
 //                reference's order (batch, dimension, side, face) so the
@@ -19,6 +20,7 @@
 #include "unsplit_args.hpp"
 #include "amr_args.hpp"
 #include "amr_device.hpp"
+#include "unsplit_device.hpp"
 
 namespace ramses {
 
@@ -140,15 +142,53 @@ __device__ __forceinline__ void update_batch(const AmrLevelArgs& a, int noct, in
   }
 }
 
-// Flux reset at faces touching a refined cell, then the level update
-// unew += flux(i3) - flux(i3+1) per dimension.  Workgroup = one nvector
-// batch at a time: first one thread per (face, lane) resets fluxes, then one
-// thread per (cell, variable, lane) accumulates the three dimensions in
-// order.
-__global__ __launch_bounds__(256) void amr_update(AmrLevelArgs a) {
+// unsplit, flux reset at faces touching a refined cell, and the level
+// update unew += flux(i3) - flux(i3+1) per dimension, one nvector batch per
+// workgroup at a time: first one thread per (row, lane) computes the batch's
+// 36 faces per oct (unsplit_task, as the batched unsplit kernel does), then
+// one thread per (face, lane) resets fluxes and one thread per (cell,
+// variable, lane) accumulates the three dimensions in order.  The batch's
+// fluxes stay in its workgroup's L2 between the phases.
+// Workgroup of 384 threads (12 rows x 32 lanes; larger nvector loops), at
+// least three waves per SIMD like the batched unsplit kernel.
+template <typename R, int RS, bool S27>
+__global__ __launch_bounds__(384, 3) void amr_solve(AmrLevelArgs a, UnsplitArgs u) {
   const int noct = level_count(a.noct, a.dnoct);
-  const int nb = (noct + a.nv - 1) / a.nv;
-  for (int b = blockIdx.x; b < nb; b += gridDim.x) update_batch(a, noct, b);
+  const int nv = a.nv;
+  const int nb = (noct + nv - 1) / nv;
+  const int ngrid_last = noct - (nb - 1) * nv;
+  for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+    for (int t = threadIdx.x; t < 12 * nv; t += blockDim.x)
+      unsplit_dev::unsplit_task<R, RS, S27>(u, (long)b * 12 * nv + t, nb, ngrid_last);
+    __syncthreads();
+    update_batch(a, noct, b);
+  }
+}
+
+template <typename R, int RS, bool S27>
+void launch_solve_t(const AmrLevelArgs& a, const UnsplitArgs& u, unsigned grid, unsigned block, hipStream_t s) {
+  hipLaunchKernelGGL((amr_solve<R, RS, S27>), dim3(grid), dim3(block), 0, s, a, u);
+}
+
+template <typename R>
+void launch_solve_r(const AmrLevelArgs& a, const UnsplitArgs& u, unsigned grid, unsigned block, hipStream_t s) {
+  const bool s27 = a.hp.slope_type == 3;
+  switch (a.hp.riemann) {
+    case kHLLC:
+      s27 ? launch_solve_t<R, kHLLC, true>(a, u, grid, block, s) : launch_solve_t<R, kHLLC, false>(a, u, grid, block, s);
+      break;
+    case kHLL:
+      s27 ? launch_solve_t<R, kHLL, true>(a, u, grid, block, s) : launch_solve_t<R, kHLL, false>(a, u, grid, block, s);
+      break;
+    case kACOUSTIC:
+    case kEXACT:
+      s27 ? launch_solve_t<R, kDynamicRiemann, true>(a, u, grid, block, s)
+          : launch_solve_t<R, kDynamicRiemann, false>(a, u, grid, block, s);
+      break;
+    default:
+      s27 ? launch_solve_t<R, kLLF, true>(a, u, grid, block, s) : launch_solve_t<R, kLLF, false>(a, u, grid, block, s);
+      break;
+  }
 }
 
 // Conservative correction of the coarser leaf cells next to the level's
@@ -415,8 +455,6 @@ __global__ void set_ints(int* p, int n, int v) {
 
 }  // namespace
 
-hipError_t launch_unsplit(const UnsplitArgs& a, bool fp32, hipStream_t stream, long max_grid = 0);
-
 // Grid of a per-cell launch over n items: sized to n for a host count, a
 // fixed persistent grid for a count read from device memory.
 inline unsigned cell_grid(long n, const int* dn) {
@@ -444,12 +482,15 @@ hipError_t launch_amr_godunov(const AmrLevelArgs& a, double dx, const double* d_
   u.d_dt = d_dt;
   u.ngroups = nbatch;
   u.ngrid_last = a.noct - (nbatch - 1) * a.nv;
-  u.d_noct = a.dnoct;
+  u.d_noct = nullptr;  // amr_solve passes the batch count itself
   u.nv = a.nv;
   u.hp = a.hp;
-  e = launch_unsplit(u, fp32, s, persist_blocks());
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(amr_update, dim3(dev ? persist_blocks() : (unsigned)nbatch), dim3(256), 0, s, a);
+  const unsigned block = 384;
+  const unsigned grid = dev ? persist_blocks() : (unsigned)nbatch;
+  if (fp32)
+    launch_solve_r<float>(a, u, grid, block, s);
+  else
+    launch_solve_r<double>(a, u, grid, block, s);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (a.active_coarse && (a.dnoct_coarse || a.noct_coarse > 0)) {

@@ -595,8 +595,8 @@ struct GraphState {
   }
 };
 
-std::vector<long> graph_key(const ramses_amr_mesh* m, int capacity) {
-  std::vector<long> k;
+void graph_key(const ramses_amr_mesh* m, int capacity, std::vector<long>& k) {
+  k.clear();
   auto P = [&](const void* q) { k.push_back((long)(uintptr_t)q); };
   P(m->tree.son);
   P(m->tree.father);
@@ -615,7 +615,6 @@ std::vector<long> graph_key(const ramses_amr_mesh* m, int capacity) {
   for (long v : {(long)m->levelmin, (long)m->nlevelmax, (long)m->adaptive, bl, (long)capacity,
                  (long)g_config_gen.load()})
     k.push_back(v);
-  return k;
 }
 
 struct GCtx {
@@ -782,7 +781,8 @@ int coarse_step_graph(ramses_amr_mesh* m, hipStream_t s) {
     gs->capacity = capacity;
     gs->drop_graph();
   }
-  std::vector<long> key = graph_key(m, capacity);
+  std::vector<long> key;
+  graph_key(m, capacity, key);
   if (!gs->exec || key != gs->key) {
     gs->drop_graph();
     GCtx g{m, gs->cap, c, gs->dev, gs->work, capacity, v, ramses::TreeMut{m->tree.son, m->tree.father, m->tree.nbor}};
