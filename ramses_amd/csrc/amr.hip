@@ -482,7 +482,6 @@ hipError_t launch_amr_godunov(const AmrLevelArgs& a, double dx, const double* d_
   u.d_dt = d_dt;
   u.ngroups = nbatch;
   u.ngrid_last = a.noct - (nbatch - 1) * a.nv;
-  u.d_noct = nullptr;  // amr_solve passes the batch count itself
   u.nv = a.nv;
   u.hp = a.hp;
   const unsigned block = 384;

@@ -17,7 +17,7 @@
 
 namespace ramses {
 struct UnsplitArgs;
-hipError_t launch_unsplit(const UnsplitArgs& a, bool fp32, hipStream_t stream, long max_grid = 0);
+hipError_t launch_unsplit(const UnsplitArgs& a, bool fp32, hipStream_t stream);
 hipError_t launch_cmpdt(const double* uu, const double* gg, int nv, int ncell, double dx,
                         const HydroParams& hp, double* d_dt, hipStream_t stream);
 hipError_t launch_set_double(double* p, double v, hipStream_t s);
@@ -193,7 +193,6 @@ int ramses_unsplit_device(const double* uin, const double* gravin, double* flux,
   a.d_dt = nullptr;
   a.ngroups = ngroups;
   a.ngrid_last = ngrid_last;
-  a.d_noct = nullptr;
   a.nv = c.nvector;
   a.hp = c.hp;
   hipError_t e = ramses::launch_unsplit(a, c.precision == RAMSES_FP32_MIXED, (hipStream_t)stream);

@@ -29,21 +29,12 @@ namespace {
 
 using unsplit_dev::unsplit_task;
 
-// Grid-stride over the (group, row, lane) tasks; with a.d_noct the group
-// count follows from the oct count in device memory at kernel time.
+// One thread per (group, row, lane) task.
 template <typename R, int RS, bool S27>
 __global__ __launch_bounds__(768) void unsplit_rows(UnsplitArgs a) {
-  long ngroups = a.ngroups;
-  int ngrid_last = a.ngrid_last;
-  if (a.d_noct) {
-    const int n = *a.d_noct;
-    ngroups = (n + a.nv - 1) / a.nv;
-    ngrid_last = n - (int)(ngroups - 1) * a.nv;
-  }
-  const long total = ngroups * 12L * a.nv;
-  const long stride = (long)gridDim.x * blockDim.x;
-  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += stride)
-    unsplit_task<R, RS, S27>(a, t, ngroups, ngrid_last);
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.ngroups * 12L * a.nv) return;
+  unsplit_task<R, RS, S27>(a, t, a.ngroups, a.ngrid_last);
 }
 
 }  // namespace
@@ -68,16 +59,15 @@ static void launch_r(const UnsplitArgs& a, long grid, int block, hipStream_t str
   }
 }
 
-hipError_t launch_unsplit(const UnsplitArgs& a, bool fp32, hipStream_t stream, long max_grid) {
+hipError_t launch_unsplit(const UnsplitArgs& a, bool fp32, hipStream_t stream) {
   const long threads = a.ngroups * 12L * a.nv;
-  if (threads <= 0 && !a.d_noct) return hipSuccess;
+  if (threads <= 0) return hipSuccess;
   // Any multiple of 64 works with the flat (batch, row, lane) mapping.  384 =
   // 12 rows x 32 lanes puts exactly one nvector=32 batch in a workgroup, so
   // re-reads of its stencil cells stay in one XCD's L2; 256 spreads a batch
   // over two workgroups.  The autotuner picks per device.
   const int block = tuning().unsplit_block ? tuning().unsplit_block : 256;
-  // with a device oct count the grid is a fixed, persistent size
-  const long grid = a.d_noct ? max_grid : (threads + block - 1) / block;
+  const long grid = (threads + block - 1) / block;
   if (fp32)
     launch_r<float>(a, grid, block, stream);
   else

@@ -16,7 +16,6 @@ struct UnsplitArgs {
   const double* d_dt;  // if not null, dt is read from device memory
   long ngroups;
   int ngrid_last;
-  const int* d_noct;   // if set: ngroups / ngrid_last follow from the oct count here
   int nv;
   HydroParams hp;
 };
