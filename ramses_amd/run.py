@@ -30,9 +30,10 @@ def _ngridmax(p):
     return max(p.ngridmax, sum(8 ** k for k in range(p.levelmin)) + 1)
 
 
-def build(params, precision="fp64", device="cuda"):
+def build(params, precision="fp64", device="cuda", graph=False):
     """An AmrSolver with the mesh and state of the parameter file's
-    initial conditions (init_refine + init_refine_2)."""
+    initial conditions (init_refine + init_refine_2); graph: run its coarse
+    steps as the captured hipGraph."""
     from .amr import AmrSolver
     from .ics import region_condinit
     from .tree import OctTree
@@ -41,7 +42,7 @@ def build(params, precision="fp64", device="cuda"):
     base = sum(8 ** k for k in range(p.levelmin))
     tree = OctTree(p.levelmin, p.levelmin, extra_grids=_ngridmax(p) - base)
     s = AmrSolver(tree, None, refine=p.levelmax > p.levelmin, nlevelmax=p.levelmax, nsubcycle=p.nsubcycle,
-                  boxlen=p.boxlen, device=device)
+                  boxlen=p.boxlen, device=device, graph=graph)
     ic = lambda x, dx: region_condinit(x, dx, p.regions, gamma=p.gamma, smallr=p.smallr, smallc=p.smallc)  # noqa
     if s.adaptive:
         s.init_refine(ic)
@@ -50,7 +51,7 @@ def build(params, precision="fp64", device="cuda"):
     return s
 
 
-def restart(params, outdir, precision="fp64", device="cuda", ordering=None):
+def restart(params, outdir, precision="fp64", device="cuda", ordering=None, graph=False):
     """An AmrSolver and snapshot.RunInfo restarted from
     outdir/output_NNNNN with NNNNN = params.nrestart (init_amr and
     init_hydro's restart, AmrSolver.from_snapshot)."""
@@ -59,21 +60,24 @@ def restart(params, outdir, precision="fp64", device="cuda", ordering=None):
     _configure(p, precision)
     return AmrSolver.from_snapshot(outdir, p.nrestart, p.levelmin, nlevelmax=p.levelmax, ngridmax=_ngridmax(p),
                                    smallr=p.smallr, gamma=p.gamma, ordering=ordering, tout=p.tout,
-                                   refine=p.levelmax > p.levelmin, nsubcycle=p.nsubcycle, device=device)
+                                   refine=p.levelmax > p.levelmin, nsubcycle=p.nsubcycle, device=device,
+                                   graph=graph)
 
 
-def run(path, outdir=".", precision="fp64", max_coarse=None, log=print, ordering="hilbert", bound_key=None):
+def run(path, outdir=".", precision="fp64", max_coarse=None, log=print, ordering="hilbert", bound_key=None,
+        graph=False):
     """ordering / bound_key: the domain decomposition the snapshot records
     (one domain; the Hilbert key range by default).  With nrestart > 0 in
     &RUN_PARAMS the run continues from outdir/output_NNNNN instead, with
-    the decomposition that snapshot records."""
+    the decomposition that snapshot records.  graph: each coarse step is
+    replayed from a captured hipGraph (same results)."""
     from . import snapshot
     from .namelist import Params
     p = Params.from_namelist(path)
     if p.nrestart > 0:
-        s, info = restart(p, outdir, precision, ordering=None)
+        s, info = restart(p, outdir, precision, ordering=None, graph=graph)
     else:
-        s = build(p, precision)
+        s = build(p, precision, graph=graph)
         info = snapshot.RunInfo(levelmin=p.levelmin, gamma=p.gamma, smallr=p.smallr, boxlen=p.boxlen,
                                 noutput=len(p.tout), tout=tuple(p.tout), ordering=ordering, bound_key=bound_key)
     info.levelmin, info.gamma, info.smallr = p.levelmin, p.gamma, p.smallr
@@ -106,8 +110,9 @@ def main(argv=None):
     ap.add_argument("--outdir", default=".")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--max-coarse", type=int, default=None, help="stop after this many coarse steps (of this run)")
+    ap.add_argument("--graph", action="store_true", help="replay each coarse step from a captured hipGraph")
     a = ap.parse_args(argv)
-    run(a.namelist, a.outdir, a.precision, a.max_coarse)
+    run(a.namelist, a.outdir, a.precision, a.max_coarse, graph=a.graph)
 
 
 if __name__ == "__main__":

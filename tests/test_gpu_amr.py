@@ -523,7 +523,8 @@ interpol_type=1
 
 
 @needs_ref
-def test_namelist_run_matches_reference(gpu):
+@pytest.mark.parametrize("graph", [False, True])
+def test_namelist_run_matches_reference(gpu, graph):
     """A parameter file run end to end (ramses_amd.run: namelist, region
     initial conditions in a box of length 0.5, init_refine, two adaptive
     coarse steps with mixed subcycling, snapshot) against the reference's
@@ -541,7 +542,7 @@ def test_namelist_run_matches_reference(gpu):
         with open(path, "w") as f:
             f.write(SMALL_SEDOV_NML)
         s, outs = run.run(path, outdir=os.path.join(base, "o"), log=lambda *a: None, ordering="planar",
-                          bound_key=(-1.0, 1e30))   # the reference build's one-domain ordering
+                          bound_key=(-1.0, 1e30), graph=graph)   # the reference build's one-domain ordering
         p = namelist.Params.from_namelist(path)
         L, lmin = p.levelmax, p.levelmin
         R = oracle.Reference()
