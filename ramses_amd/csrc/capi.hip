@@ -354,6 +354,7 @@ int ramses_amr_copy(const ramses_amr_tree* t, const ramses_amr_level* lv, const 
   int rc = amr_view(t, v);
   if (rc) return rc;
   if (!lv || !src || !dst || (lv->noct > 0 && !lv->active)) return fail(-1, "null argument");
+  if (lv->noct < 0 || lv->noct > t->ngridmax) return fail(-1, "noct out of range");
   hipError_t e = ramses::launch_amr_copy(lv->active, lv->noct, v, src, dst, v.ncell + 1, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "amr copy launch");
   return 0;
@@ -365,6 +366,7 @@ int ramses_amr_upload(const ramses_amr_tree* t, const ramses_amr_level* lv, doub
   int rc = amr_view(t, v);
   if (rc) return rc;
   if (!lv || !u || (lv->noct > 0 && !lv->active)) return fail(-1, "null argument");
+  if (lv->noct < 0 || lv->noct > t->ngridmax) return fail(-1, "noct out of range");
   hipError_t e = ramses::launch_amr_upload(lv->active, lv->noct, v, u, v.ncell + 1, c.hp.smallr, g_interpol_var,
                                            (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "amr upload launch");
@@ -378,6 +380,7 @@ int ramses_amr_courant(const ramses_amr_tree* t, const ramses_amr_level* lv, con
   int rc = amr_view(t, v);
   if (rc) return rc;
   if (!lv || !u || !d_dt || (lv->noct > 0 && !lv->active)) return fail(-1, "null argument");
+  if (lv->noct < 0 || lv->noct > t->ngridmax) return fail(-1, "noct out of range");
   if (!(lv->dx > 0)) return fail(-1, "dx must be > 0");
   double start = c.hp.courant_factor * lv->dx / c.hp.smallc;
   if (dt_start < start) start = dt_start;

@@ -77,3 +77,29 @@ def test_mesh_struct_layout_matches_header(tmp_path):
     M = amr.ramses_amr_mesh
     assert got == [ctypes.sizeof(M), M.free.offset, M.levelmin.offset, M.boxlen.offset, M.work.offset,
                    M.graph.offset, M.graph_state.offset]
+
+
+def test_amr_entry_points_reject_bad_trees_before_any_device_work():
+    """Argument checks of the oct-tree entry points run on the host before
+    any launch: a tree whose cell indices would not fit the reference's
+    32-bit integers, inconsistent coarse sizes, and a level count beyond
+    ngridmax are refused with a message (no device memory is touched: the
+    pointers here are placeholders)."""
+    from ramses_amd import amr
+    lib = ramses_amd.load_library()
+    amr._bind(lib)
+    fake = ctypes.c_void_p(0x1000)
+    lv = amr.ramses_amr_level(3, 0.125, fake, 8, fake, None, 0)
+    cases = [
+        (amr.ramses_amr_tree(fake, fake, fake, 1, 300_000_000, 1, 1, 1), "int32"),
+        (amr.ramses_amr_tree(fake, fake, fake, 2, 1000, 1, 1, 1), "ncoarse"),
+        (amr.ramses_amr_tree(None, fake, fake, 1, 1000, 1, 1, 1), "null"),
+    ]
+    for tree, msg in cases:
+        rc = lib.ramses_amr_upload(ctypes.byref(tree), ctypes.byref(lv), fake, None)
+        assert rc < 0, msg
+        assert msg in lib.ramses_last_error().decode(), lib.ramses_last_error()
+    tree = amr.ramses_amr_tree(fake, fake, fake, 1, 1000, 1, 1, 1)
+    big = amr.ramses_amr_level(3, 0.125, fake, 1001, fake, None, 0)
+    assert lib.ramses_amr_courant(ctypes.byref(tree), ctypes.byref(big), fake, 1.0, fake, None) < 0
+    assert "noct" in lib.ramses_last_error().decode()
