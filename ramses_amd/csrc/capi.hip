@@ -841,6 +841,12 @@ int ramses_amr_coarse_step(ramses_amr_mesh* m, void* stream) {
     if (m->nsubcycle[l] != 1 && m->nsubcycle[l] != 2) return fail(-2, "nsubcycle must be 1 or 2");
   }
   if (!(m->boxlen > 0)) return fail(-1, "boxlen must be > 0");
+  if (m->adaptive && m->levelmin < m->nlevelmax) {
+    const ramses_amr_free& f = m->free;
+    if (!f.ring) return fail(-1, "null free list");
+    if (f.head < 0 || f.head >= m->tree.ngridmax || f.numbf < 0 || f.numbf > m->tree.ngridmax)
+      return fail(-1, "free list head/numbf out of range");
+  }
   if (m->graph) return coarse_step_graph(m, (hipStream_t)stream);
   StepCtx c{m, (hipStream_t)stream, current().hp};
   int rc = amr_step_rec(c, m->levelmin, 1);
