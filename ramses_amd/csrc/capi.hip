@@ -81,11 +81,16 @@ namespace {
 
 thread_local std::string g_err;
 
+// The whole process-wide configuration, read as one snapshot (current())
+// by every entry point: hydro parameters, prolongation and refinement
+// criteria.
 struct Config {
   ramses::HydroParams hp{1.4, 1e-10, 1e-10, 1.5, 0.5, 1, ramses::kLLF, 10};
   int precision = RAMSES_FP64;
   int nvector = 32;
   bool set = false;
+  int interpol_type = 1, interpol_var = 0;  // hydro/hydro_parameters.f90:88-89
+  ramses::RefineParams rp{-1.0, -1.0, -1.0, 1e-10, 1e-10, 1e-10, 1};
 };
 
 std::mutex g_cfg_mu;
@@ -165,6 +170,9 @@ int ramses_hydro_config(const ramses_hydro_params* p) {
   c.nvector = p->nvector;
   c.set = true;
   std::lock_guard<std::mutex> lk(g_cfg_mu);
+  c.interpol_type = g_cfg.interpol_type;
+  c.interpol_var = g_cfg.interpol_var;
+  c.rp = g_cfg.rp;
   g_cfg = c;
   ++g_config_gen;
   return 0;
@@ -265,8 +273,6 @@ int ramses_cmpdt(const double* uu, const double* gg, double dx, int ncell, doubl
 }
 
 namespace {
-static int g_interpol_type = 1, g_interpol_var = 0;
-
 int amr_view(const ramses_amr_tree* t, ramses::AmrTreeView& v) {
   if (!t || !t->son || !t->father || !t->nbor) return fail(-1, "null tree array");
   if (t->ncoarse < 1 || t->ngridmax < 1 || t->nx < 1 || t->ny < 1 || t->nz < 1) return fail(-1, "bad tree sizes");
@@ -295,8 +301,9 @@ size_t amr_work(int noct, int nv) {
 int ramses_amr_config(int interpol_type, int interpol_var) {
   if (interpol_type < 0 || interpol_type > 3) return fail(-2, "interpol_type must be 0..3");
   if (interpol_var < 0 || interpol_var > 2) return fail(-2, "interpol_var must be 0..2");
-  g_interpol_type = interpol_type;
-  g_interpol_var = interpol_var;
+  std::lock_guard<std::mutex> lk(g_cfg_mu);
+  g_cfg.interpol_type = interpol_type;
+  g_cfg.interpol_var = interpol_var;
   ++g_config_gen;
   return 0;
 }
@@ -336,8 +343,8 @@ int ramses_amr_godunov(const ramses_amr_tree* t, const ramses_amr_level* lv, con
   double* tmp = a.flux + nb * nv * 27 * 15;
   a.ok = (unsigned char*)(tmp + nb * nv * 27 * 6);
   a.nv = nv;
-  a.interpol_type = g_interpol_type;
-  a.interpol_var = g_interpol_var;
+  a.interpol_type = c.interpol_type;
+  a.interpol_var = c.interpol_var;
   a.hp = c.hp;
   a.active_coarse = lv->active_coarse;
   a.noct_coarse = lv->active_coarse ? lv->noct_coarse : 0;
@@ -367,7 +374,7 @@ int ramses_amr_upload(const ramses_amr_tree* t, const ramses_amr_level* lv, doub
   if (rc) return rc;
   if (!lv || !u || (lv->noct > 0 && !lv->active)) return fail(-1, "null argument");
   if (lv->noct < 0 || lv->noct > t->ngridmax) return fail(-1, "noct out of range");
-  hipError_t e = ramses::launch_amr_upload(lv->active, lv->noct, v, u, v.ncell + 1, c.hp.smallr, g_interpol_var,
+  hipError_t e = ramses::launch_amr_upload(lv->active, lv->noct, v, u, v.ncell + 1, c.hp.smallr, c.interpol_var,
                                            (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "amr upload launch");
   return 0;
@@ -391,21 +398,17 @@ int ramses_amr_courant(const ramses_amr_tree* t, const ramses_amr_level* lv, con
   return 0;
 }
 
-namespace {
-static const ramses::RefineParams kRefineDefaults{-1.0, -1.0, -1.0, 1e-10, 1e-10, 1e-10, 1};
-static ramses::RefineParams g_refine = kRefineDefaults;
-}  // namespace
-
 int ramses_refine_config(const ramses_refine_params* p) {
-  ++g_config_gen;
-  if (!p) {
-    g_refine = kRefineDefaults;
-    return 0;
+  ramses::RefineParams rp = Config().rp;  // defaults: every criterion off, nexpand 1
+  if (p) {
+    if (p->nexpand < 0 || p->nexpand > 16) return fail(-2, "nexpand must be 0..16");
+    if (!(p->floor_d >= 0) || !(p->floor_u >= 0) || !(p->floor_p >= 0)) return fail(-2, "floors must be >= 0");
+    rp = ramses::RefineParams{p->err_grad_d, p->err_grad_u, p->err_grad_p,
+                              p->floor_d,    p->floor_u,    p->floor_p,    p->nexpand};
   }
-  if (p->nexpand < 0 || p->nexpand > 16) return fail(-2, "nexpand must be 0..16");
-  if (!(p->floor_d >= 0) || !(p->floor_u >= 0) || !(p->floor_p >= 0)) return fail(-2, "floors must be >= 0");
-  g_refine = ramses::RefineParams{p->err_grad_d, p->err_grad_u, p->err_grad_p,
-                                  p->floor_d,    p->floor_u,    p->floor_p,    p->nexpand};
+  std::lock_guard<std::mutex> lk(g_cfg_mu);
+  g_cfg.rp = rp;
+  ++g_config_gen;
   return 0;
 }
 
@@ -419,7 +422,7 @@ int ramses_amr_flag_fine(const ramses_amr_tree* t, const ramses_amr_level* lv, i
   if (lv->noct < 0 || lv->noct > t->ngridmax) return fail(-1, "noct out of range");
   if (lv->ilevel < 1 || levelmin < 1 || nlevelmax < levelmin) return fail(-1, "bad levels");
   hipError_t e = ramses::launch_flag_fine(lv->active, lv->noct, v, lv->ilevel, icount, levelmin, nlevelmax, uold,
-                                          v.ncell + 1, flag1, flag2, g_refine, c.hp, (hipStream_t)stream);
+                                          v.ncell + 1, flag1, flag2, c.rp, c.hp, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "flag_fine launch");
   return 0;
 }
@@ -450,7 +453,7 @@ int ramses_amr_refine_fine(const ramses_amr_tree* t, const ramses_amr_level* lv,
   hipError_t e = ramses::launch_refine_fine(lv->active, lv->noct, lv->ilevel, v, m, active_fine, noct_fine, pos,
                                             fl->ring, &fl->head, &fl->numbf, &fl->used_mem, flag1, flag2, xg,
                                             uold, unew,
-                                            g_interpol_type, g_interpol_var, c.hp.smallr, work, d_error, &nofree,
+                                            c.interpol_type, c.interpol_var, c.hp.smallr, work, d_error, &nofree,
                                             (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "refine_fine");
   if (nofree) return fail(-9, "No more free memory: increase ngridmax");
@@ -465,7 +468,7 @@ namespace {
 struct StepCtx {
   ramses_amr_mesh* m;
   hipStream_t s;
-  ramses::HydroParams hp;
+  Config cfg;
 };
 
 ramses_amr_level level_of(const ramses_amr_mesh* m, int l) {
@@ -515,15 +518,15 @@ int amr_step_rec(StepCtx& c, int l, int icount) {
   if ((rc = amr_view(&m->tree, v))) return rc;
   // newdt_fine: dtold = dtnew, dtnew = min(boxlen/smallc, courant_factor*dx/smallc,
   // dtnew(l-1)/nsubcycle(l-1)), then the courant reduction over the leaves
-  double start = c.hp.courant_factor * lv.dx / c.hp.smallc;
-  const double cap = m->boxlen / c.hp.smallc;
+  double start = c.cfg.hp.courant_factor * lv.dx / c.cfg.hp.smallc;
+  const double cap = m->boxlen / c.cfg.hp.smallc;
   if (cap < start) start = cap;
   const int coarser = l > m->levelmin;
   hipError_t e = ramses::launch_amr_dt(0, m->dt, m->dtold, m->time, l, coarser, coarser ? m->nsubcycle[l - 1] : 1,
                                        0, start, c.s);
   if (e != hipSuccess) return hip_fail(e, "dt launch");
   // courant_fine and set_unew in one pass
-  e = ramses::launch_amr_begin(lv.active, lv.noct, v, m->uold, m->unew, v.ncell + 1, lv.dx, c.hp, m->dt + l, c.s);
+  e = ramses::launch_amr_begin(lv.active, lv.noct, v, m->uold, m->unew, v.ncell + 1, lv.dx, c.cfg.hp, m->dt + l, c.s);
   if (e != hipSuccess) return hip_fail(e, "courant / set_unew launch");
   if (l < m->nlevelmax && m->noct[l + 1] > 0) {
     if ((rc = amr_step_rec(c, l + 1, 1))) return rc;
@@ -538,7 +541,7 @@ int amr_step_rec(StepCtx& c, int l, int icount) {
   rc = ramses_amr_godunov(&m->tree, &lv, m->uold, m->unew, m->dt + l, m->work, m->work_bytes, m->error, c.s);
   if (rc) return rc;
   m->cell_updates += 8LL * lv.noct;
-  e = ramses::launch_amr_finish(lv.active, lv.noct, v, m->unew, m->uold, v.ncell + 1, c.hp.smallr, g_interpol_var,
+  e = ramses::launch_amr_finish(lv.active, lv.noct, v, m->unew, m->uold, v.ncell + 1, c.cfg.hp.smallr, c.cfg.interpol_var,
                                 c.s);
   if (e != hipSuccess) return hip_fail(e, "set_uold/upload launch");
   if (m->adaptive) {
@@ -645,7 +648,7 @@ hipError_t gstep(GCtx& g, int l, int icount) {
     for (int i = l; i < m->nlevelmax; ++i) {
       e = ramses::launch_refine_fine_dev(m->active[i], dn + i, i, g.v, g.tm, m->active[i + 1], dn + i + 1, m->pos,
                                          m->free.ring, g.dev->free, g.dev->ctrl, m->flag1, m->flag2, m->xg, m->uold,
-                                         m->unew, g_interpol_type, g_interpol_var, g.c.hp.smallr, g.work, m->error,
+                                         m->unew, g.c.interpol_type, g.c.interpol_var, g.c.hp.smallr, g.work, m->error,
                                          g.capacity, g.s);
       if (e != hipSuccess) return e;
     }
@@ -686,21 +689,21 @@ hipError_t gstep(GCtx& g, int l, int icount) {
   double* tmp = a.flux + nb * nv * 27 * 15;
   a.ok = (unsigned char*)(tmp + nb * nv * 27 * 6);
   a.nv = nv;
-  a.interpol_type = g_interpol_type;
-  a.interpol_var = g_interpol_var;
+  a.interpol_type = g.c.interpol_type;
+  a.interpol_var = g.c.interpol_var;
   a.hp = g.c.hp;
   a.active_coarse = l > 1 ? m->active[l - 1] : nullptr;
   a.noct_coarse = 0;
   a.error = m->error;
   e = ramses::launch_amr_godunov(a, dx, m->dt + l, tmp, g.c.precision == RAMSES_FP32_MIXED, g.s);
   if (e != hipSuccess) return e;
-  e = ramses::launch_amr_finish(m->active[l], 0, g.v, m->unew, m->uold, us, g.c.hp.smallr, g_interpol_var, g.s,
+  e = ramses::launch_amr_finish(m->active[l], 0, g.v, m->unew, m->uold, us, g.c.hp.smallr, g.c.interpol_var, g.s,
                                 dn + l);
   if (e != hipSuccess) return e;
   if (m->adaptive) {
     const int ic = (l <= m->levelmin || m->nsubcycle[l - 1] == 2) ? icount : 2;
     e = ramses::launch_flag_fine(m->active[l], 0, g.v, l, ic, m->levelmin, m->nlevelmax, m->uold, us, m->flag1,
-                                 m->flag2, g_refine, g.c.hp, g.s, dn + l);
+                                 m->flag2, g.c.rp, g.c.hp, g.s, dn + l);
     if (e != hipSuccess) return e;
   }
   if (l > m->levelmin) {
@@ -723,12 +726,12 @@ hipError_t gcoarse(GCtx& g) {
   if (m->adaptive && m->levelmin < m->nlevelmax) {
     const long us = g.v.ncell + 1;
     for (int l = m->levelmin - 1; l >= 1; --l)
-      if ((e = ramses::launch_amr_upload(m->active[l], 0, g.v, m->uold, us, g.c.hp.smallr, g_interpol_var, g.s,
+      if ((e = ramses::launch_amr_upload(m->active[l], 0, g.v, m->uold, us, g.c.hp.smallr, g.c.interpol_var, g.s,
                                          g.dev->noct + l)) != hipSuccess)
         return e;
     for (int l = m->levelmin - 1; l >= 1; --l)
       if ((e = ramses::launch_flag_fine(m->active[l], 0, g.v, l, 2, m->levelmin, m->nlevelmax, m->uold, us, m->flag1,
-                                        m->flag2, g_refine, g.c.hp, g.s, g.dev->noct + l)) != hipSuccess)
+                                        m->flag2, g.c.rp, g.c.hp, g.s, g.dev->noct + l)) != hipSuccess)
         return e;
     if ((e = ramses::launch_set_ints(m->flag1 + 1, m->tree.ncoarse, 1, g.s)) != hipSuccess) return e;
   }
@@ -848,7 +851,7 @@ int ramses_amr_coarse_step(ramses_amr_mesh* m, void* stream) {
       return fail(-1, "free list head/numbf out of range");
   }
   if (m->graph) return coarse_step_graph(m, (hipStream_t)stream);
-  StepCtx c{m, (hipStream_t)stream, current().hp};
+  StepCtx c{m, (hipStream_t)stream, current()};
   int rc = amr_step_rec(c, m->levelmin, 1);
   if (rc) return rc;
   m->nstep_coarse += 1;
