@@ -56,6 +56,14 @@ def load_library(path=LIB_PATH):
     if not os.path.exists(path):
         raise ImportError(f"{path} is missing: build it with `python -m ramses_amd.build` "
                           "(the HIP library is the only implementation; there is no CPU fallback)")
+    # PyTorch ships its own HIP and HSA runtimes. Loading the library first
+    # would bring in the system copies, and a process holding two runtimes
+    # finds no device in the second one; importing torch first makes the
+    # library bind to the runtime torch already holds.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(path)
     lib.ramses_hydro_config.argtypes = [ctypes.POINTER(ramses_hydro_params)]
     lib.ramses_hydro_config.restype = ctypes.c_int

@@ -103,3 +103,18 @@ def test_amr_entry_points_reject_bad_trees_before_any_device_work():
     big = amr.ramses_amr_level(3, 0.125, fake, 1001, fake, None, 0)
     assert lib.ramses_amr_courant(ctypes.byref(tree), ctypes.byref(big), fake, 1.0, fake, None) < 0
     assert "noct" in lib.ramses_last_error().decode()
+
+
+def test_library_binds_after_torch_runtime():
+    """Loading the library in a fresh process imports torch first, so that
+    the library binds to the HIP runtime torch bundles rather than bringing
+    in a second one (a process with two HIP runtimes finds no device in the
+    second)."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = ("import sys, ramses_amd\n"
+            "assert 'torch' not in sys.modules\n"
+            "ramses_amd.load_library()\n"
+            "assert 'torch' in sys.modules\n")
+    subprocess.run([sys.executable, "-c", code], cwd=root, check=True, timeout=300)
