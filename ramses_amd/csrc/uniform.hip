@@ -7,9 +7,9 @@
 //
 // Kernel structure (2.5D blocking).  A workgroup owns an interior tile of
 // TX x TY columns and marches along z through a chunk of TZ planes.  Its
-// threads cover the tile extended by one cell on each side in x and y, so
-// the traced states of the halo cells that face the tile are computed by a
-// thread like any other.  Per plane:
+// threads cover the tile and the ring of cells next to its four sides (the
+// corners excluded), so the traced states of the halo cells that face the
+// tile are computed by a thread like any other.  Per plane:
 //   1. primitives of the next plane (tile + 2 cells of halo) -> LDS,
 //   2. each thread computes slopes and traced states of its own cell from
 //      its column (registers) and its x/y neighbours (LDS),
@@ -60,17 +60,25 @@ struct BatchArgs {
 
 __device__ __forceinline__ int wrapi(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
+// Threads of a TX x TY tile: one per interior cell plus one per halo cell
+// whose traced state a tile face needs (the 2*(TX+TY) non-corner cells of
+// the one-cell ring), rounded up to whole wavefronts.
+template <int TX, int TY>
+constexpr int tile_threads() { return (TX * TY + 2 * (TX + TY) + 63) / 64 * 64; }
+
 // One workgroup's tile of a level: columns [x0, x0+TX) x [y0, y0+TY),
 // planes [z0, z0+TZ).  Shared by the single-level and the batched kernels.
 template <typename R, int TX, int TY, int RS>
 __device__ __forceinline__ void level_tile(const double* __restrict__ u, double* __restrict__ unew, int n, double dx,
                                            double dt, double* d_dt_next, const HydroParams& hp, int x0, int y0,
                                            int z0, int TZ) {
-  constexpr int EX = TX + 2, EY = TY + 2;      // extended tile (threads)
+  constexpr int EX = TX + 2, EY = TY + 2;      // tile with its one-cell ring
   constexpr int QX = TX + 4, QY = TY + 4;      // primitive plane with 2-cell halo
   constexpr int NQ = QX * QY;
   constexpr int NE = EX * EY;
-  constexpr int NT = (NE + 63) / 64 * 64;
+  constexpr int NI = TX * TY;
+  constexpr int NT = tile_threads<TX, TY>();
+  static_assert(NI % 64 == 0, "interior cells fill whole wavefronts");
 
   __shared__ R qs[2][5][NQ];   // primitive planes (ring of 2)
   __shared__ __align__(16) R xch[10][NE];  // exchange: traced left states, then fluxes
@@ -79,10 +87,36 @@ __device__ __forceinline__ void level_tile(const double* __restrict__ u, double*
   const long plane = (long)n * n;
   const long vstride = plane * n;
   const int t = threadIdx.x;
-  const bool active = t < NE;
-  const int ex = active ? t % EX : 0, ey = active ? t / EX : 0;
-  const bool inx = ex >= 1 && ex <= TX, iny = ey >= 1 && ey <= TY;
-  const bool interior = active && inx && iny;
+  // Threads [0, NI) own the interior cells, x fastest.  Threads [NI, NI +
+  // 2*(TX+TY)) own the ring cells a tile face needs: the left column (its
+  // qm_x), the right column (its qp_x and the face x = TX+1/2), the bottom
+  // row (qm_y) and the top row (qp_y and the face y = TY+1/2).  Corner cells
+  // feed no face and get no thread.  Since NI is a multiple of 64, the ring
+  // cells fill waves of their own, which skip the z face, the update and the
+  // courant epilogue.
+  // (ex, ey) are coordinates in the tile with its ring, interior 1..TX, 1..TY.
+  enum { kNone = -1, kInterior = 0, kLeft, kRight, kBottom, kTop };
+  int ex = 0, ey = 0, kind = kNone;
+  if (t < NI) {
+    ex = 1 + t % TX;
+    ey = 1 + t / TX;
+    kind = kInterior;
+  } else if (t < NI + 2 * (TX + TY)) {
+    int h = t - NI;
+    if (h < 2 * TY) {
+      ex = h < TY ? 0 : TX + 1;
+      ey = 1 + h % TY;
+      kind = h < TY ? kLeft : kRight;
+    } else {
+      h -= 2 * TY;
+      ex = 1 + h % TX;
+      ey = h < TX ? 0 : TY + 1;
+      kind = h < TX ? kBottom : kTop;
+    }
+  }
+  const bool active = kind != kNone;
+  const bool interior = kind == kInterior;
+  const int e = ey * EX + ex;  // own slot in the exchange area
   const int xg = wrapi(x0 - 1 + ex, n), yg = wrapi(y0 - 1 + ey, n);
   const int qo = (ey + 1) * QX + (ex + 1);  // own cell in the primitive plane
 
@@ -153,21 +187,23 @@ __device__ __forceinline__ void level_tile(const double* __restrict__ u, double*
     double part[5] = {0, 0, 0, 0, 0};
     if (faces_xy) {
       if (active) {
-        xch[0][t] = qm_x.r; xch[1][t] = qm_x.u; xch[2][t] = qm_x.v; xch[3][t] = qm_x.w; xch[4][t] = qm_x.p;
-        xch[5][t] = qm_y.r; xch[6][t] = qm_y.u; xch[7][t] = qm_y.v; xch[8][t] = qm_y.w; xch[9][t] = qm_y.p;
+        xch[0][e] = qm_x.r; xch[1][e] = qm_x.u; xch[2][e] = qm_x.v; xch[3][e] = qm_x.w; xch[4][e] = qm_x.p;
+        xch[5][e] = qm_y.r; xch[6][e] = qm_y.u; xch[7][e] = qm_y.v; xch[8][e] = qm_y.w; xch[9][e] = qm_y.p;
       }
       __syncthreads();
       double fxs[5] = {0, 0, 0, 0, 0}, fys[5] = {0, 0, 0, 0, 0};
-      if (active && iny && ex >= 1) {  // x face at x-1/2
-        const Prim<R> l{xch[0][t - 1], xch[1][t - 1], xch[2][t - 1], xch[3][t - 1], xch[4][t - 1]};
+      if (interior || kind == kRight) {  // x face at x-1/2
+        const int lo = e - 1;
+        const Prim<R> l{xch[0][lo], xch[1][lo], xch[2][lo], xch[3][lo], xch[4][lo]};
         const Flux6<R> g = riemann_fixed<R, RS>(rotate(l, 0), rotate(qp_x, 0), sc);
         R fc[5];
         unrotate(g, 0, fc);
 #pragma unroll
         for (int v = 0; v < 5; ++v) fxs[v] = scale_flux(fc[v], fs);  // hydro/umuscl.f90:106
       }
-      if (active && inx && ey >= 1) {  // y face at y-1/2
-        const Prim<R> l{xch[5][t - EX], xch[6][t - EX], xch[7][t - EX], xch[8][t - EX], xch[9][t - EX]};
+      if (interior || kind == kTop) {  // y face at y-1/2
+        const int lo = e - EX;
+        const Prim<R> l{xch[5][lo], xch[6][lo], xch[7][lo], xch[8][lo], xch[9][lo]};
         const Flux6<R> g = riemann_fixed<R, RS>(rotate(l, 1), rotate(qp_y, 1), sc);
         R fc[5];
         unrotate(g, 1, fc);
@@ -179,26 +215,26 @@ __device__ __forceinline__ void level_tile(const double* __restrict__ u, double*
       double* fx_sh = reinterpret_cast<double*>(&xch[0][0]);
       constexpr int FSTRIDE = NE;  // doubles per variable
       static_assert(sizeof(R) * 10 * NE >= sizeof(double) * 5 * NE, "exchange area too small");
-      if (active) {
+      if (interior || kind == kRight) {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) fx_sh[v * FSTRIDE + t] = fxs[v];
+        for (int v = 0; v < 5; ++v) fx_sh[v * FSTRIDE + e] = fxs[v];
       }
       __syncthreads();
       if (interior) {
         // hydro/godunov_fine.f90:770-772, idim=1: unew = uold + (flux(i) - flux(i+1))
         const long o = (long)wrapi(p, n) * plane + (long)yg * n + xg;
 #pragma unroll
-        for (int v = 0; v < 5; ++v) part[v] = u[o + v * vstride] + (fx_sh[v * FSTRIDE + t] - fx_sh[v * FSTRIDE + t + 1]);
+        for (int v = 0; v < 5; ++v) part[v] = u[o + v * vstride] + (fx_sh[v * FSTRIDE + e] - fx_sh[v * FSTRIDE + e + 1]);
       }
       __syncthreads();
-      if (active) {
+      if (interior || kind == kTop) {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) fx_sh[v * FSTRIDE + t] = fys[v];
+        for (int v = 0; v < 5; ++v) fx_sh[v * FSTRIDE + e] = fys[v];
       }
       __syncthreads();
       if (interior) {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) part[v] = part[v] + (fx_sh[v * FSTRIDE + t] - fx_sh[v * FSTRIDE + t + EX]);  // idim=2
+        for (int v = 0; v < 5; ++v) part[v] = part[v] + (fx_sh[v * FSTRIDE + e] - fx_sh[v * FSTRIDE + e + EX]);  // idim=2
       }
     }
     // z face below plane p (between p-1 and p)
@@ -251,7 +287,7 @@ __device__ __forceinline__ void level_tile(const double* __restrict__ u, double*
 
 
 template <typename R, int TX, int TY, int RS>
-__global__ __launch_bounds__(((TX + 2) * (TY + 2) + 63) / 64 * 64)
+__global__ __launch_bounds__((tile_threads<TX, TY>()))
 __attribute__((amdgpu_waves_per_eu(3, 8))) void uniform_godunov_kernel(UniformArgs a, int TZ) {
   // XCD-aware tile order: the hardware deals workgroups round-robin to the
   // 8 XCDs, so workgroup b runs on XCD b%8.  Giving each XCD a contiguous
@@ -272,7 +308,7 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void uniform_godunov_kernel(UniformAr
 // Workgroups [blk0[s], blk0[s+1]) belong to slot s; a slot whose state says
 // `done` is skipped.  The slot's buffer parity selects input and output.
 template <typename R, int TX, int TY, int RS>
-__global__ __launch_bounds__(((TX + 2) * (TY + 2) + 63) / 64 * 64)
+__global__ __launch_bounds__((tile_threads<TX, TY>()))
 __attribute__((amdgpu_waves_per_eu(3, 8))) void batch_godunov_kernel(BatchArgs a) {
   const int b = blockIdx.x;
   int s = 0;
@@ -371,7 +407,7 @@ __global__ void cmpdt_group(const double* uu, const double* gg, int nv, int ncel
 
 template <typename R, int TX, int TY>
 hipError_t launch_tile(const UniformArgs& a, int TZ, hipStream_t s) {
-  constexpr int NT = ((TX + 2) * (TY + 2) + 63) / 64 * 64;
+  constexpr int NT = tile_threads<TX, TY>();
   dim3 grid(a.n / TX, a.n / TY, a.n / TZ);
   switch (a.hp.riemann) {
     case kHLLC: hipLaunchKernelGGL((uniform_godunov_kernel<R, TX, TY, kHLLC>), grid, dim3(NT), 0, s, a, TZ); break;
@@ -387,7 +423,7 @@ hipError_t launch_tile(const UniformArgs& a, int TZ, hipStream_t s) {
 
 template <typename R, int RS>
 void launch_batch_t(const BatchArgs& a, int nblocks, hipStream_t s) {
-  constexpr int TX = 16, TY = 8, NT = ((TX + 2) * (TY + 2) + 63) / 64 * 64;
+  constexpr int TX = 16, TY = 8, NT = tile_threads<TX, TY>();
   hipLaunchKernelGGL((batch_godunov_kernel<R, TX, TY, RS>), dim3(nblocks), dim3(NT), 0, s, a);
 }
 

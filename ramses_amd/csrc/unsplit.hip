@@ -29,10 +29,22 @@ namespace {
 
 using unsplit_dev::unsplit_task;
 
+// The dispatcher hands workgroup b to XCD b % 8, and each XCD has its own
+// L2.  Workgroup b is given the logical index that makes every XCD run one
+// contiguous range of logical workgroups (a bijection of [0, G)), so the
+// workgroups that share one batch's stencil cells (a batch is 384 tasks,
+// a workgroup 64..768) run on the same XCD and re-read those cells from
+// its L2 rather than from HBM.
+__device__ __forceinline__ unsigned xcd_contiguous(unsigned b, unsigned G) {
+  constexpr unsigned X = 8;
+  const unsigned x = b % X, q = G / X, r = G % X;
+  return x * q + (x < r ? x : r) + b / X;
+}
+
 // One thread per (group, row, lane) task.
 template <typename R, int RS, bool S27>
 __global__ __launch_bounds__(768) void unsplit_rows(UnsplitArgs a) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long t = (long)xcd_contiguous(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= a.ngroups * 12L * a.nv) return;
   unsplit_task<R, RS, S27>(a, t, a.ngroups, a.ngrid_last);
 }
