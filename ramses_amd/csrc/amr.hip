@@ -26,13 +26,9 @@ namespace ramses {
 
 namespace {
 
-// Thread = (batch, father neighbour j in 0..26, lane); workgroup = one batch
-// (grid-stride over the batches when the oct count is read from device memory).
-__device__ __forceinline__ void gather_task(const AmrLevelArgs& a, int noct, int b) {
+// One (batch b, father neighbour j in 0..26, lane) task.
+__device__ __forceinline__ void gather_task(const AmrLevelArgs& a, int noct, int b, int j, int lane) {
   const int nv = a.nv;
-  const int lane = threadIdx.x % nv;
-  const int j = threadIdx.x / nv;
-  if (j >= 27) return;
   const int oct = b * nv + lane;
   if (oct >= noct) return;
   const AmrTreeView& t = a.t;
@@ -91,10 +87,21 @@ __device__ __forceinline__ void gather_task(const AmrLevelArgs& a, int noct, int
   }
 }
 
-__global__ __launch_bounds__(1024) void amr_gather(AmrLevelArgs a) {
+// Flat grid-stride over the 27*nv tasks of every batch, lane fastest, so a
+// wavefront covers two neighbours j of 32 octs and every (cell, variable)
+// store is one coalesced segment.  A level of a few thousand octs spreads
+// over all CUs instead of one workgroup per batch: each task is a chain of
+// dependent tree loads, and the chains of one CU share its memory pipeline.
+__global__ __launch_bounds__(256) void amr_gather(AmrLevelArgs a) {
   const int noct = level_count(a.noct, a.dnoct);
-  const int nb = (noct + a.nv - 1) / a.nv;
-  for (int b = blockIdx.x; b < nb; b += gridDim.x) gather_task(a, noct, b);
+  const int nv = a.nv;
+  const long per_batch = 27L * nv;
+  const long ntask = (long)((noct + nv - 1) / nv) * per_batch;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < ntask; t += (long)gridDim.x * blockDim.x) {
+    const int b = (int)(t / per_batch);
+    const int rem = (int)(t - b * per_batch);
+    gather_task(a, noct, b, rem / nv, rem % nv);
+  }
 }
 
 __device__ __forceinline__ size_t fslot(int i3, int j3, int k3) { return (size_t)((i3 - 1) + 3 * ((j3 - 1) + 3 * (k3 - 1))); }
@@ -469,7 +476,8 @@ hipError_t launch_amr_godunov(const AmrLevelArgs& a, double dx, const double* d_
   const bool dev = a.dnoct != nullptr;
   if (!dev && a.noct <= 0) return hipSuccess;
   const int nbatch = (a.noct + a.nv - 1) / a.nv;
-  hipLaunchKernelGGL(amr_gather, dim3(dev ? persist_blocks() / 2 : (unsigned)nbatch), dim3(27 * a.nv), 0, s, a);
+  const unsigned gblocks = (unsigned)(((long)nbatch * 27 * a.nv + 255) / 256);
+  hipLaunchKernelGGL(amr_gather, dim3(dev ? persist_blocks() : gblocks), dim3(256), 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   UnsplitArgs u;
