@@ -1,5 +1,5 @@
 // One level of an oct-tree (AMR) godunov sweep on the GPU:
-// godfine1 (hydro/godunov_fine.f90:486-911) split into three launches over
+// godfine1 (hydro/godunov_fine.f90:486-911) split into four launches over
 // the level's active octs, all reading the tree (son / father / nbor) and
 // the state (uold / unew) resident in HBM in the reference's memory model:
 //   amr_gather   6x6x6 stencil of every oct in nvector-strided batches
@@ -7,9 +7,10 @@
 //                missing neighbour oct is a buffer oct interpolated from its
 //                father's 7-point neighbourhood: getnborfather :404-525 and
 //                interpol_hydro, hydro/interpol_hydro.f90:268-637);
-//   amr_solve    per batch, unsplit (the device code of unsplit.hip), then
-//                the flux reset at refined faces and the level update
-//                (:720-792);
+//   amr_unsplit  unsplit of every batch (the device code of unsplit.hip),
+//                storing a zero flux at faces next to a refined cell
+//                (:720-749);
+//   amr_update   the level update unew += F(i) - F(i+1) (:751-792);
 //   amr_reflux   the conservative correction of coarser leaf neighbours
 //                (:798-908), gathered per coarse cell and summed in the
 //                reference's order (batch, dimension, side, face) so the
@@ -109,72 +110,60 @@ __device__ __forceinline__ size_t sslot(int i3, int j3, int k3) {
   return (size_t)((i3 + 1) + 6 * ((j3 + 1) + 6 * (k3 + 1)));
 }
 
-__device__ __forceinline__ void update_batch(const AmrLevelArgs& a, int noct, int b) {
+// Level update of one (cell ind, variable v) of lane `lane` of batch b, q =
+// (ind-1)*5 + v: unew += flux(i3) - flux(i3+1) for idim = 1, 2, 3 in order
+// (hydro/godunov_fine.f90:751-792).
+__device__ __forceinline__ void update_cell(const AmrLevelArgs& a, int b, int q, int lane) {
   const int nv = a.nv;
-  const int nlane = min(nv, noct - b * nv);
-  const AmrTreeView& t = a.t;
   const size_t fvs = (size_t)nv * 27;
-  double* flux = a.flux + (size_t)b * fvs * 15;
-  const unsigned char* ok = a.ok + (size_t)b * nv * 216;
-  for (int it = threadIdx.x; it < 36 * nv; it += blockDim.x) {
-    const int lane = it % nv, f = it / nv;
-    if (lane >= nlane) continue;
-    const int d = f / 12, r = f % 12;
+  const double* flux = a.flux + (size_t)b * fvs * 15;
+  const int ind = q / 5 + 1, v = q % 5;
+  const int i2 = (ind - 1) & 1, j2 = ((ind - 1) >> 1) & 1, k2 = (ind - 1) >> 2;
+  const int i3 = 1 + i2, j3 = 1 + j2, k3 = 1 + k2;
+  const long c = (long)v * a.ustride + cell_of(a.t, ind, a.active[b * nv + lane]);
+  double acc = a.unew[c];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
     const int i0 = d == 0, j0 = d == 1, k0 = d == 2;
-    const int ni = 2 + i0, nj = 2 + j0;
-    const int i3 = 1 + r % ni, j3 = 1 + (r / ni) % nj, k3 = 1 + r / (ni * nj);
-    if (ok[sslot(i3 - i0, j3 - j0, k3 - k0) * nv + lane] || ok[sslot(i3, j3, k3) * nv + lane]) {
-#pragma unroll
-      for (int v = 0; v < 5; ++v) flux[(fslot(i3, j3, k3) + (size_t)27 * (v + 5 * d)) * nv + lane] = 0.0;
-    }
+    const size_t base = (size_t)27 * (v + 5 * d);
+    const double fl = flux[(fslot(i3, j3, k3) + base) * nv + lane];
+    const double fr = flux[(fslot(i3 + i0, j3 + j0, k3 + k0) + base) * nv + lane];
+    acc = acc + (fl - fr);
   }
-  __syncthreads();
-  for (int it = threadIdx.x; it < 40 * nv; it += blockDim.x) {
-    const int lane = it % nv, q = it / nv;
-    if (lane >= nlane) continue;
-    const int ind = q / 5 + 1, v = q % 5;
-    const int i2 = (ind - 1) & 1, j2 = ((ind - 1) >> 1) & 1, k2 = (ind - 1) >> 2;
-    const int i3 = 1 + i2, j3 = 1 + j2, k3 = 1 + k2;
-    const long c = (long)v * a.ustride + cell_of(t, ind, a.active[b * nv + lane]);
-    double acc = a.unew[c];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const int i0 = d == 0, j0 = d == 1, k0 = d == 2;
-      const size_t base = (size_t)27 * (v + 5 * d);
-      const double fl = flux[(fslot(i3, j3, k3) + base) * nv + lane];
-      const double fr = flux[(fslot(i3 + i0, j3 + j0, k3 + k0) + base) * nv + lane];
-      acc = acc + (fl - fr);
-    }
-    a.unew[c] = acc;
-  }
+  a.unew[c] = acc;
 }
 
-// unsplit, flux reset at faces touching a refined cell, and the level
-// update unew += flux(i3) - flux(i3+1) per dimension, one nvector batch per
-// workgroup at a time: first one thread per (row, lane) computes the batch's
-// 36 faces per oct (unsplit_task, as the batched unsplit kernel does), then
-// one thread per (face, lane) resets fluxes and one thread per (cell,
-// variable, lane) accumulates the three dimensions in order.  The batch's
-// fluxes stay in its workgroup's L2 between the phases.
-// Workgroup of 384 threads (12 rows x 32 lanes; larger nvector loops), at
-// least three waves per SIMD like the batched unsplit kernel.
+// The level's unsplit with one (batch, row, lane) task per thread over all
+// batches (workgroups of 128), the flux reset at faces next to a refined
+// cell folded into the face stores.  Small levels spread over every CU.
 template <typename R, int RS, bool S27>
-__global__ __launch_bounds__(384, 3) void amr_solve(AmrLevelArgs a, UnsplitArgs u) {
+__global__ __launch_bounds__(768) void amr_unsplit(AmrLevelArgs a, UnsplitArgs u) {
   const int noct = level_count(a.noct, a.dnoct);
   const int nv = a.nv;
   const int nb = (noct + nv - 1) / nv;
   const int ngrid_last = noct - (nb - 1) * nv;
-  for (int b = blockIdx.x; b < nb; b += gridDim.x) {
-    for (int t = threadIdx.x; t < 12 * nv; t += blockDim.x)
-      unsplit_dev::unsplit_task<R, RS, S27>(u, (long)b * 12 * nv + t, nb, ngrid_last);
-    __syncthreads();
-    update_batch(a, noct, b);
+  const long ntask = (long)nb * 12 * nv;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < ntask; t += (long)gridDim.x * blockDim.x)
+    unsplit_dev::unsplit_task<R, RS, S27, true>(u, t, nb, ngrid_last);
+}
+
+// The level update, one (batch, cell, variable, lane) task per thread.
+__global__ __launch_bounds__(256) void amr_update(AmrLevelArgs a) {
+  const int noct = level_count(a.noct, a.dnoct);
+  const int nv = a.nv;
+  const long per_batch = 40L * nv;
+  const long ntask = (long)((noct + nv - 1) / nv) * per_batch;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < ntask; t += (long)gridDim.x * blockDim.x) {
+    const int b = (int)(t / per_batch);
+    const int rem = (int)(t - b * per_batch);
+    const int lane = rem % nv;
+    if (b * nv + lane < noct) update_cell(a, b, rem / nv, lane);
   }
 }
 
 template <typename R, int RS, bool S27>
 void launch_solve_t(const AmrLevelArgs& a, const UnsplitArgs& u, unsigned grid, unsigned block, hipStream_t s) {
-  hipLaunchKernelGGL((amr_solve<R, RS, S27>), dim3(grid), dim3(block), 0, s, a, u);
+  hipLaunchKernelGGL((amr_unsplit<R, RS, S27>), dim3(grid), dim3(block), 0, s, a, u);
 }
 
 template <typename R>
@@ -492,12 +481,19 @@ hipError_t launch_amr_godunov(const AmrLevelArgs& a, double dx, const double* d_
   u.ngrid_last = a.noct - (nbatch - 1) * a.nv;
   u.nv = a.nv;
   u.hp = a.hp;
-  const unsigned block = 384;
-  const unsigned grid = dev ? persist_blocks() : (unsigned)nbatch;
+  u.ok = a.ok;
+  const unsigned block = 128;
+  // the fixed grids of the captured mode keep the 384 x persist_blocks()
+  // threads of one batch per 384-thread workgroup
+  const unsigned grid = dev ? 3 * persist_blocks() : (unsigned)(((long)nbatch * 12 * a.nv + block - 1) / block);
   if (fp32)
     launch_solve_r<float>(a, u, grid, block, s);
   else
     launch_solve_r<double>(a, u, grid, block, s);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const unsigned ugrid = dev ? 2 * persist_blocks() : (unsigned)(((long)nbatch * 40 * a.nv + 255) / 256);
+  hipLaunchKernelGGL(amr_update, dim3(ugrid), dim3(256), 0, s, a);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (a.active_coarse && (a.dnoct_coarse || a.noct_coarse > 0)) {

@@ -18,6 +18,10 @@ struct UnsplitArgs {
   int ngrid_last;
   int nv;
   HydroParams hp;
+  // Oct-tree sweeps only: per-batch refinement flags of the stencil cells
+  // (nv x 216 bytes per batch, lane fastest); a face next to a flagged cell
+  // gets a zero flux.  Null for the batched boundary call.
+  const unsigned char* ok = nullptr;
 };
 
 }  // namespace ramses

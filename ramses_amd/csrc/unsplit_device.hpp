@@ -109,9 +109,13 @@ __device__ __forceinline__ void store_face(double* flux, double* tmp, size_t fba
 }
 
 // One row of faces along direction D of one oct (see the file comment).
-template <typename R, int D, int RS, bool S27>
+// With ZR, ok points at the batch's refinement flags (+ lane) and a face
+// with a flagged cell on either side stores a zero flux: the value the
+// reference's reset leaves there (hydro/godunov_fine.f90:720-749).
+template <typename R, int D, int RS, bool S27, bool ZR>
 __device__ __forceinline__ void row_faces(const RowCtx<R>& c, const SolverConsts<R>& sc, int a1, int b1,
-                                          R dtdx, double* flux, double* tmp, size_t fvs, const FluxScale fs) {
+                                          R dtdx, double* flux, double* tmp, size_t fvs, const FluxScale fs,
+                                          const unsigned char* ok) {
   constexpr int T1 = (D == 0) ? 1 : 0;  // first transverse axis
   constexpr int T2 = (D == 2) ? 1 : 2;  // second transverse axis
   int ci, cj, ck;
@@ -155,6 +159,14 @@ __device__ __forceinline__ void row_faces(const RowCtx<R>& c, const SolverConsts
       const Flux6<R> g = riemann_fixed<R, RS>(fl_prev, fr, sc);
       const size_t fslot = (size_t)((i - 1) + 3 * ((j - 1) + 3 * (k - 1))) * c.nv;
       store_face(flux, tmp, fslot + (size_t)D * 5 * fvs, fslot + (size_t)D * 2 * fvs, fvs, D, g, fl_prev, fr, fs);
+      if (ZR) {
+        int li = i, lj = j, lk = k;  // left cell of the face
+        if (D == 0) --li; else if (D == 1) --lj; else --lk;
+        if (ok[(size_t)cell6(li, lj, lk) * c.nv] || ok[(size_t)cell6(i, j, k) * c.nv]) {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) flux[fslot + (size_t)D * 5 * fvs + (size_t)v * fvs] = 0.0;
+        }
+      }
     }
     fl_prev = rotate(qm, D);
     qa = qb;
@@ -162,8 +174,8 @@ __device__ __forceinline__ void row_faces(const RowCtx<R>& c, const SolverConsts
   }
 }
 
-// One (group, row, lane) task of the flat decomposition.
-template <typename R, int RS, bool S27>
+// One (group, row, lane) task of the flat decomposition (ZR: see row_faces).
+template <typename R, int RS, bool S27, bool ZR = false>
 __device__ __forceinline__ void unsplit_task(const UnsplitArgs& a, long t, long ngroups, int ngrid_last) {
   const int nv = a.nv;
   const long per_group = 12L * nv;
@@ -198,12 +210,13 @@ __device__ __forceinline__ void unsplit_task(const UnsplitArgs& a, long t, long 
   double* flux = a.flux + gidx * gsz_f + l;
   double* tmp = a.tmp + gidx * gsz_t + l;
   const size_t fvs = (size_t)nv * 27;
+  const unsigned char* ok = ZR ? a.ok + gidx * (size_t)nv * 216 + l : nullptr;
   if (d == 0)
-    row_faces<R, 0, RS, S27>(c, sc, a1, b1, (R)(dt / a.dx[0]), flux, tmp, fvs, make_scale(dt, a.dx[0]));  // :516
+    row_faces<R, 0, RS, S27, ZR>(c, sc, a1, b1, (R)(dt / a.dx[0]), flux, tmp, fvs, make_scale(dt, a.dx[0]), ok);  // :516
   else if (d == 1)
-    row_faces<R, 1, RS, S27>(c, sc, a1, b1, (R)(dt / a.dx[1]), flux, tmp, fvs, make_scale(dt, a.dx[1]));
+    row_faces<R, 1, RS, S27, ZR>(c, sc, a1, b1, (R)(dt / a.dx[1]), flux, tmp, fvs, make_scale(dt, a.dx[1]), ok);
   else
-    row_faces<R, 2, RS, S27>(c, sc, a1, b1, (R)(dt / a.dx[2]), flux, tmp, fvs, make_scale(dt, a.dx[2]));
+    row_faces<R, 2, RS, S27, ZR>(c, sc, a1, b1, (R)(dt / a.dx[2]), flux, tmp, fvs, make_scale(dt, a.dx[2]), ok);
 }
 
 }  // namespace unsplit_dev
