@@ -347,6 +347,8 @@ int ramses_amr_mesh_release(ramses_amr_mesh *m);
 #define RAMSES_TUNE_LEVEL_TILE 2    /* level tile: 1 = 32x8, 2 = 16x8, 3 = 8x8, 4 = 32x16 columns */
 #define RAMSES_TUNE_LEVEL_TZ 3      /* z planes per level workgroup: a divisor of n, >= 4 */
 #define RAMSES_TUNE_PERSIST_BLOCKS 4 /* workgroups of a captured AMR step's fixed grids (0: 1024) */
+#define RAMSES_TUNE_AMR_FACE_OCTS 5  /* levels of at most this many octs run the AMR unsplit one face per
+                                        thread instead of one row per thread (0: 8192, -1: never) */
 
 /* Set one tuning knob (amr/amr_parameters.f90:49-53 is the reference's
  * counterpart); returns 0, or < 0 for an unknown knob or invalid value.

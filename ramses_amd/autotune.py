@@ -15,7 +15,7 @@ import ctypes
 
 from . import _check, load_library
 
-UNSPLIT_BLOCK, LEVEL_TILE, LEVEL_TZ, PERSIST_BLOCKS = 1, 2, 3, 4
+UNSPLIT_BLOCK, LEVEL_TILE, LEVEL_TZ, PERSIST_BLOCKS, AMR_FACE_OCTS = 1, 2, 3, 4, 5
 _cache = {}
 
 
@@ -110,5 +110,5 @@ def tune_amr_graph(solver, key, blocks=(256, 512, 1024, 2048), rounds=2, verbose
 
 
 def reset():
-    for k in (UNSPLIT_BLOCK, LEVEL_TILE, LEVEL_TZ, PERSIST_BLOCKS):
+    for k in (UNSPLIT_BLOCK, LEVEL_TILE, LEVEL_TZ, PERSIST_BLOCKS, AMR_FACE_OCTS):
         _set(k, 0)

@@ -106,9 +106,6 @@ __global__ __launch_bounds__(256) void amr_gather(AmrLevelArgs a) {
 }
 
 __device__ __forceinline__ size_t fslot(int i3, int j3, int k3) { return (size_t)((i3 - 1) + 3 * ((j3 - 1) + 3 * (k3 - 1))); }
-__device__ __forceinline__ size_t sslot(int i3, int j3, int k3) {
-  return (size_t)((i3 + 1) + 6 * ((j3 + 1) + 6 * (k3 + 1)));
-}
 
 // Level update of one (cell ind, variable v) of lane `lane` of batch b, q =
 // (ind-1)*5 + v: unew += flux(i3) - flux(i3+1) for idim = 1, 2, 3 in order
@@ -133,18 +130,27 @@ __device__ __forceinline__ void update_cell(const AmrLevelArgs& a, int b, int q,
   a.unew[c] = acc;
 }
 
-// The level's unsplit with one (batch, row, lane) task per thread over all
-// batches (workgroups of 128), the flux reset at faces next to a refined
-// cell folded into the face stores.  Small levels spread over every CU.
+// The level's unsplit over all batches (workgroups of 128), the flux reset
+// at faces next to a refined cell folded into the face stores: one (batch,
+// face, lane) task per thread on levels of at most u.face_octs octs, where
+// the chain of dependent steps matters more than the arithmetic, else one
+// (batch, row, lane) task per thread.  Small levels spread over every CU.
 template <typename R, int RS, bool S27>
 __global__ __launch_bounds__(768) void amr_unsplit(AmrLevelArgs a, UnsplitArgs u) {
   const int noct = level_count(a.noct, a.dnoct);
   const int nv = a.nv;
   const int nb = (noct + nv - 1) / nv;
   const int ngrid_last = noct - (nb - 1) * nv;
-  const long ntask = (long)nb * 12 * nv;
-  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < ntask; t += (long)gridDim.x * blockDim.x)
-    unsplit_dev::unsplit_task<R, RS, S27, true>(u, t, nb, ngrid_last);
+  const long stride = (long)gridDim.x * blockDim.x;
+  if (noct <= u.face_octs) {
+    const long ntask = (long)nb * 36 * nv;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < ntask; t += stride)
+      unsplit_dev::face_task<R, RS, S27, true>(u, t, nb, ngrid_last);
+  } else {
+    const long ntask = (long)nb * 12 * nv;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < ntask; t += stride)
+      unsplit_dev::unsplit_task<R, RS, S27, true>(u, t, nb, ngrid_last);
+  }
 }
 
 // The level update, one (batch, cell, variable, lane) task per thread.
@@ -482,10 +488,12 @@ hipError_t launch_amr_godunov(const AmrLevelArgs& a, double dx, const double* d_
   u.nv = a.nv;
   u.hp = a.hp;
   u.ok = a.ok;
+  u.face_octs = amr_face_octs();
   const unsigned block = 128;
+  const long tasks = (long)nbatch * (a.noct <= u.face_octs ? 36 : 12) * a.nv;
   // the fixed grids of the captured mode keep the 384 x persist_blocks()
   // threads of one batch per 384-thread workgroup
-  const unsigned grid = dev ? 3 * persist_blocks() : (unsigned)(((long)nbatch * 12 * a.nv + block - 1) / block);
+  const unsigned grid = dev ? 3 * persist_blocks() : (unsigned)((tasks + block - 1) / block);
   if (fp32)
     launch_solve_r<float>(a, u, grid, block, s);
   else

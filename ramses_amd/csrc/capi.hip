@@ -69,7 +69,7 @@ hipError_t launch_refine_fine(const int* active, int noct, int ilevel, const Amr
                               int interpol_type, int interpol_var, double smallr, void* work, int* d_error,
                               int* nofree, hipStream_t s);
 Tuning& tuning() {
-  static Tuning t{0, 0, 0, 0};
+  static Tuning t{0, 0, 0, 0, 0};
   return t;
 }
 }  // namespace ramses
@@ -908,6 +908,10 @@ int ramses_tune(int knob, int value) {
       if (value != 0 && (value < 32 || value > 8192)) return fail(-2, "persist blocks must be 0 or in [32, 8192]");
       t.persist_blocks = value;
       return 0;
+    case RAMSES_TUNE_AMR_FACE_OCTS:
+      if (value < -1) return fail(-2, "amr face octs must be -1, 0 or a positive oct count");
+      t.amr_face_octs = value;
+      return 0;
     default:
       return fail(-1, "unknown tuning knob");
   }
@@ -920,6 +924,7 @@ int ramses_tune_get(int knob) {
     case RAMSES_TUNE_LEVEL_TILE: return t.level_tile;
     case RAMSES_TUNE_LEVEL_TZ: return t.level_tz;
     case RAMSES_TUNE_PERSIST_BLOCKS: return (int)ramses::persist_blocks();
+    case RAMSES_TUNE_AMR_FACE_OCTS: return ramses::amr_face_octs();
     default: return fail(-1, "unknown tuning knob");
   }
 }

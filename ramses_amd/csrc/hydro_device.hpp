@@ -38,10 +38,13 @@ struct Tuning {
   int level_tile;
   int level_tz;
   int persist_blocks;  // workgroups of a captured AMR step's fixed-grid kernels (0: default)
+  int amr_face_octs;   // largest level run face-per-thread by the AMR unsplit (0: default, -1: never)
 };
 Tuning& tuning();
 // Workgroups of the fixed grids of a captured AMR coarse step.
 inline unsigned persist_blocks() { return tuning().persist_blocks > 0 ? (unsigned)tuning().persist_blocks : 1024u; }
+// Levels of at most this many octs run the AMR unsplit one face per thread.
+inline int amr_face_octs() { return tuning().amr_face_octs > 0 ? tuning().amr_face_octs : (tuning().amr_face_octs < 0 ? 0 : 8192); }
 
 // Fortran MAX/MIN of two reals.  The hardware min/max (IEEE maxNum/minNum)
 // returns the same value as the reference for every non-NaN pair; the only

@@ -22,6 +22,9 @@ struct UnsplitArgs {
   // (nv x 216 bytes per batch, lane fastest); a face next to a flagged cell
   // gets a zero flux.  Null for the batched boundary call.
   const unsigned char* ok = nullptr;
+  // Oct-tree sweeps only: at or below this many octs the level's unsplit
+  // runs one face per thread instead of one row of faces per thread.
+  int face_octs = 0;
 };
 
 }  // namespace ramses
