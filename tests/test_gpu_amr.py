@@ -245,13 +245,25 @@ def test_flag_and_refine_vs_reference(gpu, egd, egp):
 
 
 @needs_ref
-@pytest.mark.parametrize("graph", [False, True])
-def test_amr_adaptive_steps_vs_reference(gpu, graph):
+@pytest.mark.parametrize("graph,face_octs", [(False, 0), (True, 0), (False, -1), (True, -1)])
+def test_amr_adaptive_steps_vs_reference(gpu, graph, face_octs):
     """Three coarse steps of the adaptive multi-level step (refine at the
     start of the levelmin step and of every second substep, flag after
     every level step) against the reference's routines driven in
-    amr_step's order: tree, grid lists, free list, state and dts, bitwise."""
+    amr_step's order: tree, grid lists, free list, state and dts, bitwise.
+    face_octs 0 keeps the default (these small levels run the level unsplit
+    one face per thread); -1 runs it one row of faces per thread."""
     import ramses_amd as ra
+    lib = ra.load_library()
+    assert lib.ramses_tune(5, face_octs) == 0
+    try:
+        _adaptive_steps(graph)
+    finally:
+        lib.ramses_tune(5, 0)
+    ra.configure()
+
+
+def _adaptive_steps(graph):
     from amr_cases import reference_amr_step
     A, s, tree = _refine_setup(0.12, 0.25)
     s.graph = graph
@@ -275,7 +287,6 @@ def test_amr_adaptive_steps_vs_reference(gpu, graph):
             if s.noct(l):
                 assert got[l] == dt[l], (k, l)
     assert counts[0] != counts[-1] or counts[0][6] > 0
-    ra.configure()
 
 
 def test_adaptive_mesh_golden(gpu):
