@@ -118,3 +118,20 @@ def test_library_binds_after_torch_runtime():
             "ramses_amd.load_library()\n"
             "assert 'torch' in sys.modules\n")
     subprocess.run([sys.executable, "-c", code], cwd=root, check=True, timeout=300)
+
+
+def test_tuning_knobs_validate_and_read_back():
+    """ramses_tune refuses out-of-range launch shapes with a message and
+    ramses_tune_get reads back what was set (0 restores the default)."""
+    lib = ramses_amd.load_library()
+    lib.ramses_tune.argtypes = [ctypes.c_int, ctypes.c_int]
+    lib.ramses_tune_get.argtypes = [ctypes.c_int]
+    bad = [(1, 100), (1, 832), (2, 5), (3, 2), (4, 16), (4, 9000), (5, -2), (99, 0)]
+    for knob, value in bad:
+        assert lib.ramses_tune(knob, value) < 0, (knob, value)
+        assert len(lib.ramses_last_error()) > 0
+    assert lib.ramses_tune(5, 4096) == 0 and lib.ramses_tune_get(5) == 4096
+    assert lib.ramses_tune(5, -1) == 0 and lib.ramses_tune_get(5) == 0  # never
+    assert lib.ramses_tune(5, 0) == 0 and lib.ramses_tune_get(5) == 8192  # default
+    assert lib.ramses_tune(4, 512) == 0 and lib.ramses_tune_get(4) == 512
+    assert lib.ramses_tune(4, 0) == 0 and lib.ramses_tune_get(4) == 1024
