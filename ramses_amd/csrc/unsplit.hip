@@ -17,6 +17,8 @@
 // the oct depends on (the 6^3 corners) are never read.  Because every value
 // is a deterministic function of its inputs, computing it once per row
 // instead of once per 6^3 sweep gives the reference's values bit for bit.
+#include <type_traits>
+
 #include "hydro_device.hpp"
 #include "unsplit_args.hpp"
 #include "unsplit_device.hpp"
@@ -41,19 +43,29 @@ __device__ __forceinline__ unsigned xcd_contiguous(unsigned b, unsigned G) {
   return x * q + (x < r ? x : r) + b / X;
 }
 
-// One thread per (group, row, lane) task.
-template <typename R, int RS, bool S27>
+// One thread per (group, row, lane) task (SH: partner rows share
+// primitives across the wavefront, nvector 32 only).
+template <typename R, int RS, bool S27, bool SH>
 __global__ __launch_bounds__(768) void unsplit_rows(UnsplitArgs a) {
   const long t = (long)xcd_contiguous(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= a.ngroups * 12L * a.nv) return;
-  unsplit_task<R, RS, S27>(a, t, a.ngroups, a.ngrid_last);
+  unsplit_task<R, RS, S27, false, SH>(a, t, a.ngroups, a.ngrid_last);
 }
 
 }  // namespace
 
+// Partner sharing is used in FP32 only: there it saves 4% (1.38 -> 1.33 ms
+// on 2^20 octs); in FP64 the ten 32-bit lane permutes per position sit on
+// the critical path and it measured no faster.
 template <typename R, int RS, bool S27>
 static void launch_t(const UnsplitArgs& a, long grid, int block, hipStream_t stream) {
-  hipLaunchKernelGGL((unsplit_rows<R, RS, S27>), dim3((unsigned)grid), dim3(block), 0, stream, a);
+  if constexpr (!S27 && std::is_same<R, float>::value) {
+    if (a.nv == 32) {
+      hipLaunchKernelGGL((unsplit_rows<R, RS, false, true>), dim3((unsigned)grid), dim3(block), 0, stream, a);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((unsplit_rows<R, RS, S27, false>), dim3((unsigned)grid), dim3(block), 0, stream, a);
 }
 
 template <typename R>
