@@ -108,11 +108,22 @@ __device__ __forceinline__ void store_face(double* flux, double* tmp, size_t fba
   tmp[tbase + fvs] = scale_flux(g.f[5], fs);
 }
 
-// Primitives of the lane 32 away in the same wavefront.
+// Value of the lane 32 away in the same wavefront: v_permlane32_swap of x
+// with itself leaves lanes 0-31's upper partners in the second result and
+// lanes 32-63's lower partners in the first (a VALU swap, no LDS crossbar).
+__device__ __forceinline__ unsigned partner_u32(unsigned x) {
+  const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+  return (threadIdx.x & 32) ? r[0] : r[1];
+}
+__device__ __forceinline__ float partner_val(float x) { return __uint_as_float(partner_u32(__float_as_uint(x))); }
+__device__ __forceinline__ double partner_val(double x) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = partner_u32((unsigned)b), hi = partner_u32((unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 template <typename R>
 __device__ __forceinline__ Prim<R> partner_prim(const Prim<R>& q) {
-  return Prim<R>{__shfl_xor(q.r, 32, 64), __shfl_xor(q.u, 32, 64), __shfl_xor(q.v, 32, 64),
-                 __shfl_xor(q.w, 32, 64), __shfl_xor(q.p, 32, 64)};
+  return Prim<R>{partner_val(q.r), partner_val(q.u), partner_val(q.v), partner_val(q.w), partner_val(q.p)};
 }
 
 // One row of faces along direction D of one oct (see the file comment).
