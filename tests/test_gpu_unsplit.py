@@ -114,3 +114,32 @@ def test_cmpdt_golden(gpu, golden_cmpdt):
         dt = ra.cmpdt(uu, d[f"c{ci}_gg"].copy(), c["dx"], c["ncell"])
         assert dt == float(d[f"c{ci}_dt"]), c
     ra.configure()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_unsplit_nvector_layouts_agree(gpu, precision):
+    """The same octs laid out in batches of 32 and of 16 give the same bits.
+    With 32 the FP32-mixed kernel takes the inner transverse neighbour's
+    primitives from the partner row 32 lanes away; with 16 it evaluates
+    them itself, so this pins the two forms against each other."""
+    import torch
+    rng = np.random.default_rng(77)
+    G = 8
+    u32 = random_stencils(rng, G, nv=32)
+    u16 = u32.reshape(G, 5, 6, 6, 6, 2, 16).transpose(0, 5, 1, 2, 3, 4, 6).reshape(2 * G, 5, 6, 6, 6, 16)
+    dx, dt = 1 / 128, 2e-4
+    out = {}
+    for nv, u in ((32, u32), (16, u16)):
+        ra.configure(gamma=5 / 3, riemann="hllc", slope_type=1, precision=precision, nvector=nv)
+        f = torch.zeros((u.shape[0],) + ra.flux_shape(nv), dtype=torch.float64, device="cuda")
+        t = torch.zeros((u.shape[0],) + ra.tmp_shape(nv), dtype=torch.float64, device="cuda")
+        ra.unsplit_device(torch.from_numpy(np.ascontiguousarray(u)).cuda(), None, f, t, dx, dt)
+        torch.cuda.synchronize()
+        out[nv] = (f.cpu().numpy(), t.cpu().numpy())
+    ra.configure()
+    for k in range(2):  # flux, tmp (undefined slots stay zero in both)
+        a = out[32][k]
+        b = out[16][k]
+        b = np.moveaxis(b.reshape((G, 2) + b.shape[1:]), 1, -2).reshape(a.shape)
+        assert np.abs(a).sum() > 0
+        np.testing.assert_array_equal(a, b)
