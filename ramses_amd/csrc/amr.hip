@@ -250,21 +250,36 @@ __device__ __forceinline__ void reflux_task(const AmrLevelArgs& a, long tix) {
   const int nv = a.nv;
   const size_t fvs = (size_t)nv * 27;
   const double oneontwotondim = 1.0 / 8.0;
-  double s = a.unew[(long)v * a.ustride + c];
-  for (int e = 0; e < ne; ++e) {
+  // All contributions are loaded first (independent loads, issued together),
+  // then summed in the reference's order: contributor e by key, and within
+  // it the 4 faces of the oct side in (k3, j3, i3) loop order.
+  double fv[6][4];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) {
+    if (e >= ne) break;
     const int d = (keys[e] % 6) >> 1, side = keys[e] & 1;
     const int b = posn[e] / nv, lane = posn[e] % nv;
     const double* flux = a.flux + (size_t)b * fvs * 15 + lane + (size_t)27 * (v + 5 * d) * nv;
     const int i0 = d == 0, j0 = d == 1, k0 = d == 2;
-    if (side == 0) {
-      for (int k3 = 1; k3 <= 2 - k0; ++k3)
-        for (int j3 = 1; j3 <= 2 - j0; ++j3)
-          for (int i3 = 1; i3 <= 2 - i0; ++i3) s = s - flux[fslot(i3, j3, k3) * nv] * oneontwotondim;
-    } else {
-      for (int k3 = 1 + k0; k3 <= 2; ++k3)
-        for (int j3 = 1 + j0; j3 <= 2; ++j3)
-          for (int i3 = 1 + i0; i3 <= 2; ++i3) s = s + flux[fslot(i3 + i0, j3 + j0, k3 + k0) * nv] * oneontwotondim;
+    // the side's 2x2 faces: the transverse coordinates run over {1,2} and the
+    // normal one is 1 (side 0, the oct's minus face) or 3 (side 1)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int t1 = 1 + (q & 1), t2 = 1 + (q >> 1);  // inner loop first
+      int i3, j3, k3;
+      if (d == 0) { i3 = side ? 3 : 1; j3 = t1; k3 = t2; }
+      else if (d == 1) { i3 = t1; j3 = side ? 3 : 1; k3 = t2; }
+      else { i3 = t1; j3 = t2; k3 = side ? 3 : 1; }
+      fv[e][q] = flux[fslot(i3, j3, k3) * nv];
     }
+  }
+  double s = a.unew[(long)v * a.ustride + c];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) {
+    if (e >= ne) break;
+    const int side = keys[e] & 1;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) s = side ? s + fv[e][q] * oneontwotondim : s - fv[e][q] * oneontwotondim;
   }
   a.unew[(long)v * a.ustride + c] = s;
 }
