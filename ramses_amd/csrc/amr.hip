@@ -260,7 +260,6 @@ __device__ __forceinline__ void reflux_task(const AmrLevelArgs& a, long tix) {
     const int d = (keys[e] % 6) >> 1, side = keys[e] & 1;
     const int b = posn[e] / nv, lane = posn[e] % nv;
     const double* flux = a.flux + (size_t)b * fvs * 15 + lane + (size_t)27 * (v + 5 * d) * nv;
-    const int i0 = d == 0, j0 = d == 1, k0 = d == 2;
     // the side's 2x2 faces: the transverse coordinates run over {1,2} and the
     // normal one is 1 (side 0, the oct's minus face) or 3 (side 1)
 #pragma unroll

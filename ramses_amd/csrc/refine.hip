@@ -136,16 +136,26 @@ __device__ void flag_hydro_cell(const int* active, int noct, const AmrTreeView& 
   int g, ind;
   long c;
   level_cell(active, noct, t, i, g, ind, c);
-  bool ok = false;
+  // The six neighbours' states are loaded before any test (independent
+  // loads, issued together); the tests have no side effects, so OR-ing all
+  // of them gives the flag of the reference's short-circuit order.
   const Prim3 m = refine_prim(u, us, c, hp);
+  long cn[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const long cc = nbor_cell(t, g, ind, k + 1);
+    cn[k] = cc != 0 ? cc : nbor_of(t, g, k + 1);  // missing neighbour: the father's neighbour
+  }
+  Prim3 nq[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) nq[k] = refine_prim(u, us, cn[k], hp);
+  bool ok = false;
+#pragma unroll
   for (int idim = 0; idim < 3; ++idim) {
-    long cl = nbor_cell(t, g, ind, 2 * idim + 1);
-    long cr = nbor_cell(t, g, ind, 2 * idim + 2);
-    if (cl == 0) cl = nbor_of(t, g, 2 * idim + 1);  // missing neighbour: the father's neighbour
-    if (cr == 0) cr = nbor_of(t, g, 2 * idim + 2);
-    const Prim3 l = refine_prim(u, us, cl, hp), r = refine_prim(u, us, cr, hp);
-    if (rp.err_grad_d >= 0.0) ok = ok || grad_err(l.d, m.d, r.d, rp.floor_d) > rp.err_grad_d;
-    if (rp.err_grad_p >= 0.0) ok = ok || grad_err(l.p, m.p, r.p, rp.floor_p) > rp.err_grad_p;
+    const Prim3& l = nq[2 * idim];
+    const Prim3& r = nq[2 * idim + 1];
+    if (rp.err_grad_d >= 0.0) ok = ok | (grad_err(l.d, m.d, r.d, rp.floor_d) > rp.err_grad_d);
+    if (rp.err_grad_p >= 0.0) ok = ok | (grad_err(l.p, m.p, r.p, rp.floor_p) > rp.err_grad_p);
     if (rp.err_grad_u >= 0.0) {
       const double fu2 = rp.floor_u * rp.floor_u;
       const double cg = __builtin_sqrt(fmax_f(hp.gamma * l.p / l.d, fu2));
@@ -155,7 +165,7 @@ __device__ void flag_hydro_cell(const int* active, int noct, const AmrTreeView& 
         const double vg = l.v[d], vm = m.v[d], vd = r.v[d];
         const double e1 = __builtin_fabs((vd - vm) / ((((cd + cm) + __builtin_fabs(vd)) + __builtin_fabs(vm)) + rp.floor_u));
         const double e2 = __builtin_fabs((vm - vg) / ((((cm + cg) + __builtin_fabs(vm)) + __builtin_fabs(vg)) + rp.floor_u));
-        ok = ok || 2.0 * fmax_f(e1, e2) > rp.err_grad_u;
+        ok = ok | (2.0 * fmax_f(e1, e2) > rp.err_grad_u);
       }
     }
   }
