@@ -74,6 +74,16 @@ __device__ inline void nbor_father7(const AmrTreeView& t, long f, int ilevel, lo
   const int pos = (int)(r / (unsigned)t.ngridmax) + 1;
   const int g = (int)(r - (unsigned)(pos - 1) * (unsigned)t.ngridmax) + 1;
   const int p[3] = {(pos - 1) & 1, ((pos - 1) >> 1) & 1, (pos - 1) >> 2};
+  // The six neighbour links and their sons are loaded unconditionally (in
+  // range for any grid g: nbor holds cells, and son[0] exists), so the
+  // twelve loads issue as two independent rounds instead of a chain per
+  // direction behind a branch.
+  int nbc[6], nbs[6];
+#pragma unroll
+  for (int in = 1; in <= 6; ++in) nbc[in - 1] = nbor_of(t, g, in);
+#pragma unroll
+  for (int in = 1; in <= 6; ++in) nbs[in - 1] = t.son[nbc[in - 1]];
+#pragma unroll
   for (int in = 1; in <= 6; ++in) {
     const int a = (in - 1) >> 1, plus = (in - 1) & 1;
     // getnborcells: the neighbour stays in the grid when it moves inward
@@ -81,8 +91,8 @@ __device__ inline void nbor_father7(const AmrTreeView& t, long f, int ilevel, lo
     int q[3] = {p[0], p[1], p[2]};
     q[a] = 1 - p[a];
     const int ind = 1 + q[0] + 2 * q[1] + 4 * q[2];
-    const int gn = inside ? g : t.son[nbor_of(t, g, in)];
-    out[in] = gn > 0 ? cell_of(t, ind, gn) : nbor_of(t, g, in);
+    const int gn = inside ? g : nbs[in - 1];
+    out[in] = gn > 0 ? cell_of(t, ind, gn) : nbc[in - 1];
   }
 }
 
