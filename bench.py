@@ -447,8 +447,8 @@ def main():
                     help="run only a device copy of this many MiB (for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only; multi-GPU sharding of the path is "
-              "design-only (see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py: this build measures one GPU only; sharding the path over several GPUs is "
+              "designed but not built. Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import torch
