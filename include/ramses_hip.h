@@ -123,8 +123,11 @@ int ramses_uniform_dt_init(double *d_dt, double value, void *stream);
  * Continuous batching of independent levels (serving).  Each slot is one
  * periodic uniform level advanced by the per-level step of amr_step
  * (amr/amr_step.f90:327-424: newdt_fine/courant_fine, godunov_fine,
- * set_uold) with its own dt, capped at its end time as amr_step caps dt at
- * the next output time.  Slots of different sizes share one launch.
+ * set_uold) with its own courant dt.  The last step of a slot is shortened
+ * to end exactly at the slot's end time; the reference has no such cap
+ * (pm/newdt_fine.f90:51-173 ignores the output times, amr/amr_step.f90:151
+ * writes an output after the step that crosses one), so this is the
+ * server's own contract.  Slots of different sizes share one launch.
  * --------------------------------------------------------------------- */
 #define RAMSES_MAX_SLOTS 32
 /* Slot state: 8 device doubles, read by the host to follow progress. */

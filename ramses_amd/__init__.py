@@ -162,9 +162,35 @@ def tmp_shape(nv=32):
     return (NDIM, 2, 3, 3, 3, nv)
 
 
+def _configured_nvector():
+    return _config_cache.get("nvector", 32)
+
+
+def _check_shapes(arrays, ng, nv):
+    """Raise ValueError unless every array has the unsplit layout for `ng`
+    batches (ng None: one call, no batch axis) of the configured nvector.
+    The C entry points take no sizes beyond ngrid: they address every array
+    with the configured nvector, so a smaller array would be overrun."""
+    if nv != _configured_nvector():
+        raise ValueError(f"arrays have nvector={nv} but the library is configured with "
+                         f"nvector={_configured_nvector()}")
+    lead = () if ng is None else (ng,)
+    for name, a, shape in arrays:
+        if a is None:
+            continue
+        if tuple(a.shape) != lead + shape(nv):
+            raise ValueError(f"{name} has shape {tuple(a.shape)}, expected {lead + shape(nv)}")
+        dt = getattr(a, "dtype", None)
+        if str(dt) not in ("float64", "torch.float64"):
+            raise TypeError(f"{name} must be float64, got {dt}")
+
+
 def unsplit(uin, gravin, flux, tmp, dx, dy, dz, dt, ngrid):
     """hydro/umuscl.f90:22 on host numpy arrays (flux/tmp updated in place)."""
     lib = load_library()
+    nv = uin.shape[-1]
+    _check_shapes([("uin", uin, stencil_shape), ("gravin", gravin, grav_shape), ("flux", flux, flux_shape),
+                   ("tmp", tmp, tmp_shape)], None, nv)
     _check(lib.ramses_unsplit(_ptr(uin), _ptr(gravin), _ptr(flux), _ptr(tmp), dx, dy, dz, dt, ngrid),
            "ramses_unsplit")
 
@@ -172,6 +198,11 @@ def unsplit(uin, gravin, flux, tmp, dx, dy, dz, dt, ngrid):
 def unsplit_fortran(uin, gravin, pin, flux, tmp, dx, dy, dz, dt, ngrid):
     """Calls the link-level symbol ``unsplit_`` exactly as Fortran would."""
     lib = load_library()
+    nv = uin.shape[-1]
+    _check_shapes([("uin", uin, stencil_shape), ("gravin", gravin, grav_shape), ("flux", flux, flux_shape),
+                   ("tmp", tmp, tmp_shape)], None, nv)
+    if pin is not None and tuple(pin.shape) != (6, 6, 6, nv):
+        raise ValueError(f"pin has shape {tuple(pin.shape)}, expected {(6, 6, 6, nv)}")
     d = [ctypes.c_double(v) for v in (dx, dy, dz, dt)]
     lib.unsplit_(_ptr(uin), _ptr(gravin), _ptr(pin), _ptr(flux), _ptr(tmp), ctypes.byref(d[0]),
                  ctypes.byref(d[1]), ctypes.byref(d[2]), ctypes.byref(d[3]), ctypes.byref(ctypes.c_int(ngrid)))
@@ -181,6 +212,8 @@ def unsplit_device(uin, gravin, flux, tmp, dx, dt, ngrid_last=None, dy=None, dz=
     """Batched unsplit over device tensors uin[g][5][6][6][6][nv] etc."""
     lib = load_library()
     ng, nv = uin.shape[0], uin.shape[-1]
+    _check_shapes([("uin", uin, stencil_shape), ("gravin", gravin, grav_shape), ("flux", flux, flux_shape),
+                   ("tmp", tmp, tmp_shape)], ng, nv)
     rc = lib.ramses_unsplit_device(_dev(uin), _dev(gravin), _dev(flux), _dev(tmp), dx,
                                    dx if dy is None else dy, dx if dz is None else dz, dt, ng,
                                    nv if ngrid_last is None else ngrid_last, _stream_handle(stream))

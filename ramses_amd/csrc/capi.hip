@@ -443,6 +443,11 @@ int ramses_amr_refine_fine(const ramses_amr_tree* t, const ramses_amr_level* lv,
     return fail(-1, "null argument");
   if (lv->noct < 0 || lv->noct > t->ngridmax || *noct_fine < 0 || *noct_fine > t->ngridmax)
     return fail(-1, "noct out of range");
+  // The creation and kill order (hence grid numbering, the free list and the
+  // next level's list order) follows refine_fine's loop over nvector-sized
+  // batches (amr/refine_utils.f90:395-415, 492-511); the refine kernels reproduce it
+  // for nvector = 32, the reference build's value (bin/Makefile:11).
+  if (c.nvector != 32) return fail(-8, "refine_fine requires nvector = 32 (the batch order of its grid numbering)");
   if (fl->head < 0 || fl->head >= t->ngridmax || fl->numbf < 0 || fl->numbf > t->ngridmax)
     return fail(-1, "bad free list");
   if ((unsigned long)ramses::refine_work_bytes(lv->noct, *noct_fine) > work_bytes || !work)
@@ -844,6 +849,8 @@ int ramses_amr_coarse_step(ramses_amr_mesh* m, void* stream) {
     if (m->nsubcycle[l] != 1 && m->nsubcycle[l] != 2) return fail(-2, "nsubcycle must be 1 or 2");
   }
   if (!(m->boxlen > 0)) return fail(-1, "boxlen must be > 0");
+  if (m->adaptive && m->levelmin < m->nlevelmax && current().nvector != 32)
+    return fail(-8, "adaptive coarse steps require nvector = 32 (the batch order of refine_fine's grid numbering)");
   if (m->adaptive && m->levelmin < m->nlevelmax) {
     const ramses_amr_free& f = m->free;
     if (!f.ring) return fail(-1, "null free list");

@@ -328,10 +328,15 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void batch_godunov_kernel(BatchArgs a
 
 // Per-slot bookkeeping before each batched step (one thread per slot):
 // finish the previous step (t += dt, flip the buffer parity), stop at t_end,
-// and choose this step's dt = min(courant dt, t_end - t) as the reference
-// caps dt at the next output time; then reset the fused courant accumulator
-// to newdt_fine's start value (pm/newdt_fine.f90:51) capped by cmpdt's
-// C*dx/smallc (hydro/godunov_utils.f90:114).
+// and choose this step's dt = min(courant dt, t_end - t); then reset the
+// fused courant accumulator to newdt_fine's start value
+// (pm/newdt_fine.f90:51) capped by cmpdt's C*dx/smallc
+// (hydro/godunov_utils.f90:114).  The end-time cap is this server's own
+// contract (a request asks for the state at t_end) and has no counterpart in
+// the reference: newdt_fine (pm/newdt_fine.f90:51-173) never looks at the
+// output times, and amr_step writes an output once t >= tout
+// (amr/amr_step.f90:151), after the step that crossed it.  Every other step
+// uses the reference's courant dt unchanged.
 __global__ void batch_advance_kernel(BatchArgs a) {
   const int s = threadIdx.x;
   if (s >= a.nslots) return;

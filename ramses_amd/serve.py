@@ -2,7 +2,9 @@
 
 Each request is a periodic uniform level (side n, a multiple of 16) advanced
 from t=0 to its end time by the reference's per-level step
-(amr/amr_step.f90:327-424), each with its own courant dt.  Requests that
+(amr/amr_step.f90:327-424), each with its own courant dt; the last step is
+shortened to end exactly at the request's end time, a cap the reference does
+not have (its newdt_fine ignores output times).  Requests that
 arrive while others run are admitted as soon as a slot and enough HBM are
 free; every scheduler tick then advances *all* in-flight requests with one
 bookkeeping launch and one update launch (``ramses_level_batch_step``), so

@@ -13,6 +13,23 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X GPU and the built HIP library")
+    config.addinivalue_line("markers", "needs_ref: compares with the reference build oracle/_ref "
+                            "(fails in a GPU test when it is missing, skips in a CPU test)")
+
+
+def pytest_runtest_setup(item):
+    """A GPU parity test that needs the reference build fails when the build
+    is missing (a silent skip would hide a whole class of parity checks); a
+    CPU test skips, so the suite still runs where flang is absent."""
+    if item.get_closest_marker("needs_ref") is None:
+        return
+    import oracle
+    if oracle.reference_available():
+        return
+    msg = "reference build oracle/_ref/libramses_ref.so is missing (make -C oracle ref)"
+    if item.get_closest_marker("gpu") is not None:
+        pytest.fail(msg, pytrace=False)
+    pytest.skip(msg)
 
 
 @pytest.fixture(scope="session")

@@ -135,3 +135,65 @@ def test_tuning_knobs_validate_and_read_back():
     assert lib.ramses_tune(5, 0) == 0 and lib.ramses_tune_get(5) == 8192  # default
     assert lib.ramses_tune(4, 512) == 0 and lib.ramses_tune_get(4) == 512
     assert lib.ramses_tune(4, 0) == 0 and lib.ramses_tune_get(4) == 1024
+
+
+def test_unsplit_wrappers_reject_mismatched_shapes():
+    """The C entry points take no array sizes: they address uin, gravin,
+    flux and tmp with the configured nvector.  The Python wrappers therefore
+    refuse, before any call into the library, arrays whose nvector differs
+    from the configuration, flux/tmp/gravin whose batch count or shape
+    differs from uin, and non-float64 arrays."""
+    import numpy as np
+    ra = ramses_amd
+    ra.configure(gamma=5 / 3, riemann="hllc", slope_type=1)  # nvector 32
+    uin = np.zeros(ra.stencil_shape(32))
+    flux = np.zeros(ra.flux_shape(32))
+    tmp = np.zeros(ra.tmp_shape(32))
+    with pytest.raises(ValueError, match="nvector"):
+        ra.unsplit(np.zeros(ra.stencil_shape(16)), None, np.zeros(ra.flux_shape(16)),
+                   np.zeros(ra.tmp_shape(16)), 0.1, 0.1, 0.1, 1e-3, 16)
+    with pytest.raises(ValueError, match="flux"):
+        ra.unsplit(uin, None, np.zeros(ra.flux_shape(32))[:2], tmp, 0.1, 0.1, 0.1, 1e-3, 32)
+    with pytest.raises(ValueError, match="tmp"):
+        ra.unsplit(uin, None, flux, np.zeros((3, 2, 3, 3, 2, 32)), 0.1, 0.1, 0.1, 1e-3, 32)
+    with pytest.raises(ValueError, match="gravin"):
+        ra.unsplit(uin, np.zeros((2, 6, 6, 6, 32)), flux, tmp, 0.1, 0.1, 0.1, 1e-3, 32)
+    with pytest.raises(TypeError, match="float64"):
+        ra.unsplit(uin.astype(np.float32), None, flux, tmp, 0.1, 0.1, 0.1, 1e-3, 32)
+    with pytest.raises(ValueError, match="pin"):
+        ra.unsplit_fortran(uin, None, np.zeros((6, 6, 6, 16)), flux, tmp, 0.1, 0.1, 0.1, 1e-3, 32)
+    # batched device form: the same checks on the batch axis (numpy stand-ins
+    # are enough, the check runs before any device pointer is taken)
+    g_u = np.zeros((4,) + ra.stencil_shape(32))
+    with pytest.raises(ValueError, match="flux"):
+        ra.unsplit_device(g_u, None, np.zeros((3,) + ra.flux_shape(32)), np.zeros((4,) + ra.tmp_shape(32)),
+                          0.1, 1e-3)
+    with pytest.raises(ValueError, match="tmp"):
+        ra.unsplit_device(g_u, None, np.zeros((4,) + ra.flux_shape(32)), np.zeros((2,) + ra.tmp_shape(32)),
+                          0.1, 1e-3)
+    with pytest.raises(ValueError, match="nvector"):
+        ra.unsplit_device(np.zeros((4,) + ra.stencil_shape(16)), None, np.zeros((4,) + ra.flux_shape(16)),
+                          np.zeros((4,) + ra.tmp_shape(16)), 0.1, 1e-3)
+    ra.configure()
+
+
+def test_refinement_refuses_nvector_other_than_32():
+    """refine_fine numbers new grids in the order of its nvector-sized
+    batches; the device refinement reproduces the reference build's
+    nvector = 32 only, so another nvector is refused up front (host-side
+    check: the pointers are placeholders and nothing is launched)."""
+    from ramses_amd import amr
+    lib = ramses_amd.load_library()
+    amr._bind(lib)
+    fake = ctypes.c_void_p(0x1000)
+    tree = amr.ramses_amr_tree(fake, fake, fake, 1, 1000, 1, 1, 1)
+    lv = amr.ramses_amr_level(3, 0.125, fake, 8, fake, None, 0)
+    fl = amr.ramses_amr_free(fake, 0, 10, 990)
+    nfine = ctypes.c_int(0)
+    try:
+        ramses_amd.configure(nvector=16)
+        rc = lib.ramses_amr_refine_fine(ctypes.byref(tree), ctypes.byref(lv), fake, ctypes.byref(nfine), fake,
+                                        ctypes.byref(fl), fake, fake, None, fake, fake, fake, 1 << 20, fake, None)
+        assert rc == -8 and "nvector = 32" in lib.ramses_last_error().decode()
+    finally:
+        ramses_amd.configure()

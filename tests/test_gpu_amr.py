@@ -13,7 +13,7 @@ from amr_cases import smooth_state, sphere_tree
 pytestmark = pytest.mark.gpu
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden", "amr_sphere.npz")
-needs_ref = pytest.mark.skipif(not oracle.reference_available(), reason="reference build (oracle/_ref) not present")
+needs_ref = pytest.mark.needs_ref
 
 
 def _gpu_sweep(tree, u, l, dt, levelmin=3):

@@ -10,7 +10,7 @@ import oracle
 from ramses_amd import namelist
 
 HERE = os.path.dirname(__file__)
-needs_ref = pytest.mark.skipif(not oracle.reference_available(), reason="reference build (oracle/_ref) not present")
+needs_ref = pytest.mark.needs_ref
 
 
 def test_parse_sedov3d():

@@ -65,7 +65,7 @@ def test_oracle_sedov16_golden(oracle_lib, golden_sedov16):
             assert np.array_equal(u, g[f"u_step{s}"])
 
 
-@pytest.mark.skipif(not oracle.reference_available(), reason="reference build (oracle/_ref) not present")
+@pytest.mark.needs_ref
 @pytest.mark.parametrize("riemann", ["llf", "hllc", "hll", "acoustic", "exact"])
 @pytest.mark.parametrize("slope", [0, 1, 2, 3, 7, 8])
 def test_oracle_vs_compiled_reference(oracle_lib, riemann, slope):
@@ -194,7 +194,7 @@ def test_cmpdt_closed_form(oracle_lib):
     np.testing.assert_allclose(dt, expect, rtol=1e-13)
 
 
-@pytest.mark.skipif(not oracle.reference_available(), reason="reference build (oracle/_ref) not present")
+@pytest.mark.needs_ref
 def test_refine_fixture_reproduces():
     """The committed adaptive-mesh digests are what the reference's own
     flag_fine / refine_fine / amr_step sequence produces here."""

@@ -13,7 +13,7 @@ import oracle
 from amr_cases import reference_amr_step, smooth_state, sphere_tree
 from ramses_amd import snapshot
 
-needs_ref = pytest.mark.skipif(not oracle.reference_available(), reason="reference build (oracle/_ref) not present")
+needs_ref = pytest.mark.needs_ref
 
 L = 6
 GAMMA = 5 / 3
