@@ -9,7 +9,12 @@
 * ``turbulent(n, mach, ...)`` -- a periodic Gaussian random velocity field
   with power spectrum P(k) ~ k^-4 (|v_hat| ~ k^-2, k=0 removed), scaled to
   the requested 3D rms Mach number for c_s=1, optionally a lognormal density
-  and an isothermal pressure P = rho/gamma.
+  and an isothermal pressure P = rho/gamma.  The default generator draws its
+  white noise from numpy's ``default_rng(seed)`` (PCG64) on the host --
+  velocity x, y, z, then the density -- and filters it with scipy's FFT, so
+  the field can be regenerated with numpy alone; ``generator="device"``
+  draws torch Philox noise on the GPU instead (fast enough to create a
+  request's field inside a serving loop, but only reproducible with torch).
 Both return conserved state u[var][z][y][x] as a float64 tensor.
 * ``region_condinit(x, dx, regions)`` -- the &INIT_PARAMS regions evaluated
   at arbitrary cell centres x (N, 3) with the cell size dx of their level,
@@ -17,6 +22,7 @@ Both return conserved state u[var][z][y][x] as a float64 tensor.
   the conserved state (5, N).  ``sedov_regions`` is the sedov3d.nml case.
 """
 import math
+import os
 
 
 def sedov(n, gamma=1.4, e_total=1.0, device="cuda"):
@@ -33,8 +39,13 @@ def sedov(n, gamma=1.4, e_total=1.0, device="cuda"):
     return u
 
 
-def turbulent(n, mach, gamma=5.0 / 3.0, rho_sigma=0.0, seed=0, device="cuda", slope=-2.0):
+def turbulent(n, mach, gamma=5.0 / 3.0, rho_sigma=0.0, seed=0, device="cuda", slope=-2.0, generator="host"):
     import torch
+    if generator == "host":
+        u = turbulent_host(n, mach, gamma=gamma, rho_sigma=rho_sigma, seed=seed, slope=slope)
+        return torch.from_numpy(u).to(device)
+    if generator != "device":
+        raise ValueError("generator must be 'host' or 'device'")
     gen = torch.Generator(device=device)
     gen.manual_seed(seed)
     k1 = torch.fft.fftfreq(n, d=1.0 / n, device=device, dtype=torch.float64)
@@ -60,6 +71,86 @@ def turbulent(n, mach, gamma=5.0 / 3.0, rho_sigma=0.0, seed=0, device="cuda", sl
     u[4] = p / (gamma - 1.0) + 0.5 * rho * (vel * vel).sum(0)
     del vel
     return u
+
+
+def turbulent_host(n, mach, gamma=5.0 / 3.0, rho_sigma=0.0, seed=0, slope=-2.0, workers=None):
+    """The k^-4 velocity field of ``turbulent`` as a host numpy array (5, n,
+    n, n), from ``numpy.random.default_rng(seed)``: three standard-normal
+    cubes (vx, vy, vz), each filtered in Fourier space by |k|^slope with the
+    k = 0 mode removed, the mean removed, scaled to 3D rms Mach `mach` for
+    c_s = 1; then, when rho_sigma > 0, rho = exp(rho_sigma * N(0,1)) per
+    cell from the same stream, else rho = 1; P = rho / gamma."""
+    import numpy as np
+    import scipy.fft as sfft
+    if workers is None:
+        workers = min(16, os.cpu_count() or 1)
+    rng = np.random.default_rng(seed)
+    k1 = np.fft.fftfreq(n, d=1.0 / n)
+    kr = np.fft.rfftfreq(n, d=1.0 / n)
+    kk = np.sqrt(k1[:, None, None] ** 2 + k1[None, :, None] ** 2 + kr[None, None, :] ** 2)
+    amp = np.zeros_like(kk)
+    np.power(kk, slope, out=amp, where=kk > 0)
+    vel = np.empty((3, n, n, n))
+    for d in range(3):
+        white = rng.standard_normal((n, n, n))
+        spec = sfft.rfftn(white, workers=workers)
+        del white
+        spec *= amp
+        vel[d] = sfft.irfftn(spec, s=(n, n, n), workers=workers)
+        del spec
+    vel -= vel.mean(axis=(1, 2, 3), keepdims=True)
+    rms = np.sqrt((vel * vel).sum(0).mean())
+    vel *= mach / rms  # c_s = 1
+    if rho_sigma > 0:
+        rho = np.exp(rho_sigma * rng.standard_normal((n, n, n)))
+    else:
+        rho = np.ones((n, n, n))
+    u = np.empty((5, n, n, n))
+    u[0] = rho
+    for d in range(3):
+        u[1 + d] = rho * vel[d]
+    ek = vel[0] * vel[0]
+    ek += vel[1] * vel[1]
+    ek += vel[2] * vel[2]
+    del vel
+    ek *= 0.5 * rho
+    u[4] = rho / gamma / (gamma - 1.0) + ek
+    return u
+
+
+def oct_batches(noct, nv=32, seed=1, gamma=5.0 / 3.0, dx=1.0 / 128, cfl=0.5, device="cuda", chunk=2048):
+    """Synthetic gathered oct batches for the batched unsplit entry point:
+    uin[g][5][6][6][6][nv] (the reference's uin layout per batch, axes
+    reversed) for noct // nv batches, every stencil cell drawn i.i.d. as
+    rho ~ logN(0, 1), u, v, w ~ N(0, 1), P ~ logN(-0.5, 0.5) and converted
+    to conserved variables; and dt = the smallest cmpdt time step at CFL
+    `cfl` without gravity (hydro/godunov_utils.f90:5-120) over the 8 centre
+    cells of every oct.  The noise is torch Philox drawn on the device in
+    chunks of `chunk` batches (2^20 octs are 9 GB of stencils, too many to
+    draw on the host and upload in a benchmark's set-up)."""
+    import torch
+    G = noct // nv
+    uin = torch.empty((G, 5, 6, 6, 6, nv), dtype=torch.float64, device=device)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+    dtmin = float("inf")
+    for g0 in range(0, G, chunk):
+        g1 = min(G, g0 + chunk)
+        shp = (g1 - g0, 6, 6, 6, nv)
+        rho = torch.exp(torch.randn(shp, generator=gen, device=device, dtype=torch.float64))
+        v = torch.randn((3,) + shp, generator=gen, device=device, dtype=torch.float64)
+        p = torch.exp(-0.5 + 0.5 * torch.randn(shp, generator=gen, device=device, dtype=torch.float64))
+        u = uin[g0:g1]
+        u[:, 0] = rho
+        u[:, 1:4] = (rho.unsqueeze(0) * v).permute(1, 0, 2, 3, 4, 5)
+        u[:, 4] = p / (gamma - 1.0) + 0.5 * rho * (v * v).sum(0)
+        c = torch.sqrt(gamma * p[:, 1:3, 1:3, 1:3] / rho[:, 1:3, 1:3, 1:3])
+        w = 3.0 * c + v[:, :, 1:3, 1:3, 1:3].abs().sum(0)
+        r = 1e-4
+        dtc = dx / w * ((1.0 + 2.0 * cfl * r) ** 0.5 - 1.0) / r
+        dtmin = min(dtmin, float(dtc.min()))
+        del rho, v, p, c, w, dtc
+    return uin, dtmin
 
 
 def sound_crossing_time(boxlen=1.0, cs=1.0):

@@ -18,6 +18,13 @@ Families
                       CFL 0.8): conserved state after steps 1, 2 and 10 and
                       the dt sequence, composed from the reference unsplit
                       and cmpdt by the uniform-level driver below.
+  sedov64.npz         F3 at the Sedov configuration's own size: 64^3
+                      periodic Sedov (gamma=1.4, hllc, minmod, CFL 0.8),
+                      the initial state, the dt of each of 20 level steps
+                      and the conserved state after steps 1, 2, 3 and 20,
+                      from the reference's own courant_fine, set_unew,
+                      godunov_fine and set_uold on a fully refined level-6
+                      oct tree (oracle/ref_uniform.py).
   amr_sphere.npz      F4: an oct tree of levels 3-5 (tests/amr_cases.py)
                       driven through the reference's own courant_fine,
                       set_unew, godunov_fine and upload_fine
@@ -247,6 +254,24 @@ def sedov16(R):
     print("sedov16.npz: dt =", dts[:3], "...")
 
 
+def sedov64(R):
+    from oracle.ref_uniform import ReferenceUniformLevel
+    n, steps = 64, 20
+    lvl = ReferenceUniformLevel(n, 1.4, riemann="hllc", slope_type=1, courant_factor=0.8, ref=R)
+    u0 = sedov_ic(n)
+    lvl.load(u0)
+    dts, snaps = [], {}
+    for step in range(1, steps + 1):
+        dts.append(lvl.step())
+        if step in (1, 2, 3, steps):
+            snaps[f"u_step{step}"] = lvl.state()
+    lvl.release()
+    np.savez_compressed(os.path.join(HERE, "sedov64.npz"), u0=u0, dt=np.array(dts),
+                        meta=json.dumps(dict(n=n, gamma=1.4, riemann="hllc", slope_type=1,
+                                             courant_factor=0.8, steps=steps)), **snaps)
+    print("sedov64.npz: dt =", dts[:3], "...")
+
+
 def amr_sphere(R):
     """F4: one oct tree (levels 3-5, refined ball + blob), smooth state; for
     each level: the reference's courant_fine dt, then set_unew(l-1),
@@ -329,7 +354,7 @@ def amr_refine(R):
 def main():
     R = oracle.Reference()
     only = sys.argv[1:]
-    for name, fn in [("unsplit", unsplit_cases), ("cmpdt", cmpdt_cases), ("sedov16", sedov16),
+    for name, fn in [("unsplit", unsplit_cases), ("cmpdt", cmpdt_cases), ("sedov16", sedov16), ("sedov64", sedov64),
                      ("amr_sphere", amr_sphere), ("amr_refine", amr_refine)]:
         if not only or name in only:
             fn(R)
