@@ -137,6 +137,12 @@ def unsplit_cpu_baseline(budget_s=12.0):
                       f"same input distribution; {'flang -O2 build of hydro/umuscl.f90' if kind == 'reference' else 'C restatement'}"}
 
 
+def progress(msg):
+    """One line on stderr per finished stage (set-up stages take tens of
+    seconds at 512^3)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def host_cores():
     """Host cores this run may use: the scheduler's affinity set, at most 16
     (the host share of one GPU on the machines this runs on)."""
@@ -232,6 +238,7 @@ def level_bench(torch, ra, args):
         u0 = ics.turbulent(n, w["mach"], gamma=w["gamma"], rho_sigma=w["rho_sigma"], seed=w["seed"])
         data = f"synthetic: numpy default_rng({w['seed']}) noise, k^-4 filtered (scipy FFT), resident in HBM"
     torch.cuda.synchronize()
+    progress(f"initial state {n}^3 resident")
     cells = n ** 3
     out, states = {}, {}
     for prec in ("fp64", "fp32"):
@@ -272,6 +279,7 @@ def level_bench(torch, ra, args):
                                   "frac": ach / HBM_PEAK, "traffic": traffic}}
         states[prec] = lvl.u.clone()
         del lvl
+        progress(f"{prec}: {kavg:.3f} ms per launch, tile {tile} tz {tz}")
     err = {}
     names = ["rho", "mx", "my", "mz", "E"]
     for v in range(5):

@@ -114,6 +114,26 @@ int ramses_uniform_courant(const double *u, int n, double dx, double dt_start, d
 int ramses_uniform_godunov(const double *u, double *unew, int n, double dx, const double *d_dt,
                            double *d_dt_next, void *stream);
 
+/* courant_fine's conservation sums (hydro/courant_fine.f90:95-119) beside
+ * the dt: the *_sums forms of the two calls above also write, for the state
+ * they reduce (the level's state, or the new state of the step), d_sums[0] =
+ * total mass sum rho*vol, d_sums[1] = total energy sum E*vol (the
+ * reference's ekin_tot), d_sums[2] = internal energy sum (E - |m|^2/(2
+ * max(rho,smallr)))*vol, vol = dx^3, into 3 device doubles (overwritten).
+ * The sums are reduced per workgroup and then over workgroups in a fixed
+ * order, so a given launch gives the same bits on every run (the reference's
+ * serial order rounds differently).  `work` is device memory of at least
+ * ramses_uniform_work_bytes(n) bytes; d_sums NULL computes no sums and needs
+ * no work buffer. */
+unsigned long ramses_uniform_work_bytes(int n);
+int ramses_uniform_courant_sums(const double *u, int n, double dx, double dt_start, double *d_dt, double *d_sums,
+                                void *work, unsigned long work_bytes, void *stream);
+/* godfine1 over the level with the fused courant_fine of the new state and
+ * its sums (hydro/godunov_fine.f90:5-35, hydro/courant_fine.f90:95-119). */
+int ramses_uniform_godunov_sums(const double *u, double *unew, int n, double dx, const double *d_dt,
+                                double *d_dt_next, double *d_sums, void *work, unsigned long work_bytes,
+                                void *stream);
+
 /* Writes `value` into one device double with a kernel (graph-capture safe):
  * the per-step reset of the dt accumulator, dtnew(ilevel)=boxlen/smallc in
  * newdt_fine (pm/newdt_fine.f90:51). */
@@ -324,6 +344,13 @@ typedef struct ramses_amr_mesh {
   int graph;
   int graph_capacity;
   void *graph_state; /* library-owned: NULL before the first step */
+  /* Optional (NULL: not computed): 3 device doubles that each coarse step
+   * resets and every courant_fine of the step adds its level's leaf sums to,
+   * as mass_tot, ekin_tot and eint_tot are (amr/adaptive_loop.f90:89-91,
+   * hydro/courant_fine.f90:95-119,152-154): total mass, sum of E*vol and
+   * internal energy, vol = dx^3 of each level.  Deterministic: workgroup
+   * partial sums added in block order. */
+  double *sums;
 } ramses_amr_mesh;
 
 /* amr_step(levelmin, 1) (amr/amr_step.f90) followed by the end of a coarse

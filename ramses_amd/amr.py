@@ -63,7 +63,8 @@ class ramses_amr_mesh(ctypes.Structure):
                 ("nsubcycle", ctypes.c_int * (MAX_LEVELS + 1)), ("adaptive", ctypes.c_int),
                 ("boxlen", ctypes.c_double), ("nstep", ctypes.c_longlong), ("nstep_coarse", ctypes.c_longlong),
                 ("cell_updates", ctypes.c_longlong), ("work", ctypes.c_void_p), ("work_bytes", ctypes.c_ulong),
-                ("graph", ctypes.c_int), ("graph_capacity", ctypes.c_int), ("graph_state", ctypes.c_void_p)]
+                ("graph", ctypes.c_int), ("graph_capacity", ctypes.c_int), ("graph_state", ctypes.c_void_p),
+                ("sums", ctypes.c_void_p)]
 
 
 def _bind(lib):
@@ -412,12 +413,15 @@ class AmrSolver(AmrLevels):
     refine=False the tree is static (the reference's `static` mode).
     """
 
-    def __init__(self, tree, u, nsubcycle=2, refine=False, graph=False, graph_capacity=0, **kw):
+    def __init__(self, tree, u, nsubcycle=2, refine=False, graph=False, graph_capacity=0, sums=False, **kw):
         """nsubcycle: 1 or 2 for every level, or a list per level (index
         0 is level 1), as &RUN_PARAMS nsubcycle (amr/amr_commons.f90:51).
         graph: run each coarse step as the library's captured hipGraph (one
         launch and one synchronisation per step; identical results), with a
-        workspace for graph_capacity octs per level (0: ngridmax)."""
+        workspace for graph_capacity octs per level (0: ngridmax).
+        sums: keep courant_fine's conservation totals of each coarse step
+        in ``self.sums`` (mass_tot, ekin_tot = sum E*vol, eint_tot;
+        hydro/courant_fine.f90:95-119,152-154), on the device."""
         super().__init__(tree, u, **kw)
         self.graph = bool(graph)
         self.graph_capacity = int(graph_capacity)
@@ -435,6 +439,7 @@ class AmrSolver(AmrLevels):
         self.fine_steps = 0        # nstep: finest-level update_time calls
         self.nstep_coarse = 0
         self.cell_updates = 0
+        self.sums = torch.zeros(3, dtype=torch.float64, device=self.device) if sums else None
         self._mesh = ramses_amr_mesh()
 
     @classmethod
@@ -540,6 +545,7 @@ class AmrSolver(AmrLevels):
         m.boxlen = self.boxlen
         m.nstep, m.nstep_coarse, m.cell_updates = self.fine_steps, self.nstep_coarse, self.cell_updates
         m.graph, m.graph_capacity = int(self.graph), self.graph_capacity
+        m.sums = None if self.sums is None else self.sums.data_ptr()
         rc = self.lib.ramses_amr_coarse_step(ctypes.byref(m), self._s())
         for l in range(1, self.nlevelmax + 1):
             self.nocts[l] = m.noct[l]

@@ -24,6 +24,8 @@
 #include "unsplit_device.hpp"
 
 namespace ramses {
+hipError_t launch_cons_finish(const double* partials, long nblocks, double* out, int accumulate, hipStream_t s);
+
 
 namespace {
 
@@ -399,10 +401,12 @@ __global__ void amr_courant(const int* active, int noct_h, const int* dn, AmrTre
 // speed, and the block's dt limit is min-combined into *d_dt (exact: the dt
 // limit is non-increasing in the wave speed).
 __global__ void amr_begin(const int* active, int noct_h, const int* dn, AmrTreeView t, const double* u, double* unew,
-                          long ustride, double dx, HydroParams hp, double* d_dt) {
+                          long ustride, double dx, HydroParams hp, double* d_dt, double* partials) {
   const int noct = level_count(noct_h, dn);
   const long n = 8L * noct, stride = (long)gridDim.x * blockDim.x;
   double w = 0.0;
+  ConsSums cs{0.0, 0.0, 0.0};  // courant_fine's sums over the leaves (partials set)
+  const double vol = dx * dx * dx;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const long c = cell_of(t, (int)(i / noct) + 1, active[i % noct]);
     double q[5];
@@ -410,7 +414,14 @@ __global__ void amr_begin(const int* active, int noct_h, const int* dn, AmrTreeV
     for (int v = 0; v < 5; ++v) q[v] = u[(long)v * ustride + c];
 #pragma unroll
     for (int v = 0; v < 5; ++v) unew[(long)v * ustride + c] = q[v];
-    if (t.son[c] == 0) w = fmax_f(w, cmpdt_w(q[0], q[1], q[2], q[3], q[4], hp));
+    if (t.son[c] == 0) {
+      w = fmax_f(w, cmpdt_w(q[0], q[1], q[2], q[3], q[4], hp));
+      if (partials) cons_add(cs, q[0], q[1], q[2], q[3], q[4], vol, hp.smallr);
+    }
+  }
+  if (partials) {
+    __shared__ double cred[3 * 4];
+    cons_block_store(cs, partials, blockIdx.x, cred);
   }
   for (int off = 32; off > 0; off >>= 1) w = fmax_f(w, __shfl_xor(w, off, 64));
   __shared__ double red[4];
@@ -549,13 +560,23 @@ hipError_t launch_amr_courant(const int* active, int noct, const AmrTreeView& t,
   return hipGetLastError();
 }
 
+// With partials (room for amr_begin_blocks(noct, dn) workgroups) the
+// level's conservation sums are added into sums[0..2] after the pass.
 hipError_t launch_amr_begin(const int* active, int noct, const AmrTreeView& t, const double* u, double* unew,
                             long ustride, double dx, const HydroParams& hp, double* d_dt, hipStream_t s,
-                            const int* dn) {
+                            const int* dn, double* partials, double* sums) {
   if (!dn && noct <= 0) return hipSuccess;
-  hipLaunchKernelGGL(amr_begin, dim3(cell_grid(8L * noct, dn)), dim3(256), 0, s, active, noct, dn, t, u, unew,
-                     ustride, dx, hp, d_dt);
-  return hipGetLastError();
+  const unsigned nb = cell_grid(8L * noct, dn);
+  hipLaunchKernelGGL(amr_begin, dim3(nb), dim3(256), 0, s, active, noct, dn, t, u, unew, ustride, dx, hp, d_dt,
+                     partials);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !partials) return e;
+  return launch_cons_finish(partials, nb, sums, 1, s);
+}
+
+size_t amr_begin_partials_bytes(int noct, bool persistent) {
+  const size_t nb = persistent ? persist_blocks() : (size_t)((8L * noct + 255) / 256);
+  return (nb > 0 ? nb : 1) * 3 * sizeof(double);
 }
 
 hipError_t launch_amr_finish(const int* active, int noct, const AmrTreeView& t, const double* unew, double* u,

@@ -21,9 +21,14 @@ hipError_t launch_unsplit(const UnsplitArgs& a, bool fp32, hipStream_t stream);
 hipError_t launch_cmpdt(const double* uu, const double* gg, int nv, int ncell, double dx,
                         const HydroParams& hp, double* d_dt, hipStream_t stream);
 hipError_t launch_set_double(double* p, double v, hipStream_t s);
-hipError_t launch_courant(const double* u, int n, double dx, const HydroParams& hp, double* d_dt, hipStream_t s);
+hipError_t launch_courant(const double* u, int n, double dx, const HydroParams& hp, double* d_dt, hipStream_t s,
+                          double* partials = nullptr, double* d_sums = nullptr);
+size_t uniform_sums_work_bytes(int n);
+size_t amr_begin_partials_bytes(int noct, bool persistent);
+hipError_t launch_set_doubles(double* p, int n, double v, hipStream_t s);
 hipError_t launch_uniform_godunov(const double* u, double* unew, int n, double dx, const double* d_dt,
-                                  double* d_dt_next, const HydroParams& hp, bool fp32, hipStream_t s);
+                                  double* d_dt_next, const HydroParams& hp, bool fp32, hipStream_t s,
+                                  double* partials = nullptr, double* d_sums = nullptr);
 hipError_t launch_level_batch(double* const* buf0, double* const* buf1, double* const* state, const int* n,
                               const double* dx, int nslots, const HydroParams& hp, bool fp32, hipStream_t s);
 hipError_t launch_init_slot(double* state, double t_end, double start, hipStream_t s);
@@ -41,7 +46,8 @@ hipError_t launch_set_ints(int* p, int n, int v, hipStream_t s);
 hipError_t launch_clear_bits(int* p, int mask, hipStream_t s);
 hipError_t launch_amr_begin(const int* active, int noct, const AmrTreeView& t, const double* u, double* unew,
                             long ustride, double dx, const HydroParams& hp, double* d_dt, hipStream_t s,
-                            const int* dn = nullptr);
+                            const int* dn = nullptr, double* partials = nullptr,
+                            double* sums = nullptr);
 hipError_t launch_amr_finish(const int* active, int noct, const AmrTreeView& t, const double* unew, double* u,
                              long ustride, double smallr, int ivar_mode, hipStream_t s, const int* dn = nullptr);
 struct RefineParams {
@@ -530,8 +536,11 @@ int amr_step_rec(StepCtx& c, int l, int icount) {
   hipError_t e = ramses::launch_amr_dt(0, m->dt, m->dtold, m->time, l, coarser, coarser ? m->nsubcycle[l - 1] : 1,
                                        0, start, c.s);
   if (e != hipSuccess) return hip_fail(e, "dt launch");
-  // courant_fine and set_unew in one pass
-  e = ramses::launch_amr_begin(lv.active, lv.noct, v, m->uold, m->unew, v.ncell + 1, lv.dx, c.cfg.hp, m->dt + l, c.s);
+  // courant_fine and set_unew in one pass (with the conservation sums:
+  // mass_tot = mass_tot + mass_all etc., hydro/courant_fine.f90:152-154)
+  if (m->sums && (rc = ensure_work(m, ramses::amr_begin_partials_bytes(lv.noct, false), c.s))) return rc;
+  e = ramses::launch_amr_begin(lv.active, lv.noct, v, m->uold, m->unew, v.ncell + 1, lv.dx, c.cfg.hp, m->dt + l, c.s,
+                               nullptr, m->sums ? (double*)m->work : nullptr, m->sums);
   if (e != hipSuccess) return hip_fail(e, "courant / set_unew launch");
   if (l < m->nlevelmax && m->noct[l + 1] > 0) {
     if ((rc = amr_step_rec(c, l + 1, 1))) return rc;
@@ -618,7 +627,7 @@ void graph_key(const ramses_amr_mesh* m, int capacity, std::vector<long>& k) {
   }
   for (const void* q : {(const void*)m->pos, (const void*)m->xg, (const void*)m->flag1, (const void*)m->flag2,
                         (const void*)m->free.ring, (const void*)m->uold, (const void*)m->unew, (const void*)m->dt,
-                        (const void*)m->dtold, (const void*)m->time, (const void*)m->error})
+                        (const void*)m->dtold, (const void*)m->time, (const void*)m->error, (const void*)m->sums})
     P(q);
   long bl;
   std::memcpy(&bl, &m->boxlen, sizeof(bl));
@@ -666,7 +675,9 @@ hipError_t gstep(GCtx& g, int l, int icount) {
   e = ramses::launch_amr_dt(0, m->dt, m->dtold, m->time, l, coarser, coarser ? m->nsubcycle[l - 1] : 1, 0, start, g.s,
                             dn, g.dev->cnt);
   if (e != hipSuccess) return e;
-  e = ramses::launch_amr_begin(m->active[l], 0, g.v, m->uold, m->unew, us, dx, g.c.hp, m->dt + l, g.s, dn + l);
+  double* partials = m->sums ? (double*)(((uintptr_t)g.work + 255) & ~(uintptr_t)255) : nullptr;
+  e = ramses::launch_amr_begin(m->active[l], 0, g.v, m->uold, m->unew, us, dx, g.c.hp, m->dt + l, g.s, dn + l,
+                               partials, m->sums);
   if (e != hipSuccess) return e;  // courant_fine + set_unew
   if (l < m->nlevelmax) {
     if ((e = gstep(g, l + 1, 1)) != hipSuccess) return e;
@@ -726,6 +737,8 @@ hipError_t gcoarse(GCtx& g) {
   // the refusal bits (4: no free grid, 8: capacity) describe this step only
   hipError_t e = ramses::launch_clear_bits(m->error, 4 | 8, g.s);
   if (e != hipSuccess) return e;
+  // adaptive_loop resets the totals at every coarse step (amr/adaptive_loop.f90:89-91)
+  if (m->sums && (e = ramses::launch_set_doubles(m->sums, 3, 0.0, g.s)) != hipSuccess) return e;
   e = gstep(g, m->levelmin, 1);
   if (e != hipSuccess) return e;
   if (m->adaptive && m->levelmin < m->nlevelmax) {
@@ -786,6 +799,8 @@ int coarse_step_graph(ramses_amr_mesh* m, hipStream_t s) {
     size_t need = amr_work(capacity, c.nvector);
     const size_t rneed = ramses::refine_work_bytes(capacity, capacity);
     if (rneed > need) need = rneed;
+    const size_t pneed = ramses::amr_begin_partials_bytes(0, true) + 256;
+    if (pneed > need) need = pneed;
     if ((e = hipMalloc(&gs->work, need)) != hipSuccess) return hip_fail(e, "graph workspace");
     gs->work_bytes = need;
     gs->capacity = capacity;
@@ -859,6 +874,11 @@ int ramses_amr_coarse_step(ramses_amr_mesh* m, void* stream) {
   }
   if (m->graph) return coarse_step_graph(m, (hipStream_t)stream);
   StepCtx c{m, (hipStream_t)stream, current()};
+  // adaptive_loop resets the totals at every coarse step (amr/adaptive_loop.f90:89-91)
+  if (m->sums) {
+    hipError_t e = ramses::launch_set_doubles(m->sums, 3, 0.0, c.s);
+    if (e != hipSuccess) return hip_fail(e, "sums reset launch");
+  }
   int rc = amr_step_rec(c, m->levelmin, 1);
   if (rc) return rc;
   m->nstep_coarse += 1;
@@ -990,34 +1010,54 @@ int ramses_uniform_dt_init(double* d_dt, double value, void* stream) {
   return 0;
 }
 
-int ramses_uniform_courant(const double* u, int n, double dx, double dt_start, double* d_dt, void* stream) {
+unsigned long ramses_uniform_work_bytes(int n) {
+  return n < 2 ? 0 : (unsigned long)ramses::uniform_sums_work_bytes(n);
+}
+
+int ramses_uniform_courant_sums(const double* u, int n, double dx, double dt_start, double* d_dt, double* d_sums,
+                                void* work, unsigned long work_bytes, void* stream) {
   const Config c = current();
   if (!u || !d_dt) return fail(-1, "null pointer");
   if (n < 2) return fail(-1, "n must be >= 2");
   if (!(dx > 0)) return fail(-1, "dx must be > 0");
+  if (d_sums && (!work || work_bytes < ramses::uniform_sums_work_bytes(n)))
+    return fail(-1, "work buffer too small (see ramses_uniform_work_bytes)");
   // courant_fine starts from dtnew (dt_start); every cmpdt batch starts from
   // courant_factor*dx/smallc (hydro/courant_fine.f90:35, godunov_utils.f90:114)
   double start = c.hp.courant_factor * dx / c.hp.smallc;
   if (dt_start < start) start = dt_start;
   hipError_t e = ramses::launch_set_double(d_dt, start, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "dt init launch");
-  e = ramses::launch_courant(u, n, dx, c.hp, d_dt, (hipStream_t)stream);
+  e = ramses::launch_courant(u, n, dx, c.hp, d_dt, (hipStream_t)stream, d_sums ? (double*)work : nullptr, d_sums);
   if (e != hipSuccess) return hip_fail(e, "courant launch");
   return 0;
 }
 
-int ramses_uniform_godunov(const double* u, double* unew, int n, double dx, const double* d_dt, double* d_dt_next,
-                           void* stream) {
+int ramses_uniform_courant(const double* u, int n, double dx, double dt_start, double* d_dt, void* stream) {
+  return ramses_uniform_courant_sums(u, n, dx, dt_start, d_dt, nullptr, nullptr, 0, stream);
+}
+
+int ramses_uniform_godunov_sums(const double* u, double* unew, int n, double dx, const double* d_dt,
+                                double* d_dt_next, double* d_sums, void* work, unsigned long work_bytes,
+                                void* stream) {
   const Config c = current();
   if (!u || !unew || !d_dt) return fail(-1, "null pointer");
   if (u == unew) return fail(-1, "unew must not alias u");
   if (n < 8 || n % 8 != 0) return fail(-1, "n must be a multiple of 8 (>= 8)");
   if (!(dx > 0)) return fail(-1, "dx must be > 0");
   if (c.hp.slope_type == 3) return fail(-4, "slope_type 3 is not supported on the resident level path");
+  if (d_sums && (!work || work_bytes < ramses::uniform_sums_work_bytes(n)))
+    return fail(-1, "work buffer too small (see ramses_uniform_work_bytes)");
   hipError_t e = ramses::launch_uniform_godunov(u, unew, n, dx, d_dt, d_dt_next, c.hp,
-                                                c.precision == RAMSES_FP32_MIXED, (hipStream_t)stream);
+                                                c.precision == RAMSES_FP32_MIXED, (hipStream_t)stream,
+                                                d_sums ? (double*)work : nullptr, d_sums);
   if (e != hipSuccess) return hip_fail(e, "uniform godunov launch");
   return 0;
+}
+
+int ramses_uniform_godunov(const double* u, double* unew, int n, double dx, const double* d_dt, double* d_dt_next,
+                           void* stream) {
+  return ramses_uniform_godunov_sums(u, unew, n, dx, d_dt, d_dt_next, nullptr, nullptr, 0, stream);
 }
 
 }  // extern "C"

@@ -668,6 +668,59 @@ __device__ __forceinline__ double cmpdt_from_w(double w, double dx, const HydroP
   return dx / w * (__builtin_sqrt(1.0 + 2.0 * hp.courant_factor * r) - 1.0) / r;
 }
 
+// courant_fine's conservation sums over the leaf cells of a level
+// (hydro/courant_fine.f90:95-119): mass += rho*vol, ekin += E*vol (the
+// reference keeps the total energy under the name ekin), eint += E*vol minus
+// 0.5*m_d^2/max(rho,smallr)*vol for d = x, y, z.  Accumulated per thread,
+// reduced per workgroup in a fixed order (cons_block_store) and summed over
+// workgroups in block order by cons_finish, so the result is the same on
+// every run; its rounding differs from the reference's serial order.
+struct ConsSums {
+  double m, e, ei;
+};
+
+__device__ __forceinline__ void cons_add(ConsSums& s, double u0, double u1, double u2, double u3, double u4,
+                                         double vol, double smallr) {
+  s.m = s.m + u0 * vol;
+  s.e = s.e + u4 * vol;
+  const double r = fmax_f(u0, smallr);
+  double ei = u4 * vol;
+  ei = ei - 0.5 * (u1 * u1) / r * vol;
+  ei = ei - 0.5 * (u2 * u2) / r * vol;
+  ei = ei - 0.5 * (u3 * u3) / r * vol;
+  s.ei = s.ei + ei;
+}
+
+// Workgroup reduction of the sums in a fixed order (xor shuffles inside each
+// wavefront, then the wavefronts in order); thread 0 writes the three sums
+// of workgroup `block` to partials[3*block .. 3*block+2].  `lds` holds 3
+// doubles per wavefront.  Every thread of the workgroup must call it.
+__device__ __forceinline__ void cons_block_store(ConsSums s, double* partials, long block, double* lds) {
+  for (int off = 32; off > 0; off >>= 1) {
+    s.m = s.m + __shfl_xor(s.m, off, 64);
+    s.e = s.e + __shfl_xor(s.e, off, 64);
+    s.ei = s.ei + __shfl_xor(s.ei, off, 64);
+  }
+  const int wave = threadIdx.x >> 6, nw = (int)((blockDim.x + 63) >> 6);
+  if ((threadIdx.x & 63) == 0) {
+    lds[3 * wave] = s.m;
+    lds[3 * wave + 1] = s.e;
+    lds[3 * wave + 2] = s.ei;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = lds[0], e = lds[1], ei = lds[2];
+    for (int w = 1; w < nw; ++w) {
+      m = m + lds[3 * w];
+      e = e + lds[3 * w + 1];
+      ei = ei + lds[3 * w + 2];
+    }
+    partials[3 * block] = m;
+    partials[3 * block + 1] = e;
+    partials[3 * block + 2] = ei;
+  }
+}
+
 // z planes per workgroup of a batched level slot (n a multiple of 16): a
 // divisor of n, at most 32.
 __host__ __device__ inline int batch_tz(int n) { return n <= 32 ? n : (n % 32 == 0 ? 32 : 16); }

@@ -33,6 +33,7 @@ struct UniformArgs {
   double* unew;
   const double* d_dt;     // time step (device)
   double* d_dt_next;      // fused courant output or null
+  double* partials;       // courant_fine sums per workgroup (3 doubles each) or null
   int n;
   double dx;
   HydroParams hp;
@@ -68,10 +69,10 @@ constexpr int tile_threads() { return (TX * TY + 2 * (TX + TY) + 63) / 64 * 64; 
 
 // One workgroup's tile of a level: columns [x0, x0+TX) x [y0, y0+TY),
 // planes [z0, z0+TZ).  Shared by the single-level and the batched kernels.
-template <typename R, int TX, int TY, int RS>
+template <typename R, int TX, int TY, int RS, bool SUMS = false>
 __device__ __forceinline__ void level_tile(const double* __restrict__ u, double* __restrict__ unew, int n, double dx,
                                            double dt, double* d_dt_next, const HydroParams& hp, int x0, int y0,
-                                           int z0, int TZ) {
+                                           int z0, int TZ, double* partials = nullptr, long block = 0) {
   constexpr int EX = TX + 2, EY = TY + 2;      // tile with its one-cell ring
   constexpr int QX = TX + 4, QY = TY + 4;      // primitive plane with 2-cell halo
   constexpr int NQ = QX * QY;
@@ -153,6 +154,8 @@ __device__ __forceinline__ void level_tile(const double* __restrict__ u, double*
   double fz_prev[5] = {0, 0, 0, 0, 0};  // flux through the z face below plane p-1 (scaled)
   double part_prev[5] = {0, 0, 0, 0, 0};  // (u + dFx) + dFy of plane p-1
   double wmax = 0.0;  // largest cmpdt wave speed of the new state
+  ConsSums cs{0.0, 0.0, 0.0};  // courant_fine's sums of the new state (SUMS)
+  const double vol = dx * dx * dx;
 
   // prologue: planes z0-2 and z0-1
   load_plane(z0 - 2, 0);
@@ -258,6 +261,7 @@ __device__ __forceinline__ void level_tile(const double* __restrict__ u, double*
         unew[o + v * vstride] = s;
       }
       if (d_dt_next) wmax = fmax_f(wmax, cmpdt_w(un[0], un[1], un[2], un[3], un[4], hp));
+      if constexpr (SUMS) cons_add(cs, un[0], un[1], un[2], un[3], un[4], vol, hp.smallr);
     }
     // carry to the next plane
     fz_left = rotate(qm_z, 2);
@@ -283,10 +287,14 @@ __device__ __forceinline__ void level_tile(const double* __restrict__ u, double*
       if (m > 0.0) atomicMin(reinterpret_cast<unsigned long long*>(d_dt_next), dt_bits(cmpdt_from_w(m, dx, hp)));
     }
   }
+  if constexpr (SUMS) {
+    __shared__ double cred[3 * (NT / 64)];
+    cons_block_store(cs, partials, block, cred);
+  }
 }
 
 
-template <typename R, int TX, int TY, int RS>
+template <typename R, int TX, int TY, int RS, bool SUMS>
 __global__ __launch_bounds__((tile_threads<TX, TY>()))
 __attribute__((amdgpu_waves_per_eu(3, 8))) void uniform_godunov_kernel(UniformArgs a, int TZ) {
   // XCD-aware tile order: the hardware deals workgroups round-robin to the
@@ -301,7 +309,8 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void uniform_godunov_kernel(UniformAr
   int logical = (b & 7) * per_xcd + (b >> 3);
   if ((nb & 7) != 0) logical = b;  // ragged grids keep the plain order
   const int tx = logical % ntx, ty = (logical / ntx) % nty, tzc = logical / (ntx * nty);
-  level_tile<R, TX, TY, RS>(a.u, a.unew, a.n, a.dx, *a.d_dt, a.d_dt_next, a.hp, tx * TX, ty * TY, tzc * TZ, TZ);
+  level_tile<R, TX, TY, RS, SUMS>(a.u, a.unew, a.n, a.dx, *a.d_dt, a.d_dt_next, a.hp, tx * TX, ty * TY, tzc * TZ,
+                                  TZ, a.partials, b);
 }
 
 // Many independent levels advanced by one launch (continuous batching).
@@ -362,10 +371,19 @@ __global__ void batch_advance_kernel(BatchArgs a) {
 // courant_fine over a uniform level without gravity: max of the cmpdt wave
 // speed per block, then the block's dt limit min-combined across blocks
 // (exactly min over cells of cmpdt, see cmpdt_w).
-__global__ void courant_kernel(const double* u, long ncell, double dx, HydroParams hp, double* d_dt) {
+__global__ void courant_kernel(const double* u, long ncell, double dx, HydroParams hp, double* d_dt,
+                               double* partials) {
   double wmax = 0.0;
-  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += (long)gridDim.x * blockDim.x)
+  ConsSums cs{0.0, 0.0, 0.0};
+  const double vol = dx * dx * dx;
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += (long)gridDim.x * blockDim.x) {
     wmax = fmax_f(wmax, cmpdt_w(u[c], u[c + ncell], u[c + 2 * ncell], u[c + 3 * ncell], u[c + 4 * ncell], hp));
+    if (partials) cons_add(cs, u[c], u[c + ncell], u[c + 2 * ncell], u[c + 3 * ncell], u[c + 4 * ncell], vol, hp.smallr);
+  }
+  if (partials) {
+    __shared__ double cred[3 * 4];
+    cons_block_store(cs, partials, blockIdx.x, cred);
+  }
   for (int off = 32; off > 0; off >>= 1) wmax = fmax_f(wmax, __shfl_xor(wmax, off, 64));
   __shared__ double red[4];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -379,6 +397,42 @@ __global__ void courant_kernel(const double* u, long ncell, double dx, HydroPara
 }
 
 __global__ void set_double(double* p, double v) { *p = v; }
+
+__global__ void set_doubles(double* p, int n, double v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Sum of the per-workgroup conservation sums in block order: thread t adds
+// blocks t, t+256, ... in order, then a fixed pairwise tree; out[0..2] is
+// overwritten, or incremented when `accumulate` (courant_fine's
+// mass_tot = mass_tot + mass_all, hydro/courant_fine.f90:152-154).
+__global__ __launch_bounds__(256) void cons_finish(const double* partials, long nblocks, double* out,
+                                                  int accumulate) {
+  __shared__ double sh[3][256];
+  const int t = threadIdx.x;
+  double m = 0.0, e = 0.0, ei = 0.0;
+  for (long b = t; b < nblocks; b += 256) {
+    m = m + partials[3 * b];
+    e = e + partials[3 * b + 1];
+    ei = ei + partials[3 * b + 2];
+  }
+  sh[0][t] = m;
+  sh[1][t] = e;
+  sh[2][t] = ei;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) {
+      sh[0][t] = sh[0][t] + sh[0][t + h];
+      sh[1][t] = sh[1][t] + sh[1][t + h];
+      sh[2][t] = sh[2][t] + sh[2][t + h];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    for (int k = 0; k < 3; ++k) out[k] = accumulate ? out[k] + sh[k][0] : sh[k][0];
+  }
+}
 
 __global__ void init_slot_kernel(double* st, double t_end, double start) {
   st[kTime] = 0.0;
@@ -410,19 +464,31 @@ __global__ void cmpdt_group(const double* uu, const double* gg, int nv, int ncel
   }
 }
 
-template <typename R, int TX, int TY>
-hipError_t launch_tile(const UniformArgs& a, int TZ, hipStream_t s) {
+template <typename R, int TX, int TY, bool SUMS>
+hipError_t launch_tile_s(const UniformArgs& a, int TZ, hipStream_t s) {
   constexpr int NT = tile_threads<TX, TY>();
   dim3 grid(a.n / TX, a.n / TY, a.n / TZ);
   switch (a.hp.riemann) {
-    case kHLLC: hipLaunchKernelGGL((uniform_godunov_kernel<R, TX, TY, kHLLC>), grid, dim3(NT), 0, s, a, TZ); break;
-    case kHLL: hipLaunchKernelGGL((uniform_godunov_kernel<R, TX, TY, kHLL>), grid, dim3(NT), 0, s, a, TZ); break;
+    case kHLLC: hipLaunchKernelGGL((uniform_godunov_kernel<R, TX, TY, kHLLC, SUMS>), grid, dim3(NT), 0, s, a, TZ); break;
+    case kHLL: hipLaunchKernelGGL((uniform_godunov_kernel<R, TX, TY, kHLL, SUMS>), grid, dim3(NT), 0, s, a, TZ); break;
     case kACOUSTIC:
     case kEXACT:
-      hipLaunchKernelGGL((uniform_godunov_kernel<R, TX, TY, kDynamicRiemann>), grid, dim3(NT), 0, s, a, TZ);
+      hipLaunchKernelGGL((uniform_godunov_kernel<R, TX, TY, kDynamicRiemann, SUMS>), grid, dim3(NT), 0, s, a, TZ);
       break;
-    default: hipLaunchKernelGGL((uniform_godunov_kernel<R, TX, TY, kLLF>), grid, dim3(NT), 0, s, a, TZ); break;
+    default: hipLaunchKernelGGL((uniform_godunov_kernel<R, TX, TY, kLLF, SUMS>), grid, dim3(NT), 0, s, a, TZ); break;
   }
+  return hipGetLastError();
+}
+
+// Launch the level kernel; with sums, then add the workgroups' conservation
+// sums into d_sums[0..2] in block order.
+template <typename R, int TX, int TY>
+hipError_t launch_tile(const UniformArgs& a, int TZ, double* d_sums, hipStream_t s) {
+  if (!a.partials) return launch_tile_s<R, TX, TY, false>(a, TZ, s);
+  hipError_t e = launch_tile_s<R, TX, TY, true>(a, TZ, s);
+  if (e != hipSuccess) return e;
+  const long nblocks = (long)(a.n / TX) * (a.n / TY) * (a.n / TZ);
+  hipLaunchKernelGGL(cons_finish, dim3(1), dim3(256), 0, s, (const double*)a.partials, nblocks, d_sums, 0);
   return hipGetLastError();
 }
 
@@ -482,24 +548,46 @@ hipError_t launch_cmpdt(const double* uu, const double* gg, int nv, int ncell, d
   return hipGetLastError();
 }
 
+hipError_t launch_cons_finish(const double* partials, long nblocks, double* out, int accumulate, hipStream_t s) {
+  hipLaunchKernelGGL(cons_finish, dim3(1), dim3(256), 0, s, partials, nblocks, out, accumulate);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_doubles(double* p, int n, double v, hipStream_t s) {
+  hipLaunchKernelGGL(set_doubles, dim3((n + 63) / 64), dim3(64), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
 hipError_t launch_set_double(double* p, double v, hipStream_t s) {
   hipLaunchKernelGGL(set_double, dim3(1), dim3(1), 0, s, p, v);
   return hipGetLastError();
 }
 
-hipError_t launch_courant(const double* u, int n, double dx, const HydroParams& hp, double* d_dt, hipStream_t s) {
+hipError_t launch_courant(const double* u, int n, double dx, const HydroParams& hp, double* d_dt, hipStream_t s,
+                          double* partials, double* d_sums) {
   const long ncell = (long)n * n * n;
   long blocks = (ncell + 255) / 256;
   if (blocks > 4096) blocks = 4096;
-  hipLaunchKernelGGL(courant_kernel, dim3((unsigned)blocks), dim3(256), 0, s, u, ncell, dx, hp, d_dt);
+  hipLaunchKernelGGL(courant_kernel, dim3((unsigned)blocks), dim3(256), 0, s, u, ncell, dx, hp, d_dt, partials);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !partials) return e;
+  hipLaunchKernelGGL(cons_finish, dim3(1), dim3(256), 0, s, (const double*)partials, blocks, d_sums, 0);
   return hipGetLastError();
+}
+
+// Workgroups of the largest level launch (8x8 tiles of 4 planes; the
+// courant reduction uses at most as many), 3 doubles of sums each.
+size_t uniform_sums_work_bytes(int n) {
+  const long blocks = (long)n * n * n / 256;
+  return (size_t)(blocks > 1 ? blocks : 1) * 3 * sizeof(double);
 }
 
 // Tile choice: the largest supported interior tile dividing n, and a z chunk
 // that leaves enough workgroups to fill the chip.
 hipError_t launch_uniform_godunov(const double* u, double* unew, int n, double dx, const double* d_dt,
-                                  double* d_dt_next, const HydroParams& hp, bool fp32, hipStream_t s) {
-  UniformArgs a{u, unew, d_dt, d_dt_next, n, dx, hp};
+                                  double* d_dt_next, const HydroParams& hp, bool fp32, hipStream_t s,
+                                  double* partials, double* d_sums) {
+  UniformArgs a{u, unew, d_dt, d_dt_next, partials, n, dx, hp};
   int TX, TY;
   const int tile = tuning().level_tile;
   if (tile == 1 && n % 32 == 0) { TX = 32; TY = 8; }
@@ -515,15 +603,15 @@ hipError_t launch_uniform_godunov(const double* u, double* unew, int n, double d
   const int tz = tuning().level_tz;
   if (tz >= 4 && tz <= n && n % tz == 0) TZ = tz;
   if (fp32) {
-    if (TX == 32 && TY == 16) return launch_tile<float, 32, 16>(a, TZ, s);
-    if (TX == 32) return launch_tile<float, 32, 8>(a, TZ, s);
-    if (TX == 16) return launch_tile<float, 16, 8>(a, TZ, s);
-    return launch_tile<float, 8, 8>(a, TZ, s);
+    if (TX == 32 && TY == 16) return launch_tile<float, 32, 16>(a, TZ, d_sums, s);
+    if (TX == 32) return launch_tile<float, 32, 8>(a, TZ, d_sums, s);
+    if (TX == 16) return launch_tile<float, 16, 8>(a, TZ, d_sums, s);
+    return launch_tile<float, 8, 8>(a, TZ, d_sums, s);
   }
-  if (TX == 32 && TY == 16) return launch_tile<double, 32, 16>(a, TZ, s);
-  if (TX == 32) return launch_tile<double, 32, 8>(a, TZ, s);
-  if (TX == 16) return launch_tile<double, 16, 8>(a, TZ, s);
-  return launch_tile<double, 8, 8>(a, TZ, s);
+  if (TX == 32 && TY == 16) return launch_tile<double, 32, 16>(a, TZ, d_sums, s);
+  if (TX == 32) return launch_tile<double, 32, 8>(a, TZ, d_sums, s);
+  if (TX == 16) return launch_tile<double, 16, 8>(a, TZ, d_sums, s);
+  return launch_tile<double, 8, 8>(a, TZ, d_sums, s);
 }
 
 }  // namespace ramses

@@ -17,10 +17,29 @@ import ctypes
 from . import _check, _stream_handle, load_library
 
 
+def _bind_sums(lib):
+    if getattr(lib, "_sums_bound", False):
+        return
+    vp = ctypes.c_void_p
+    lib.ramses_uniform_work_bytes.argtypes = [ctypes.c_int]
+    lib.ramses_uniform_work_bytes.restype = ctypes.c_ulong
+    lib.ramses_uniform_courant_sums.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, vp, vp, vp,
+                                                ctypes.c_ulong, vp]
+    lib.ramses_uniform_courant_sums.restype = ctypes.c_int
+    lib.ramses_uniform_godunov_sums.argtypes = [vp, vp, ctypes.c_int, ctypes.c_double, vp, vp, vp, vp,
+                                                ctypes.c_ulong, vp]
+    lib.ramses_uniform_godunov_sums.restype = ctypes.c_int
+    lib._sums_bound = True
+
+
 class UniformLevel:
     """Conserved state of an n^3 periodic level resident on the GPU."""
 
-    def __init__(self, u0, boxlen=1.0, smallc=1e-10, device="cuda", stream=None):
+    def __init__(self, u0, boxlen=1.0, smallc=1e-10, device="cuda", stream=None, sums=False):
+        """sums=True also computes courant_fine's conservation sums
+        (hydro/courant_fine.f90:95-119) of the current state with every dt
+        reduction: ``self.sums`` = (mass, sum E*vol, internal energy) on the
+        device, after each step those of the new state."""
         import torch
         n = u0.shape[-1]
         if tuple(u0.shape) != (5, n, n, n):
@@ -38,6 +57,13 @@ class UniformLevel:
         # dt slots: [0] current step, [1] fused reduction for the next step
         self.dt = torch.zeros(2, dtype=torch.float64, device=device)
         self.lib = load_library()
+        _bind_sums(self.lib)
+        self.sums = None
+        self._work = None
+        if sums:
+            self.sums = torch.zeros(3, dtype=torch.float64, device=device)
+            nbytes = int(self.lib.ramses_uniform_work_bytes(n))
+            self._work = torch.empty((nbytes + 7) // 8, dtype=torch.float64, device=device)
         self._dt_ready = False
         self.steps = 0
         self.t = 0.0
@@ -49,8 +75,9 @@ class UniformLevel:
         """courant_fine on the current state; returns dt (host float)."""
         start = self.boxlen / self.smallc if dt_start is None else dt_start
         s = _stream_handle(self.stream)
-        _check(self.lib.ramses_uniform_courant(self._ptr(self.u), self.n, self.dx, start,
-                                               self._ptr(self.dt, 0), s), "ramses_uniform_courant")
+        _check(self.lib.ramses_uniform_courant_sums(self._ptr(self.u), self.n, self.dx, start,
+                                                    self._ptr(self.dt, 0), *self._sums_args(), s),
+               "ramses_uniform_courant")
         self._dt_ready = True
         return self.dt[0]
 
@@ -72,8 +99,10 @@ class UniformLevel:
             _check(self.lib.ramses_uniform_dt_init(d_next, start, s), "ramses_uniform_dt_init")
         if events is not None:
             events[0].record(self._torch_stream())
-        _check(self.lib.ramses_uniform_godunov(self._ptr(self.u), self._ptr(self.unew), self.n, self.dx,
-                                               d_dt, d_next if fused else None, s), "ramses_uniform_godunov")
+        _check(self.lib.ramses_uniform_godunov_sums(self._ptr(self.u), self._ptr(self.unew), self.n, self.dx,
+                                                    d_dt, d_next if fused else None,
+                                                    *(self._sums_args() if fused else (None, None, 0)), s),
+               "ramses_uniform_godunov")
         if events is not None:
             events[1].record(self._torch_stream())
         used = self.dt[0].clone()
@@ -122,9 +151,15 @@ class UniformLevel:
         s = _stream_handle(self.stream)
         if not self._dt_ready:
             self.courant()
-        _check(self.lib.ramses_uniform_godunov(self._ptr(self.u), self._ptr(self.unew), self.n, self.dx,
-                                               self._ptr(self.dt, 0), self._ptr(self.dt, 1), s),
+        _check(self.lib.ramses_uniform_godunov_sums(self._ptr(self.u), self._ptr(self.unew), self.n, self.dx,
+                                                    self._ptr(self.dt, 0), self._ptr(self.dt, 1),
+                                                    *self._sums_args(), s),
                "ramses_uniform_godunov")
+
+    def _sums_args(self):
+        if self.sums is None:
+            return (None, None, 0)
+        return (self._ptr(self.sums), self._ptr(self._work), self._work.numel() * 8)
 
     def _torch_stream(self):
         import torch

@@ -64,7 +64,9 @@ def reference_amr_step(A, levelmin, nlevelmax, dt, dtold, adaptive=True, smallc=
     reference's compiled routines, with refinement (refine_fine at
     levelmin / second substeps, flag_fine after each level step).
     nsub: {level: 1 or 2} (default 2 everywhere); stats["cells"] counts
-    8 cells per oct per godunov_fine call; clock["t"] / clock["nstep"]
+    8 cells per oct per godunov_fine call and stats["sums"] adds up the
+    three conservation sums each courant_fine call returns (mass, sum
+    E*vol, eint), as mass_tot etc. do; clock["t"] / clock["nstep"]
     advance as update_time advances t and nstep (amr/update_time.f90:330-332)
     where amr_step calls it (amr/amr_step.f90:357,363)."""
     ns = nsub or {}
@@ -84,7 +86,12 @@ def reference_amr_step(A, levelmin, nlevelmax, dt, dtold, adaptive=True, smallc=
             for i in range(l, nlevelmax + 1):
                 A.refine_fine(i)
         dtold[l] = dt[l]
-        dt[l] = A.courant(l, boxlen / smallc)[0]
+        cf = A.courant(l, boxlen / smallc)
+        dt[l] = cf[0]
+        if stats is not None:
+            acc = stats.setdefault("sums", [0.0, 0.0, 0.0])
+            for k in range(3):
+                acc[k] = acc[k] + cf[1 + k]
         if l > levelmin:
             dt[l] = min(dt[l - 1] / float(ns.get(l - 1, 2)), dt[l])
         A.set_unew(l)

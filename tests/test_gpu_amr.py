@@ -181,7 +181,7 @@ def test_amr_subcycled_steps_vs_reference(gpu, graph):
     ra.configure()
 
 
-def _refine_setup(egd, egp, nlevelmax=6, seed=3, extra_grids=20000):
+def _refine_setup(egd, egp, nlevelmax=6, seed=3, extra_grids=20000, **solver_kw):
     import ramses_amd as ra
     from ramses_amd.amr import AmrSolver, configure_interpolation, configure_refinement
     R = oracle.Reference()
@@ -197,7 +197,7 @@ def _refine_setup(egd, egp, nlevelmax=6, seed=3, extra_grids=20000):
     A.set_refine(3, err_grad_d=egd, err_grad_p=egp)
     configure_interpolation(1, 0)
     configure_refinement(err_grad_d=egd, err_grad_p=egp)
-    s = AmrSolver(tree, u, refine=True, nlevelmax=nlevelmax)
+    s = AmrSolver(tree, u, refine=True, nlevelmax=nlevelmax, **solver_kw)
     return A, s, tree
 
 
@@ -758,4 +758,40 @@ def test_captured_step_errors(gpu):
     with pytest.raises(ra.RamsesError, match="free memory"):
         s.coarse_step()
     configure_refinement()
+    ra.configure()
+
+
+@needs_ref
+@pytest.mark.parametrize("graph", [False, True])
+def test_coarse_step_conservation_sums_vs_reference(gpu, graph):
+    """courant_fine's mass, sum E*vol and internal-energy totals of each
+    adaptive coarse step (hydro/courant_fine.f90:95-119, summed over every
+    courant_fine call of the step and reset per coarse step as
+    amr/adaptive_loop.f90:89-91 does) agree with the reference's to 1e-12;
+    computing them leaves the mesh and state bitwise equal to the
+    reference's, and the totals are the same bits on a second run."""
+    import ramses_amd as ra
+    from amr_cases import reference_amr_step
+    runs = []
+    for rep in range(2):
+        A, s, tree = _refine_setup(0.12, 0.25, graph=graph, sums=True)
+        A.set_xg(s.xg[1:].cpu().numpy())
+        for l in range(6, 0, -1):
+            if s.noct(l):
+                A.flag_fine(l, 2)
+        s.flag_all()
+        dt, dtold = [0.0] * 8, [0.0] * 8
+        got = []
+        for step in range(3):
+            stats = {}
+            reference_amr_step(A, 3, 6, dt, dtold, stats=stats)
+            s.coarse_step()
+            s.check()
+            mine = s.sums.cpu().numpy()
+            np.testing.assert_allclose(mine, stats["sums"], rtol=1e-12, atol=0)
+            got.append(mine.copy())
+        assert np.array_equal(s.state("old"), A.get(0))
+        runs.append(got)
+        del s
+    assert all(np.array_equal(a, b) for a, b in zip(*runs))
     ra.configure()

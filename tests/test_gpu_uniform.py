@@ -94,3 +94,39 @@ def test_graph_replay_equals_eager(gpu):
     for _ in range(3):
         eager.step()
     assert torch.equal(eager.u, graphed.u)
+
+
+def test_uniform_conservation_sums_vs_reference(gpu):
+    """courant_fine's sums (hydro/courant_fine.f90:95-119) of a uniform
+    level: the initial state's and, after each fused step, the new state's,
+    against the reference's own courant_fine on the same states to 1e-12;
+    the sums change neither dt nor the state, and repeat bit for bit."""
+    import torch
+    from oracle.ref_uniform import ReferenceUniformLevel
+    from ramses_amd import ics
+    n = 32
+    u0 = ics.turbulent_host(n, 5.0, rho_sigma=0.5, seed=4)
+    ra.configure(gamma=5 / 3, riemann="hllc", slope_type=1, courant_factor=0.5)
+    R = ReferenceUniformLevel(n, 5 / 3, riemann="hllc", slope_type=1, courant_factor=0.5)
+    R.load(u0)
+    plain = ra.UniformLevel(u0)
+    runs = []
+    for rep in range(2):
+        lvl = ra.UniformLevel(u0, sums=True)
+        lvl.courant()
+        got = [lvl.sums.cpu().numpy().copy()]
+        for s in range(3):
+            lvl.step()
+            got.append(lvl.sums.cpu().numpy().copy())
+        runs.append((got, lvl.u.clone()))
+    for s in range(3):
+        plain.step()
+    assert torch.equal(runs[0][1], plain.u) and torch.equal(runs[1][1], plain.u)
+    assert all(np.array_equal(a, b) for a, b in zip(runs[0][0], runs[1][0]))
+    for s in range(4):
+        ref = R.courant()
+        np.testing.assert_allclose(runs[0][0][s], ref[1:], rtol=1e-12, atol=0)
+        if s < 3:
+            R.step(ref[0])
+    R.release()
+    ra.configure()
