@@ -395,10 +395,14 @@ def amr_bench(torch, ra, args):
 
 
 def amr_cpu_baseline(w, lmin, lmax, budget_s=15.0):
-    """The reference's own compiled level routines (oracle/_ref: init_flow,
-    flag_fine, refine_fine, courant_fine, godunov_fine, set_unew/uold,
-    upload_fine) driven in amr_step's order on the same adaptive Sedov
-    workload, one host core; cell-updates counted as on the GPU."""
+    """The reference's own coarse step (oracle/_ref: adaptive_loop's body
+    around amr_step(levelmin,1), with newdt_fine, courant_fine,
+    godunov_fine, flag_fine, refine_fine and upload_fine as the reference
+    calls them) on the same adaptive Sedov workload after the reference's
+    init_refine sequence, one host core.  Cell-updates are counted as on
+    the GPU (8 per oct per godunov_fine call), by a second, untimed pass of
+    the same steps through the level routines one by one, which gives the
+    same mesh bit for bit."""
     import numpy as np
     import oracle
     if not oracle.reference_available():
@@ -406,32 +410,41 @@ def amr_cpu_baseline(w, lmin, lmax, budget_s=15.0):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from amr_cases import reference_amr_step, reference_init_refine
     from ramses_amd.tree import OctTree
-    R = oracle.Reference()
-    A = oracle.ReferenceAMR(R)
-    R.set_params(gamma=w["gamma"], riemann="hllc", slope_type=1, courant_factor=w["courant"])
-    tree = OctTree(lmin, lmin, extra_grids=w["ngridmax"] - sum(8 ** k for k in range(lmin)))
-    A.install(tree, nlevelmax=lmax)
-    A.set_interpol(1, 0)
-    A.set_refine(lmin, err_grad_d=w["err_grad_d"], err_grad_p=w["err_grad_p"])
-    xg = np.zeros((tree.ngridmax, 3))
-    for (l, gx, gy, gz), i in tree.grid_at.items():
-        xg[i - 1] = (np.array([gx, gy, gz]) + 0.5) * 0.5 ** (l - 1)
-    A.set_xg(xg)
-    A.set_sedov(1.0, 1e-5, 0.4, (0.5, 0.5, 0.5))
-    reference_init_refine(A, lmin, lmax)
-    dt, dtold = [0.0] * (lmax + 2), [0.0] * (lmax + 2)
-    stats, steps = {"cells": 0}, 0
+
+    def setup():
+        R = oracle.Reference()
+        A = oracle.ReferenceAMR(R)
+        R.set_params(gamma=w["gamma"], riemann="hllc", slope_type=1, courant_factor=w["courant"])
+        tree = OctTree(lmin, lmin, extra_grids=w["ngridmax"] - sum(8 ** k for k in range(lmin)))
+        A.install(tree, nlevelmax=lmax)
+        A.set_interpol(1, 0)
+        A.set_refine(lmin, err_grad_d=w["err_grad_d"], err_grad_p=w["err_grad_p"])
+        xg = np.zeros((tree.ngridmax, 3))
+        for (l, gx, gy, gz), i in tree.grid_at.items():
+            xg[i - 1] = (np.array([gx, gy, gz]) + 0.5) * 0.5 ** (l - 1)
+        A.set_xg(xg)
+        A.set_sedov(1.0, 1e-5, 0.4, (0.5, 0.5, 0.5))
+        reference_init_refine(A, lmin, lmax)
+        A.flag_coarse()
+        return A
+    A = setup()
+    steps = 0
     t0 = time.perf_counter()
     while time.perf_counter() - t0 < budget_s and steps < 8:
-        reference_amr_step(A, lmin, lmax, dt, dtold, stats=stats)
+        A.coarse_step()
         steps += 1
     el = time.perf_counter() - t0
+    A = setup()
+    dt, dtold = [0.0] * (lmax + 2), [0.0] * (lmax + 2)
+    stats = {"cells": 0}
+    for _ in range(steps):
+        reference_amr_step(A, lmin, lmax, dt, dtold, stats=stats)
     cells = stats["cells"]
     A.install(OctTree(1, 1), nlevelmax=1)  # release the reference's arrays
     return {"value": cells / el, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
             "sample": f"{steps} coarse steps of the same adaptive Sedov run ({cells} cell-updates) in {el:.1f} s "
-                      f"after init_refine; flang -O2 build of the reference's level routines driven in amr_step's "
-                      f"order"}
+                      f"after init_refine; flang -O2 build of the reference's own amr_step driver and level "
+                      f"routines"}
 
 
 def serve_bench(torch, ra, args):

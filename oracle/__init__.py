@@ -228,6 +228,9 @@ class ReferenceAMR:
         lib.ref_amr_numbf.restype = ctypes.c_int
         lib.ref_amr_set_outputs.argtypes = [ctypes.c_int, _dp]
         lib.ref_amr_dump.argtypes = [ctypes.c_char_p, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        lib.ref_amr_coarse_step.argtypes = [_dp]
+        lib.ref_amr_set_clock.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        lib.ref_amr_get_run.argtypes = [ctypes.c_int, _dp, ip, _dp, _dp]
         self.lib = lib
 
     def install(self, tree, nlevelmax=None):
@@ -428,6 +431,27 @@ class ReferenceAMR:
         dto, dtn = np.zeros(L), np.zeros(L)
         self.lib.ref_amr_get_run(L, _ptr(t), ints.ctypes.data_as(ip), _ptr(dto), _ptr(dtn))
         return (float(t[0]),) + tuple(int(v) for v in ints) + (dto, dtn)
+
+    def coarse_step(self):
+        """One coarse step of the reference's own driver: adaptive_loop's
+        body around amr_step(levelmin,1) (amr/adaptive_loop.f90:84-194,
+        amr/amr_step.f90, pm/newdt_fine.f90, amr/update_time.f90), with the
+        level time steps, t and nstep kept in the reference's module
+        globals.  The coarse cell must be flagged (flag_coarse) before the
+        first call.  Returns the step's conservation sums (mass, sum E*vol,
+        internal energy) as courant_fine accumulates them."""
+        sums = np.zeros(3)
+        self.lib.ref_amr_coarse_step(_ptr(sums))
+        return tuple(float(v) for v in sums)
+
+    def set_clock(self, t=0.0, nstep=0, nstep_coarse=0):
+        """The run clock of the reference's driver: t, nstep, nstep_coarse."""
+        self.lib.ref_amr_set_clock(t, nstep, nstep_coarse)
+
+    def clock(self):
+        """(t, nstep, nstep_coarse, dtold[1:L], dtnew[1:L]) of the reference's driver."""
+        st = self.run_state()
+        return st[0], st[1], st[2], st[6], st[7]
 
     def dump(self, directory, t, nstep=0, nstep_coarse=0):
         """dump_all's snapshot files into an existing directory."""

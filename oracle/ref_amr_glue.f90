@@ -61,6 +61,7 @@ subroutine ref_amr_init(nlevelmax_in, ngridmax_in) bind(C, name="ref_amr_init")
      boundary(1:MAXBOUND,il)%ngrid = 0
   end do
   mass_tot = 0; ekin_tot = 0; eint_tot = 0
+  dtold = 0; dtnew = 0     ! a fresh run's level steps (module zeros)
 end subroutine ref_amr_init
 
 subroutine ref_amr_set_tree(son_in, father_in, nbor_in) bind(C, name="ref_amr_set_tree")
@@ -668,3 +669,70 @@ subroutine ref_amr_get_run(n, tout_, ints, dtold_out, dtnew_out) bind(C, name="r
   dtold_out(1:n) = dtold(1:n)
   dtnew_out(1:n) = dtnew(1:n)
 end subroutine ref_amr_get_run
+
+! One coarse step of the reference's own driver for one domain:
+! amr/adaptive_loop.f90:84-194 around amr_step(levelmin,1)
+! (amr/amr_step.f90, which calls newdt_fine pm/newdt_fine.f90 and
+! update_time amr/update_time.f90): the conservation-sum reset (:88-91),
+! refine_coarse and the coarse levels' build_comm / refine_fine (:95-133),
+! amr_step, then upload_fine and flag_fine of the coarse levels and
+! flag_coarse (:143-191), and nstep_coarse+1 (:194).  The driver writes no
+! snapshot here (foutput is 0 during the call; the caller writes outputs
+! with ref_amr_dump), ncontrol is raised so that update_time prints only
+! at step 0, and with no output times set (noutput 0) update_time's end
+! test reads tout(1) = HUGE for the call instead of tout(0).  The coarse
+! cell's flag must be set (flag_coarse) before the first call, as the
+! reference's init_refine leaves it, or refine_coarse removes level 1.
+! Returns the conservation sums accumulated by the step's courant_fine
+! calls (mass_tot, ekin_tot, eint_tot).
+subroutine ref_amr_coarse_step(sums) bind(C, name="ref_amr_coarse_step")
+  use iso_c_binding
+  use amr_commons
+  use hydro_commons
+  implicit none
+  real(c_double) :: sums(3)
+  integer :: ilevel, fout_save, nctl_save, nout_save
+  real(dp) :: tout1_save
+  fout_save = foutput; nctl_save = ncontrol; nout_save = noutput; tout1_save = tout(1)
+  if (noutput < 1) then
+     noutput = 1
+     tout(1) = huge(1.0d0)
+  end if
+  foutput = 0
+  ncontrol = huge(ncontrol)
+  epot_tot = 0; ekin_tot = 0; mass_tot = 0; eint_tot = 0
+  if (levelmin < nlevelmax .and. .not. static) then
+     call refine_coarse
+     do ilevel = 1, levelmin
+        call build_comm(ilevel)
+        if (ilevel < levelmin) call refine_fine(ilevel)
+     end do
+  end if
+  call amr_step(levelmin, 1)
+  if (levelmin < nlevelmax .and. .not. static) then
+     do ilevel = levelmin-1, 1, -1
+        call upload_fine(ilevel)
+     end do
+     do ilevel = levelmin-1, 1, -1
+        call flag_fine(ilevel, 2)
+     end do
+     call flag_coarse
+  end if
+  nstep_coarse = nstep_coarse + 1
+  foutput = fout_save; ncontrol = nctl_save; noutput = nout_save; tout(1) = tout1_save
+  sums(1) = mass_tot; sums(2) = ekin_tot; sums(3) = eint_tot
+end subroutine ref_amr_coarse_step
+
+! Run clock and level time steps as the reference's driver keeps them:
+! t, nstep and dtold/dtnew(1:n) in, for a caller that resumes a run.
+subroutine ref_amr_set_clock(tnow, nstep_in, nstep_coarse_in) bind(C, name="ref_amr_set_clock")
+  use iso_c_binding
+  use amr_commons
+  implicit none
+  real(c_double), value :: tnow
+  integer(c_int), value :: nstep_in, nstep_coarse_in
+  t = tnow
+  nstep = nstep_in
+  nstep_coarse = nstep_coarse_in
+  nstep_coarse_old = nstep_coarse_in
+end subroutine ref_amr_set_clock

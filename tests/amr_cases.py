@@ -126,6 +126,28 @@ def reference_amr_step(A, levelmin, nlevelmax, dt, dtold, adaptive=True, smallc=
         A.flag_coarse()
 
 
+def reference_driver_step(A, dt=None, dtold=None, stats=None, clock=None):
+    """One coarse step of the reference's OWN driver (oracle.ReferenceAMR.
+    coarse_step: adaptive_loop's body around amr_step(levelmin,1),
+    amr/adaptive_loop.f90:84-194 with amr/amr_step.f90, pm/newdt_fine.f90
+    and amr/update_time.f90), which keeps the level time steps, t and nstep
+    in its own module globals.  Copies them out for the caller: dt[l] /
+    dtold[l] (dtnew / dtold of level l), stats["sums"] the step's
+    conservation sums, clock["t"] / clock["nstep"].  The coarse cell must
+    have been flagged (A.flag_coarse()) before the first step."""
+    sums = A.coarse_step()
+    t, nstep, _, dto, dtn = A.clock()
+    for l in range(1, len(dtn) + 1):
+        if dt is not None and l < len(dt):
+            dt[l] = float(dtn[l - 1])
+        if dtold is not None and l < len(dtold):
+            dtold[l] = float(dto[l - 1])
+    if stats is not None:
+        stats["sums"] = list(sums)
+    if clock is not None:
+        clock["t"], clock["nstep"] = t, nstep
+
+
 def reference_init_refine(A, levelmin, nlevelmax):
     """init_refine + init_refine_2 (amr/init_refine.f90) on a tree installed
     up to levelmin, with the reference's own init_flow / flag_fine /

@@ -312,10 +312,10 @@ def amr_sphere(R):
 
 def amr_refine(R):
     """F5: flag_fine on every level, refine_fine of levels 3-5, then two
-    adaptive coarse steps, on the REFINE_CASE tree; digests of the mesh
-    after each stage."""
+    adaptive coarse steps of the reference's own driver, on the
+    REFINE_CASE tree; digests of the mesh after each stage."""
     sys.path.insert(0, os.path.dirname(HERE))
-    from amr_cases import REFINE_CASE as C, mesh_digests, reference_amr_step, smooth_state, sphere_tree
+    from amr_cases import REFINE_CASE as C, mesh_digests, reference_driver_step, smooth_state, sphere_tree
     tree = sphere_tree(C["levelmin"], C["levelmax"], extra_grids=C["extra_grids"])
     u = smooth_state(tree, seed=C["seed"])
     A = oracle.ReferenceAMR(R)
@@ -340,10 +340,11 @@ def amr_refine(R):
     for l in range(C["levelmin"], L):
         A.refine_fine(l)
     stages["refined"] = snap()
+    A.flag_coarse()
     dt, dtold = [0.0] * (L + 2), [0.0] * (L + 2)
     dts = []
     for k in range(2):
-        reference_amr_step(A, C["levelmin"], L, dt, dtold)
+        reference_driver_step(A, dt, dtold)
         stages[f"step{k + 1}"] = snap()
         dts.append([float(x).hex() for x in dt])
     with open(os.path.join(HERE, "amr_refine.json"), "w") as f:

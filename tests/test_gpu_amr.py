@@ -249,8 +249,9 @@ def test_flag_and_refine_vs_reference(gpu, egd, egp):
 def test_amr_adaptive_steps_vs_reference(gpu, graph, face_octs):
     """Three coarse steps of the adaptive multi-level step (refine at the
     start of the levelmin step and of every second substep, flag after
-    every level step) against the reference's routines driven in
-    amr_step's order: tree, grid lists, free list, state and dts, bitwise.
+    every level step) against the reference's own coarse step
+    (amr/amr_step.f90 under adaptive_loop's body): tree, grid lists, free
+    list, state and dts, bitwise.
     face_octs 0 keeps the default (these small levels run the level unsplit
     one face per thread); -1 runs it one row of faces per thread."""
     import ramses_amd as ra
@@ -264,18 +265,19 @@ def test_amr_adaptive_steps_vs_reference(gpu, graph, face_octs):
 
 
 def _adaptive_steps(graph):
-    from amr_cases import reference_amr_step
+    from amr_cases import reference_driver_step
     A, s, tree = _refine_setup(0.12, 0.25)
     s.graph = graph
     for l in range(6, 0, -1):
         if s.noct(l):
             A.flag_fine(l, 2)
+    A.flag_coarse()
     s.flag_all()
     dt = [0.0] * 8
     dtold = [0.0] * 8
     counts = []
     for k in range(3):
-        reference_amr_step(A, 3, 6, dt, dtold)
+        reference_driver_step(A, dt, dtold)
         s.coarse_step()
         s.check()
         counts.append({l: s.noct(l) for l in range(3, 7)})
@@ -333,10 +335,10 @@ def test_sedov_amr_init_and_steps_vs_reference(gpu, graph):
     hllc, minmod, CFL 0.8, err_grad_d = err_grad_p = 0.1, interpol_type 1)
     at levelmin 3, levelmax 6: the initial mesh built by init_refine /
     init_refine_2 on the GPU against the reference's own init_flow /
-    flag_fine / refine_fine, then three adaptive coarse steps -- mesh, grid
-    centres, state and dts bitwise."""
+    flag_fine / refine_fine, then three adaptive coarse steps of the
+    reference's own driver -- mesh, grid centres, state and dts bitwise."""
     import ramses_amd as ra
-    from amr_cases import mesh_digests, reference_amr_step, reference_init_refine
+    from amr_cases import mesh_digests, reference_driver_step, reference_init_refine
     from ramses_amd.amr import AmrSolver, configure_interpolation, configure_refinement
     from ramses_amd.ics import sedov_regions
     from ramses_amd.tree import OctTree
@@ -377,9 +379,10 @@ def test_sedov_amr_init_and_steps_vs_reference(gpu, graph):
                              f"cells {k.tolist()}: gpu {go[:, k].T.tolist()} ref {ro[:, k].T.tolist()}")
     assert snap(s) == snap(A)
     assert s.noct(L) > 0
+    A.flag_coarse()   # init_refine_2's last pass ends with it (amr/init_refine.f90:121)
     dt, dtold = [0.0] * (L + 2), [0.0] * (L + 2)
     for k in range(3):
-        reference_amr_step(A, lmin, L, dt, dtold)
+        reference_driver_step(A, dt, dtold)
         s.coarse_step()
         s.check()
         assert snap(s) == snap(A), k
@@ -399,7 +402,7 @@ def test_snapshot_after_adaptive_steps_vs_reference(gpu):
     import shutil
     import tempfile
     import ramses_amd as ra
-    from amr_cases import mesh_digests, reference_amr_step
+    from amr_cases import reference_driver_step
     from ramses_amd import snapshot
     A, s, tree = _refine_setup(0.12, 0.25)
     xg = s.xg[1:].cpu().numpy()
@@ -407,19 +410,19 @@ def test_snapshot_after_adaptive_steps_vs_reference(gpu):
     for l in range(6, 0, -1):
         if s.noct(l):
             A.flag_fine(l, 2)
+    A.flag_coarse()
     s.flag_all()
     L = 6
-    dt, dtold = [0.0] * 8, [0.0] * 8
+    clock = {}
     for _ in range(2):
-        reference_amr_step(A, 3, L, dt, dtold)
+        reference_driver_step(A, clock=clock)
         s.coarse_step()
     s.check()
+    assert (float(s.time.item()), s.fine_steps) == (clock["t"], clock["nstep"])
     base = tempfile.mkdtemp(prefix="snap")
     try:
         os.makedirs(os.path.join(base, "ref"))
-        A.set_dt(dtold[1:L + 1], dt[1:L + 1])
-        t = float(s.time.item())
-        A.dump(os.path.join(base, "ref"), t, s.fine_steps, 2)
+        A.dump(os.path.join(base, "ref"), clock["t"], clock["nstep"], 2)
         run = snapshot.RunInfo(levelmin=3, gamma=5 / 3, ordering="planar", bound_key=(-1.0, 1e30))
         out = s.write_output(os.path.join(base, "ours"), run)
         for name in ("info_00001.txt", "amr_00001.out00001", "hydro_00001.out00001", "hydro_file_descriptor.txt"):
@@ -427,7 +430,7 @@ def test_snapshot_after_adaptive_steps_vs_reference(gpu):
                 open(os.path.join(base, "ref", name), "rb").read(), name
     finally:
         shutil.rmtree(base)
-    reference_amr_step(A, 3, L, dt, dtold)
+    reference_driver_step(A)
     s.coarse_step()
     s.check()
     son, father, nbor, flag1 = s.tree_arrays()
@@ -448,7 +451,7 @@ def test_sedov_amr_no_subcycling_vs_reference(gpu, graph):
     step per level per coarse step, the coarse dt taken from the finest
     level; init_refine and three coarse steps bitwise."""
     import ramses_amd as ra
-    from amr_cases import mesh_digests, reference_amr_step, reference_init_refine
+    from amr_cases import reference_driver_step, reference_init_refine
     from ramses_amd.amr import AmrSolver, configure_interpolation, configure_refinement
     from ramses_amd.ics import sedov_regions
     from ramses_amd.tree import OctTree
@@ -469,10 +472,10 @@ def test_sedov_amr_no_subcycling_vs_reference(gpu, graph):
     A.set_sedov(1.0, 1e-5, 0.4, (0.5, 0.5, 0.5))
     reference_init_refine(A, lmin, L)
     s.init_refine(lambda x, dx: sedov_regions(x, dx, gamma=1.4))
+    A.flag_coarse()
     dt, dtold = [0.0] * (L + 2), [0.0] * (L + 2)
-    ns = {l: 1 for l in range(1, L + 1)}
     for k in range(3):
-        reference_amr_step(A, lmin, L, dt, dtold, nsub=ns)
+        reference_driver_step(A, dt, dtold)
         s.coarse_step()
         s.check()
         son, father, nbor, flag1 = s.tree_arrays()
@@ -539,12 +542,12 @@ def test_namelist_run_matches_reference(gpu, graph):
     """A parameter file run end to end (ramses_amd.run: namelist, region
     initial conditions in a box of length 0.5, init_refine, two adaptive
     coarse steps with mixed subcycling, snapshot) against the reference's
-    own routines driven through the same sequence: the four snapshot files
-    are byte-identical."""
+    own routines (init_refine's sequence, then the reference's own coarse
+    step): the four snapshot files are byte-identical."""
     import shutil
     import tempfile
     import ramses_amd as ra
-    from amr_cases import reference_amr_step, reference_init_refine
+    from amr_cases import reference_driver_step, reference_init_refine
     from ramses_amd import namelist, run
     from ramses_amd.tree import OctTree
     base = tempfile.mkdtemp(prefix="nml")
@@ -572,12 +575,10 @@ def test_namelist_run_matches_reference(gpu, graph):
         A.set_xg(xg)
         A.set_regions(p.regions)
         reference_init_refine(A, lmin, L)
-        dt, dtold = [0.0] * (L + 2), [0.0] * (L + 2)
-        ns = {l: p.nsubcycle[l - 1] for l in range(1, L + 1)}
-        clock = {"t": 0.0, "nstep": 0}
+        A.flag_coarse()
+        clock = {}
         for _ in range(p.nstepmax):
-            reference_amr_step(A, lmin, L, dt, dtold, boxlen=p.boxlen, nsub=ns, clock=clock)
-        A.set_dt(dtold[1:L + 1], dt[1:L + 1])
+            reference_driver_step(A, clock=clock)
         os.makedirs(os.path.join(base, "ref"))
         A.dump(os.path.join(base, "ref"), clock["t"], clock["nstep"], p.nstepmax)
         out = outs[-1]
@@ -598,7 +599,7 @@ def test_refine_interpolation_variants_vs_reference(gpu, itype, ivar):
     interpolation type / variable choice gives the reference's new cells,
     and one adaptive coarse step stays bitwise."""
     import ramses_amd as ra
-    from amr_cases import reference_amr_step
+    from amr_cases import reference_driver_step
     from ramses_amd.amr import configure_interpolation
     A, s, tree = _refine_setup(0.12, 0.25, extra_grids=60000)
     A.set_interpol(itype, ivar)
@@ -606,6 +607,7 @@ def test_refine_interpolation_variants_vs_reference(gpu, itype, ivar):
     for l in range(6, 0, -1):
         if s.noct(l):
             A.flag_fine(l, 2)
+    A.flag_coarse()
     s.flag_all()
     for l in range(3, 6):
         A.refine_fine(l)
@@ -613,8 +615,7 @@ def test_refine_interpolation_variants_vs_reference(gpu, itype, ivar):
     s.check()
     _same_tree(A, s, 6, "refine")
     assert np.array_equal(s.state("old"), A.get(0))
-    dt, dtold = [0.0] * 8, [0.0] * 8
-    reference_amr_step(A, 3, 6, dt, dtold)
+    reference_driver_step(A)
     s.coarse_step()
     s.check()
     _same_tree(A, s, 6, "step")
@@ -767,11 +768,12 @@ def test_coarse_step_conservation_sums_vs_reference(gpu, graph):
     """courant_fine's mass, sum E*vol and internal-energy totals of each
     adaptive coarse step (hydro/courant_fine.f90:95-119, summed over every
     courant_fine call of the step and reset per coarse step as
-    amr/adaptive_loop.f90:89-91 does) agree with the reference's to 1e-12;
+    amr/adaptive_loop.f90:89-91 does) agree with those of the reference's
+    own coarse step to 1e-12;
     computing them leaves the mesh and state bitwise equal to the
     reference's, and the totals are the same bits on a second run."""
     import ramses_amd as ra
-    from amr_cases import reference_amr_step
+    from amr_cases import reference_driver_step
     runs = []
     for rep in range(2):
         A, s, tree = _refine_setup(0.12, 0.25, graph=graph, sums=True)
@@ -779,12 +781,13 @@ def test_coarse_step_conservation_sums_vs_reference(gpu, graph):
         for l in range(6, 0, -1):
             if s.noct(l):
                 A.flag_fine(l, 2)
+        A.flag_coarse()
         s.flag_all()
         dt, dtold = [0.0] * 8, [0.0] * 8
         got = []
         for step in range(3):
             stats = {}
-            reference_amr_step(A, 3, 6, dt, dtold, stats=stats)
+            reference_driver_step(A, dt, dtold, stats=stats)
             s.coarse_step()
             s.check()
             mine = s.sums.cpu().numpy()

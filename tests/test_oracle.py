@@ -197,9 +197,10 @@ def test_cmpdt_closed_form(oracle_lib):
 @pytest.mark.needs_ref
 def test_refine_fixture_reproduces():
     """The committed adaptive-mesh digests are what the reference's own
-    flag_fine / refine_fine / amr_step sequence produces here."""
+    flag_fine / refine_fine and its own coarse step (amr_step under
+    adaptive_loop's body) produce here."""
     import json
-    from amr_cases import REFINE_CASE as C, mesh_digests, reference_amr_step, smooth_state, sphere_tree
+    from amr_cases import REFINE_CASE as C, mesh_digests, reference_driver_step, smooth_state, sphere_tree
     with open(os.path.join(os.path.dirname(__file__), "golden", "amr_refine.json")) as f:
         gold = json.load(f)
     R = oracle.Reference()
@@ -223,8 +224,9 @@ def test_refine_fixture_reproduces():
     assert got == gold["stages"]["flagged"]
     for l in range(C["levelmin"], L):
         A.refine_fine(l)
+    A.flag_coarse()
     dt, dtold = [0.0] * (L + 2), [0.0] * (L + 2)
-    reference_amr_step(A, C["levelmin"], L, dt, dtold)
+    reference_driver_step(A, dt, dtold)
     son, father, nbor, flag1 = A.tree_arrays()
     got = mesh_digests(son, father, nbor, flag1, {l: A.grids(l) for l in range(1, L + 1)}, A.free_list(),
                        A.get(0), A.get(1))
@@ -275,3 +277,53 @@ def test_reference_uniform_level_vs_oracle():
         assert got[0] == want[0]
         np.testing.assert_allclose(got[1:], want[1:], rtol=1e-12)
         R.release()
+
+
+@pytest.mark.needs_ref
+def test_reference_driver_matches_restatement():
+    """The reference's own coarse step (amr/amr_step.f90, pm/newdt_fine.f90
+    and amr/update_time.f90 under adaptive_loop's body, compiled in
+    oracle/_ref) and tests/amr_cases.reference_amr_step (a restatement of
+    the same order driving the reference's compiled level routines one by
+    one) give the same mesh, state, level dts, t, nstep and conservation
+    sums over three adaptive coarse steps.  This pins the restatement,
+    which the fixed-mesh tests still use: the reference has no multi-level
+    mode without refinement (its `static` switch also stops godunov_fine)."""
+    from amr_cases import reference_amr_step, reference_driver_step, smooth_state, sphere_tree
+
+    def setup():
+        A = oracle.ReferenceAMR(oracle.Reference())
+        tree = sphere_tree(extra_grids=20000)
+        u = smooth_state(tree, seed=3)
+        A.ref.set_params(gamma=5 / 3, riemann="hllc", slope_type=1, courant_factor=0.5)
+        A.install(tree, nlevelmax=6)
+        A.set_interpol(1, 0)
+        A.put(0, u)
+        A.put(1, u)
+        A.set_refine(3, err_grad_d=0.12, err_grad_p=0.25)
+        for lev in range(6, 0, -1):
+            if len(A.grids(lev)):
+                A.flag_fine(lev, 2)
+        A.flag_coarse()
+        return A
+    A = setup()
+    drv = []
+    for _ in range(3):
+        dt, dtold, stats, clock = [0.0] * 8, [0.0] * 8, {}, {}
+        reference_driver_step(A, dt, dtold, stats=stats, clock=clock)
+        drv.append((A.get(0), A.tree_arrays(), A.free_list(), dt, dtold, stats["sums"], clock))
+    A = setup()
+    dt, dtold, clock = [0.0] * 8, [0.0] * 8, {"t": 0.0, "nstep": 0}
+    nocts = []
+    for k in range(3):
+        stats = {}
+        reference_amr_step(A, 3, 6, dt, dtold, stats=stats, clock=clock)
+        u, tr, free, rdt, rdtold, rsums, rclock = drv[k]
+        assert np.array_equal(A.get(0), u), k
+        assert all(np.array_equal(a, b) for a, b in zip(A.tree_arrays(), tr)), k
+        assert np.array_equal(A.free_list(), free), k
+        assert dt[3:7] == rdt[3:7] and dtold[3:7] == rdtold[3:7], k
+        assert stats["sums"] == rsums, k
+        assert (clock["t"], clock["nstep"]) == (rclock["t"], rclock["nstep"]), k
+        nocts.append([len(A.grids(l)) for l in range(3, 7)])
+    assert nocts[0] != nocts[-1]

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import oracle
-from amr_cases import reference_amr_step, smooth_state, sphere_tree
+from amr_cases import reference_driver_step, smooth_state, sphere_tree
 from ramses_amd import snapshot
 
 needs_ref = pytest.mark.needs_ref
@@ -40,12 +40,12 @@ def _reference_snapshot(base, seed=3):
     for l in range(L, 0, -1):
         if len(A.grids(l)):
             A.flag_fine(l, 2)
-    dt, dtold = [0.0] * (L + 2), [0.0] * (L + 2)
-    reference_amr_step(A, 3, L, dt, dtold)
-    A.set_dt(dtold[1:L + 1], dt[1:L + 1])
+    A.flag_coarse()
+    clock = {}
+    reference_driver_step(A, clock=clock)
     d = os.path.join(base, "output_00001")
     os.makedirs(d)
-    A.dump(d, 0.0125, 7, 1)
+    A.dump(d, clock["t"], clock["nstep"], 1)
     return R, A, tree
 
 
@@ -126,7 +126,6 @@ def test_restarted_run_matches_reference(gpu):
     files, then two adaptive coarse steps on each: tree, grid lists, free
     list, refinement map, state and level dts equal after every step."""
     import ramses_amd as ra
-    from amr_cases import reference_amr_step
     from ramses_amd.amr import AmrSolver, configure_interpolation, configure_refinement
     base = tempfile.mkdtemp(prefix="rst")
     try:
@@ -142,7 +141,7 @@ def test_restarted_run_matches_reference(gpu):
         assert float(s.time.item()) == t and (s.fine_steps, s.nstep_coarse) == (nstep, nstep_coarse)
         dt, dtold = [0.0] + list(dtn) + [0.0], [0.0] + list(dto) + [0.0]
         for k in range(2):
-            reference_amr_step(A, 3, L, dt, dtold)
+            reference_driver_step(A, dt, dtold)
             s.coarse_step()
             s.check()
             rs, rf, rn, rfl = A.tree_arrays()
@@ -173,7 +172,6 @@ def test_namelist_restart_matches_reference(gpu):
     takes the same two coarse steps: output_00002's four files are
     byte-identical."""
     import ramses_amd as ra
-    from amr_cases import reference_amr_step
     from ramses_amd import namelist, run
     from test_gpu_amr import SMALL_SEDOV_NML
     base = tempfile.mkdtemp(prefix="nml")
@@ -200,12 +198,9 @@ def test_namelist_restart_matches_reference(gpu):
         A.set_outputs(p.tout)
         t, nstep, nstep_coarse, ifout, _, _, dto, dtn = A.run_state()
         assert (ifout, nstep_coarse) == (2, 2)
-        dt, dtold = [0.0] + list(dtn) + [0.0], [0.0] + list(dto) + [0.0]
-        ns = {l: p.nsubcycle[l - 1] for l in range(1, L + 1)}
-        clock = {"t": t, "nstep": nstep}
+        clock = {}
         for _ in range(2):
-            reference_amr_step(A, lmin, L, dt, dtold, boxlen=p.boxlen, nsub=ns, clock=clock)
-        A.set_dt(dtold[1:L + 1], dt[1:L + 1])
+            reference_driver_step(A, clock=clock)
         os.makedirs(os.path.join(base, "ref"))
         A.dump(os.path.join(base, "ref"), clock["t"], clock["nstep"], 4)
         for name in ("info_00002.txt", "amr_00002.out00001", "hydro_00002.out00001", "hydro_file_descriptor.txt"):

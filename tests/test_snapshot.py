@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle
-from amr_cases import reference_amr_step, smooth_state, sphere_tree
+from amr_cases import reference_driver_step, smooth_state, sphere_tree
 from ramses_amd import snapshot
 
 needs_ref = pytest.mark.needs_ref
@@ -53,16 +53,17 @@ def test_snapshot_matches_reference_writers():
     for l in range(L, 0, -1):
         if len(A.grids(l)):
             A.flag_fine(l, 2)
-    dt, dtold = [0.0] * (L + 2), [0.0] * (L + 2)
-    reference_amr_step(A, 3, L, dt, dtold)
-    A.set_dt(dtold[1:L + 1], dt[1:L + 1])
+    A.flag_coarse()
+    dt, dtold, clock = [0.0] * (L + 2), [0.0] * (L + 2), {}
+    reference_driver_step(A, dt, dtold, clock=clock)
     base = tempfile.mkdtemp(prefix="snap")         # the reference's file names hold 80 characters
     ref_dir = os.path.join(base, "ref")
     os.makedirs(ref_dir)
-    A.dump(ref_dir, 0.0125, 7, 1)                  # defrag + dump_all's writers
+    A.dump(ref_dir, clock["t"], clock["nstep"], 1)  # defrag + dump_all's writers
     mesh = reference_mesh(A, L)                    # the defragmented mesh
     assert np.array_equal(mesh.lists[1], [1])
-    run = snapshot.RunInfo(levelmin=3, gamma=5 / 3, t=0.0125, nstep=7, nstep_coarse=1, dtold=tuple(dtold[1:L + 1]),
+    run = snapshot.RunInfo(levelmin=3, gamma=5 / 3, t=clock["t"], nstep=clock["nstep"], nstep_coarse=1,
+                           dtold=tuple(dtold[1:L + 1]),
                            dtnew=tuple(dt[1:L + 1]), ordering="planar", bound_key=(-1.0, 1e30))
     out = snapshot.write_output(mesh, run, os.path.join(base, "ours"))
     assert run.ifout == 2
