@@ -351,6 +351,18 @@ typedef struct ramses_amr_mesh {
    * internal energy, vol = dx^3 of each level.  Deterministic: workgroup
    * partial sums added in block order. */
   double *sums;
+  /* Optional (NULL: not kept): update_time's conservation baselines
+   * (amr/update_time.f90:229-246), 2 device doubles {einit, mass_tot_0}
+   * that the snapshot files carry.  At the first update_time of a coarse
+   * step with nstep_coarse != nstep_coarse_old (every step but a run's
+   * first, when nstep_coarse_old starts equal to nstep_coarse as
+   * adaptive_loop sets it), each one still zero is set from mass_tot and
+   * epot_tot + ekin_tot as they stand there: the courant_fine sums of the
+   * levels the step's first descent visits, each level summed in the
+   * reference's serial order, so the values round as the reference's do.
+   * The step then sets nstep_coarse_old = nstep_coarse. */
+  double *cons0;
+  long long nstep_coarse_old;
 } ramses_amr_mesh;
 
 /* amr_step(levelmin, 1) (amr/amr_step.f90) followed by the end of a coarse

@@ -231,6 +231,7 @@ class ReferenceAMR:
         lib.ref_amr_coarse_step.argtypes = [_dp]
         lib.ref_amr_set_clock.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int]
         lib.ref_amr_get_run.argtypes = [ctypes.c_int, _dp, ip, _dp, _dp]
+        lib.ref_amr_get_baselines.argtypes = [_dp]
         self.lib = lib
 
     def install(self, tree, nlevelmax=None):
@@ -452,6 +453,12 @@ class ReferenceAMR:
         """(t, nstep, nstep_coarse, dtold[1:L], dtnew[1:L]) of the reference's driver."""
         st = self.run_state()
         return st[0], st[1], st[2], st[6], st[7]
+
+    def baselines(self):
+        """update_time's conservation baselines (einit, mass_tot_0)."""
+        out = np.zeros(2)
+        self.lib.ref_amr_get_baselines(_ptr(out))
+        return out
 
     def dump(self, directory, t, nstep=0, nstep_coarse=0):
         """dump_all's snapshot files into an existing directory."""

@@ -62,6 +62,11 @@ subroutine ref_amr_init(nlevelmax_in, ngridmax_in) bind(C, name="ref_amr_init")
   end do
   mass_tot = 0; ekin_tot = 0; eint_tot = 0
   dtold = 0; dtnew = 0     ! a fresh run's level steps (module zeros)
+  ! update_time's conservation baselines and step bookkeeping as a fresh
+  ! run starts them (amr/amr_commons.f90:10-22)
+  nstep_coarse_old = 0
+  epot_tot = 0; epot_tot_old = 0; epot_tot_int = 0; einit = 0; aexp_old = 1
+  mass_tot_0 = 0
 end subroutine ref_amr_init
 
 subroutine ref_amr_set_tree(son_in, father_in, nbor_in) bind(C, name="ref_amr_set_tree")
@@ -652,6 +657,7 @@ subroutine ref_amr_restart(nlevelmax_in, ngridmax_in, nrestart_in) bind(C, name=
   call init_amr
   call init_hydro
   nrestart = 0
+  nstep_coarse_old = nstep_coarse   ! as adaptive_loop sets it before its loop (amr/adaptive_loop.f90:75)
 end subroutine ref_amr_restart
 
 ! Run state after a restart: t, nstep, nstep_coarse, ifout, iout, used_mem
@@ -736,3 +742,14 @@ subroutine ref_amr_set_clock(tnow, nstep_in, nstep_coarse_in) bind(C, name="ref_
   nstep_coarse = nstep_coarse_in
   nstep_coarse_old = nstep_coarse_in
 end subroutine ref_amr_set_clock
+
+! update_time's conservation baselines: out = (einit, mass_tot_0).
+subroutine ref_amr_get_baselines(out) bind(C, name="ref_amr_get_baselines")
+  use iso_c_binding
+  use amr_commons
+  use hydro_commons
+  implicit none
+  real(c_double) :: out(2)
+  out(1) = einit
+  out(2) = mass_tot_0
+end subroutine ref_amr_get_baselines

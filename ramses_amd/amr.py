@@ -64,7 +64,7 @@ class ramses_amr_mesh(ctypes.Structure):
                 ("boxlen", ctypes.c_double), ("nstep", ctypes.c_longlong), ("nstep_coarse", ctypes.c_longlong),
                 ("cell_updates", ctypes.c_longlong), ("work", ctypes.c_void_p), ("work_bytes", ctypes.c_ulong),
                 ("graph", ctypes.c_int), ("graph_capacity", ctypes.c_int), ("graph_state", ctypes.c_void_p),
-                ("sums", ctypes.c_void_p)]
+                ("sums", ctypes.c_void_p), ("cons0", ctypes.c_void_p), ("nstep_coarse_old", ctypes.c_longlong)]
 
 
 def _bind(lib):
@@ -440,6 +440,11 @@ class AmrSolver(AmrLevels):
         self.nstep_coarse = 0
         self.cell_updates = 0
         self.sums = torch.zeros(3, dtype=torch.float64, device=self.device) if sums else None
+        # update_time's conservation baselines {einit, mass_tot_0}
+        # (amr/update_time.f90:229-246) and its nstep_coarse_old, which
+        # adaptive_loop starts equal to nstep_coarse (amr/adaptive_loop.f90:75)
+        self.cons0 = torch.zeros(2, dtype=torch.float64, device=self.device)
+        self.nstep_coarse_old = 0
         self._mesh = ramses_amr_mesh()
 
     @classmethod
@@ -464,6 +469,8 @@ class AmrSolver(AmrLevels):
         s.dtold.copy_(s.dtold.new_tensor(dto))
         s.time.fill_(run.t)
         s.fine_steps, s.nstep_coarse = run.nstep, run.nstep_coarse
+        s.nstep_coarse_old = run.nstep_coarse
+        s.cons0.copy_(s.cons0.new_tensor([run.einit, run.mass_tot_0]))
         return s, run
 
     def __del__(self):
@@ -546,11 +553,13 @@ class AmrSolver(AmrLevels):
         m.nstep, m.nstep_coarse, m.cell_updates = self.fine_steps, self.nstep_coarse, self.cell_updates
         m.graph, m.graph_capacity = int(self.graph), self.graph_capacity
         m.sums = None if self.sums is None else self.sums.data_ptr()
+        m.cons0, m.nstep_coarse_old = self.cons0.data_ptr(), self.nstep_coarse_old
         rc = self.lib.ramses_amr_coarse_step(ctypes.byref(m), self._s())
         for l in range(1, self.nlevelmax + 1):
             self.nocts[l] = m.noct[l]
         self.free.head, self.free.numbf, self.free.used_mem = m.free.head, m.free.numbf, m.free.used_mem
         self.fine_steps, self.nstep_coarse, self.cell_updates = m.nstep, m.nstep_coarse, m.cell_updates
+        self.nstep_coarse_old = m.nstep_coarse_old
         _check(rc, "ramses_amr_coarse_step")
 
     def write_output(self, outdir, run):
@@ -560,6 +569,7 @@ class AmrSolver(AmrLevels):
         self.defrag()
         run.t = float(self.time.item())
         run.nstep, run.nstep_coarse = self.fine_steps, self.nstep_coarse
+        run.einit, run.mass_tot_0 = (float(v) for v in self.cons0.cpu().numpy())
         run.dtold = tuple(self.dtold[1:self.nlevelmax + 1].cpu().numpy())
         run.dtnew = tuple(self.dt[1:self.nlevelmax + 1].cpu().numpy())
         return snapshot.write_output(self.host_mesh(), run, outdir)

@@ -473,6 +473,64 @@ __global__ void amr_dt_kernel(int op, double* dt, double* dtold, double* time, i
   }
 }
 
+// update_time's conservation baselines (amr/update_time.f90:229-246) at
+// the first update_time of a coarse step: mass_tot_0 = mass_tot and einit =
+// epot_tot + ekin_tot (epot_tot = 0 without gravity), each only while still
+// zero.  mass_tot and ekin_tot are then the courant_fine sums of the levels
+// the step's first descent has visited (amr/amr_step.f90:327-352): levelmin
+// down to the first level with no finer octs, each level's sum added as one
+// term (hydro/courant_fine.f90:152-153).  A level's sum runs in the
+// reference's serial order -- batches of nvector grids, cell position ind,
+// grid in the batch (:44-102) -- so the values have its rounding: one
+// wavefront loads 64 cells of that order at a time and adds them in lane
+// order.  A cell that is not a leaf adds +0.0, which leaves the positive
+// running sum unchanged.  With dn the level counts are read from device
+// memory and `armed` (nonzero: nstep_coarse changed since the last
+// update_time) decides at run time.
+__global__ __launch_bounds__(64) void amr_cons0_kernel(ConsLevels lv, const int* dn, AmrTreeView t, const double* u,
+                                                      long ustride, int levelmin, int nlevelmax, double boxlen, int nv,
+                                                      const int* armed, double* cons0) {
+  if (armed && *armed == 0) return;
+  if (cons0[0] != 0.0 && cons0[1] != 0.0) return;
+  const int lane = threadIdx.x;
+  double mass_tot = 0.0, ekin_tot = 0.0;
+  for (int l = levelmin; l <= nlevelmax; ++l) {
+    const int n = dn ? dn[l] : lv.noct[l];
+    if (n <= 0) break;
+    const int* active = lv.active[l];
+    const double dx = __builtin_ldexp(1.0, -l) * boxlen;
+    const double vol = dx * dx * dx;
+    double m = 0.0, e = 0.0;
+    for (int b0 = 0; b0 < n; b0 += nv) {
+      const int ng = min(nv, n - b0);
+      const int nslot = 8 * ng;
+      for (int s0 = 0; s0 < nslot; s0 += 64) {
+        const int sl = s0 + lane;
+        double vm = 0.0, ve = 0.0;
+        if (sl < nslot) {
+          const long c = cell_of(t, sl / ng + 1, active[b0 + sl % ng]);
+          if (t.son[c] == 0) {
+            vm = u[c] * vol;                 // :95-97
+            ve = u[4 * ustride + c] * vol;   // :100-102
+          }
+        }
+        const int cnt = min(64, nslot - s0);
+        for (int k = 0; k < cnt; ++k) {
+          m = m + __shfl(vm, k, 64);
+          e = e + __shfl(ve, k, 64);
+        }
+      }
+    }
+    mass_tot = mass_tot + m;
+    ekin_tot = ekin_tot + e;
+    if (l == nlevelmax || (dn ? dn[l + 1] : lv.noct[l + 1]) == 0) break;
+  }
+  if (lane == 0) {
+    if (cons0[1] == 0.0) cons0[1] = mass_tot;        // mass_tot_0
+    if (cons0[0] == 0.0) cons0[0] = 0.0 + ekin_tot;  // einit = epot_tot + ekin_tot
+  }
+}
+
 __global__ void clear_bits(int* p, int mask) { p[0] &= ~mask; }
 
 __global__ void set_ints(int* p, int n, int v) {
@@ -591,6 +649,14 @@ hipError_t launch_amr_dt(int op, double* dt, double* dtold, double* time, int l,
                          double start, hipStream_t s, const int* dn, long long* cnt) {
   hipLaunchKernelGGL(amr_dt_kernel, dim3(1), dim3(1), 0, s, op, dt, dtold, time, l, coarser, nsub, icount, start, dn,
                      cnt);
+  return hipGetLastError();
+}
+
+hipError_t launch_amr_cons0(const ConsLevels& lv, const int* dn, const AmrTreeView& t, const double* u, long ustride,
+                            int levelmin, int nlevelmax, double boxlen, int nv, const int* armed, double* cons0,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(amr_cons0_kernel, dim3(1), dim3(64), 0, s, lv, dn, t, u, ustride, levelmin, nlevelmax, boxlen, nv,
+                     armed, cons0);
   return hipGetLastError();
 }
 

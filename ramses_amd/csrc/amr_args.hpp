@@ -41,4 +41,12 @@ struct AmrLevelArgs {
   int* error;                // set nonzero if the tree violates the refinement rule
 };
 
+// Every level's grid list and host oct count (index = level), for a kernel
+// that walks several levels.
+constexpr int kConsMaxLevels = 33;  // RAMSES_MAX_LEVELS + 1
+struct ConsLevels {
+  const int* active[kConsMaxLevels];
+  int noct[kConsMaxLevels];
+};
+
 }  // namespace ramses

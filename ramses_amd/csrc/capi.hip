@@ -43,6 +43,9 @@ hipError_t launch_amr_courant(const int* active, int noct, const AmrTreeView& t,
 hipError_t launch_amr_dt(int op, double* dt, double* dtold, double* time, int l, int coarser, int nsub, int icount,
                          double start, hipStream_t s, const int* dn = nullptr, long long* cnt = nullptr);
 hipError_t launch_set_ints(int* p, int n, int v, hipStream_t s);
+hipError_t launch_amr_cons0(const ConsLevels& lv, const int* dn, const AmrTreeView& t, const double* u, long ustride,
+                            int levelmin, int nlevelmax, double boxlen, int nv, const int* armed, double* cons0,
+                            hipStream_t s);
 hipError_t launch_clear_bits(int* p, int mask, hipStream_t s);
 hipError_t launch_amr_begin(const int* active, int noct, const AmrTreeView& t, const double* u, double* unew,
                             long ustride, double dx, const HydroParams& hp, double* d_dt, hipStream_t s,
@@ -480,7 +483,19 @@ struct StepCtx {
   ramses_amr_mesh* m;
   hipStream_t s;
   Config cfg;
+  bool cons_armed;  // nstep_coarse changed since the last update_time
+  bool cons_done;   // the step's first update_time has been passed
 };
+
+// The grid lists and host counts of every level.
+ramses::ConsLevels cons_levels(const ramses_amr_mesh* m) {
+  ramses::ConsLevels lv{};
+  for (int l = 1; l <= m->nlevelmax && l < ramses::kConsMaxLevels; ++l) {
+    lv.active[l] = m->active[l];
+    lv.noct[l] = m->noct[l];
+  }
+  return lv;
+}
 
 ramses_amr_level level_of(const ramses_amr_mesh* m, int l) {
   ramses_amr_level lv;
@@ -549,6 +564,13 @@ int amr_step_rec(StepCtx& c, int l, int icount) {
     e = ramses::launch_amr_dt(1, m->dt, m->dtold, m->time, l, l < m->nlevelmax, m->nsubcycle[l], 0, 0.0, c.s);
     if (e != hipSuccess) return hip_fail(e, "dt launch");
     m->nstep += 1;
+    // update_time's conservation baselines (amr/update_time.f90:229-246)
+    if (m->cons0 && !c.cons_done && c.cons_armed) {
+      e = ramses::launch_amr_cons0(cons_levels(m), nullptr, v, m->uold, v.ncell + 1, m->levelmin, m->nlevelmax,
+                                   m->boxlen, c.cfg.nvector, nullptr, m->cons0, c.s);
+      if (e != hipSuccess) return hip_fail(e, "conservation baselines launch");
+    }
+    c.cons_done = true;
   }
   lv = level_of(m, l);  // the finer levels' refinements changed noct(l+1) only
   if ((rc = ensure_work(m, amr_work(lv.noct, current().nvector), c.s))) return rc;
@@ -581,7 +603,7 @@ struct AmrDevState {
   int free[4];       // head, numbf, used_mem
   int ctrl[4];       // refine decision: ncreate, nkill, ok
   int error;         // *m->error after the step
-  int pad;
+  int cons_armed;    // nstep_coarse changed since the last update_time
   long long cnt[2];  // nstep, cell_updates
 };
 
@@ -627,7 +649,8 @@ void graph_key(const ramses_amr_mesh* m, int capacity, std::vector<long>& k) {
   }
   for (const void* q : {(const void*)m->pos, (const void*)m->xg, (const void*)m->flag1, (const void*)m->flag2,
                         (const void*)m->free.ring, (const void*)m->uold, (const void*)m->unew, (const void*)m->dt,
-                        (const void*)m->dtold, (const void*)m->time, (const void*)m->error, (const void*)m->sums})
+                        (const void*)m->dtold, (const void*)m->time, (const void*)m->error, (const void*)m->sums,
+                        (const void*)m->cons0})
     P(q);
   long bl;
   std::memcpy(&bl, &m->boxlen, sizeof(bl));
@@ -645,6 +668,7 @@ struct GCtx {
   int capacity;
   ramses::AmrTreeView v;
   ramses::TreeMut tm;
+  bool cons_done = false;  // the baselines' launch has been emitted
 };
 
 // amr_step(l, icount) as a fixed launch sequence: every level step down to
@@ -686,6 +710,17 @@ hipError_t gstep(GCtx& g, int l, int icount) {
   e = ramses::launch_amr_dt(1, m->dt, m->dtold, m->time, l, l < m->nlevelmax, m->nsubcycle[l], 0, 0.0, g.s, dn,
                             g.dev->cnt);
   if (e != hipSuccess) return e;
+  // update_time's conservation baselines: emitted after the first update_time
+  // of the capture order (the deepest level's).  At run time the step's
+  // first update_time may come at a shallower level, but every launch
+  // between the two belongs to an empty level and does nothing, and the
+  // kernel finds the levels of the first descent from the device counts.
+  if (m->cons0 && !g.cons_done) {
+    g.cons_done = true;
+    e = ramses::launch_amr_cons0(cons_levels(m), dn, g.v, m->uold, us, m->levelmin, m->nlevelmax, m->boxlen,
+                                 g.c.nvector, &g.dev->cons_armed, m->cons0, g.s);
+    if (e != hipSuccess) return e;
+  }
   ramses::AmrLevelArgs a;
   a.t = g.v;
   a.uold = m->uold;
@@ -830,6 +865,7 @@ int coarse_step_graph(ramses_amr_mesh* m, hipStream_t s) {
   h->free[2] = m->free.used_mem;
   h->cnt[0] = m->nstep;
   h->cnt[1] = m->cell_updates;
+  h->cons_armed = m->nstep_coarse != m->nstep_coarse_old;
   if ((e = hipMemcpyAsync(gs->dev, h, sizeof(AmrDevState), hipMemcpyHostToDevice, s)) != hipSuccess)
     return hip_fail(e, "graph state in");
   if ((e = hipGraphLaunch(gs->exec, s)) != hipSuccess) return hip_fail(e, "graph launch");
@@ -844,6 +880,7 @@ int coarse_step_graph(ramses_amr_mesh* m, hipStream_t s) {
   m->free.used_mem = h->free[2];
   m->nstep = h->cnt[0];
   m->cell_updates = h->cnt[1];
+  m->nstep_coarse_old = m->nstep_coarse;  // update_time's bookkeeping (amr/update_time.f90:310)
   m->nstep_coarse += 1;
   if (h->error & 4) return fail(-9, "No more free memory: increase ngridmax");
   if (h->error & 8) return fail(-10, "captured mode: a refinement exceeded graph_capacity");
@@ -873,7 +910,7 @@ int ramses_amr_coarse_step(ramses_amr_mesh* m, void* stream) {
       return fail(-1, "free list head/numbf out of range");
   }
   if (m->graph) return coarse_step_graph(m, (hipStream_t)stream);
-  StepCtx c{m, (hipStream_t)stream, current()};
+  StepCtx c{m, (hipStream_t)stream, current(), m->nstep_coarse != m->nstep_coarse_old, false};
   // adaptive_loop resets the totals at every coarse step (amr/adaptive_loop.f90:89-91)
   if (m->sums) {
     hipError_t e = ramses::launch_set_doubles(m->sums, 3, 0.0, c.s);
@@ -881,6 +918,7 @@ int ramses_amr_coarse_step(ramses_amr_mesh* m, void* stream) {
   }
   int rc = amr_step_rec(c, m->levelmin, 1);
   if (rc) return rc;
+  m->nstep_coarse_old = m->nstep_coarse;  // update_time's bookkeeping (amr/update_time.f90:310)
   m->nstep_coarse += 1;
   if (m->adaptive && m->levelmin < m->nlevelmax) {
     for (int l = m->levelmin - 1; l >= 1; --l) {

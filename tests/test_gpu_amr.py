@@ -771,7 +771,9 @@ def test_coarse_step_conservation_sums_vs_reference(gpu, graph):
     amr/adaptive_loop.f90:89-91 does) agree with those of the reference's
     own coarse step to 1e-12;
     computing them leaves the mesh and state bitwise equal to the
-    reference's, and the totals are the same bits on a second run."""
+    reference's, and the totals are the same bits on a second run.
+    update_time's baselines einit and mass_tot_0, which the snapshots
+    carry, are the reference's bits from the second step on."""
     import ramses_amd as ra
     from amr_cases import reference_driver_step
     runs = []
@@ -793,6 +795,11 @@ def test_coarse_step_conservation_sums_vs_reference(gpu, graph):
             mine = s.sums.cpu().numpy()
             np.testing.assert_allclose(mine, stats["sums"], rtol=1e-12, atol=0)
             got.append(mine.copy())
+            # update_time's baselines: set at the second step's first
+            # update_time, summed in the reference's order -- the same bits
+            b = A.baselines()
+            assert np.array_equal(s.cons0.cpu().numpy(), b), step
+            assert (b[0] != 0.0) == (step >= 1), step
         assert np.array_equal(s.state("old"), A.get(0))
         runs.append(got)
         del s
