@@ -79,6 +79,19 @@ void unsplit_(const double *uin, const double *gravin, const double *pin, double
 int ramses_unsplit(const double *uin, const double *gravin, double *flux, double *tmp, double dx,
                    double dy, double dz, double dt, int ngrid);
 
+/* ramses_unsplit without any device allocation; it has
+ * no counterpart in the reference.  For callers that cannot allocate device memory (tight
+ * per-batch loops such as godfine1's, harnesses that forbid hipMalloc).
+ * Same arguments and results as ramses_unsplit, enqueued on `stream`
+ * (hipStream_t, NULL = default stream) and synchronised before it returns.
+ * The batch is staged in a fixed 2 MiB device array of the library (one per
+ * GPU, held until the process ends), so nvector <= 114; a larger configured
+ * nvector returns -11.  Calls on one GPU share that array: the library
+ * serialises them within a process, and every value the kernel reads is
+ * copied in by the call itself. */
+int ramses_unsplit_noalloc(const double *uin, const double *gravin, double *flux, double *tmp, double dx,
+                           double dy, double dz, double dt, int ngrid, void *stream);
+
 /* Device-resident batched form: `ngroups` consecutive unsplit array sets in
  * device memory (each laid out exactly like one call above, group g at
  * offset g*nvector*216*5 for uin, etc.), every group full except the last

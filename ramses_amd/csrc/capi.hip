@@ -248,6 +248,74 @@ int ramses_unsplit(const double* uin, const double* gravin, double* flux, double
   return 0;
 }
 
+}  // extern "C"
+
+// Staging of ramses_unsplit_noalloc: device memory the library never
+// allocates.  One call stages uin, gravin, flux and tmp of one batch, 2295
+// doubles per lane of nvector, so the 2 MiB array serves nvector <= 114.
+// Each GPU the library runs on holds its own copy until the process ends.
+constexpr size_t kNoAllocDoubles = 262144;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+namespace {
+constexpr int kNoAllocMaxDevices = 64;
+struct NoAllocStaging {
+  std::mutex mu;
+  // Device address of `scratch` on each device, taken once per device by
+  // hipGetSymbolAddress with that device current.  Host code uses only
+  // these addresses: the name `scratch` in host code is a host-side stand-in.
+  double* ptr[kNoAllocMaxDevices] = {};
+};
+NoAllocStaging g_noalloc;
+}  // namespace
+
+extern "C" {
+
+int ramses_unsplit_noalloc(const double* uin, const double* gravin, double* flux, double* tmp, double dx, double dy,
+                           double dz, double dt, int ngrid, void* stream) {
+  const Config c = current();
+  if (!uin || !flux || !tmp) return fail(-1, "null array");
+  if (ngrid < 0 || ngrid > c.nvector) return fail(-1, "ngrid out of [0, nvector]");
+  const size_t nv = (size_t)c.nvector;
+  const size_t nu = nv * 216 * 5, ng = nv * 216 * 3, nf = nv * 27 * 5 * 3, nt = nv * 27 * 2 * 3;
+  if (nu + ng + nf + nt > kNoAllocDoubles)
+    return fail(-11, "ramses_unsplit_noalloc: nvector " + std::to_string(nv) + " needs " +
+                         std::to_string((nu + ng + nf + nt) * 8) + " bytes of staging, more than its fixed " +
+                         std::to_string(kNoAllocDoubles * 8) + " (nvector <= 114)");
+  if (ngrid == 0) return 0;
+  int dev = 0;
+  hipError_t e;
+  if ((e = hipGetDevice(&dev)) != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev < 0 || dev >= kNoAllocMaxDevices) return fail(-1, "ramses_unsplit_noalloc: device index beyond 63");
+  // One call at a time per process: every call on a device shares its
+  // staging, and the call returns only after its results reached the host.
+  std::lock_guard<std::mutex> lk(g_noalloc.mu);
+  if (!g_noalloc.ptr[dev]) {
+    void* p = nullptr;
+    if ((e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch))) != hipSuccess) return hip_fail(e, "staging address");
+    g_noalloc.ptr[dev] = static_cast<double*>(p);
+  }
+  double* d_u = g_noalloc.ptr[dev];
+  double* d_g = d_u + nu;
+  double* d_f = d_g + ng;
+  double* d_t = d_f + nf;
+  hipStream_t s = (hipStream_t)stream;
+  // flux/tmp travel both ways so that entries the reference never writes
+  // keep the caller's values; the staging keeps the previous call's data,
+  // and every word the kernel reads is copied in first
+  if ((e = hipMemcpyAsync(d_u, uin, nu * 8, hipMemcpyHostToDevice, s)) != hipSuccess) return hip_fail(e, "H2D uin");
+  if (gravin && (e = hipMemcpyAsync(d_g, gravin, ng * 8, hipMemcpyHostToDevice, s)) != hipSuccess)
+    return hip_fail(e, "H2D gravin");
+  if ((e = hipMemcpyAsync(d_f, flux, nf * 8, hipMemcpyHostToDevice, s)) != hipSuccess) return hip_fail(e, "H2D flux");
+  if ((e = hipMemcpyAsync(d_t, tmp, nt * 8, hipMemcpyHostToDevice, s)) != hipSuccess) return hip_fail(e, "H2D tmp");
+  int rc = ramses_unsplit_device(d_u, gravin ? d_g : nullptr, d_f, d_t, dx, dy, dz, dt, 1, ngrid, s);
+  if (rc) return rc;
+  if ((e = hipMemcpyAsync(flux, d_f, nf * 8, hipMemcpyDeviceToHost, s)) != hipSuccess) return hip_fail(e, "D2H flux");
+  if ((e = hipMemcpyAsync(tmp, d_t, nt * 8, hipMemcpyDeviceToHost, s)) != hipSuccess) return hip_fail(e, "D2H tmp");
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "unsplit sync");
+  return 0;
+}
+
 void unsplit_(const double* uin, const double* gravin, const double* pin, double* flux, double* tmp,
               const double* dx, const double* dy, const double* dz, const double* dt, const int* ngrid) {
   (void)pin;  // declared but never read by the reference body (hydro/umuscl.f90:32)

@@ -18,12 +18,13 @@ def test_library_exports_header():
 
 
 def test_header_cites_reference():
-    """Every entry point's comment block names the reference file it replaces."""
+    """Every entry point's comment block names the reference file it
+    replaces, or says that it has none."""
     text = open(ramses_amd.HEADER).read()
     for n in ramses_amd.header_functions():
         i = text.index(n + "(")
         block = text[max(0, i - 900):i]
-        assert ".f90:" in block, n
+        assert ".f90:" in block or "no counterpart in the reference" in block, n
 
 
 @pytest.mark.parametrize("bad", [dict(slope_type=4), dict(slope_type=5), dict(riemann_code=7), dict(difmag=0.1),
