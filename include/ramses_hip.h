@@ -215,8 +215,10 @@ typedef struct ramses_amr_level {
   int noct_coarse;
 } ramses_amr_level;
 
-/* interpol_type (0-3) and interpol_var (0-2) of the buffer-oct
+/* interpol_type (0-4) and interpol_var (0-2) of the buffer-oct
  * prolongation, hydro/hydro_parameters.f90:88-89 (defaults 1 and 0).
+ * Type 4 requires interpol_var 2, as the reference does
+ * (hydro/interpol_hydro.f90:367-377); other pairs with type 4 return -2.
  * interpol_var also selects what the restriction averages (upl,
  * hydro/interpol_hydro.f90:226-261): total energy for 0, internal energy
  * for 1 and 2. */

@@ -144,7 +144,10 @@ __device__ inline void interp_slopes(const double a[7], int itype, double w[3]) 
 }
 
 // interpol_hydro for one buffer oct: u1[7][5] -> u2[8][5]
-// (hydro/interpol_hydro.f90:268-444; interpol_var 0, 1, 2; types 0-3)
+// (hydro/interpol_hydro.f90:268-444; interpol_var 0, 1, 2; types 0-4).
+// Type 4 (with interpol_var 2 only) takes the unlimited central slope for
+// the velocities and the monotonized central one for density and internal
+// energy (:367-377).
 __device__ inline void interpol_oct(double u1[7][5], double u2[8][5], int itype, int ivar_mode, double smallr) {
   if (ivar_mode == 1 || ivar_mode == 2) {
     for (int j = 0; j < 7; ++j) {
@@ -159,7 +162,8 @@ __device__ inline void interpol_oct(double u1[7][5], double u2[8][5], int itype,
   for (int v = 0; v < 5; ++v) {
     double a[7], w[3] = {0.0, 0.0, 0.0};
     for (int j = 0; j < 7; ++j) a[j] = u1[j][v];
-    if (itype >= 1 && itype <= 3) interp_slopes(a, itype, w);
+    const int t = itype == 4 ? ((v >= 1 && v <= 3) ? 3 : 2) : itype;
+    if (t >= 1 && t <= 3) interp_slopes(a, t, w);
     for (int ind = 0; ind < 8; ++ind) {
       double s = a[0];
       for (int d = 0; d < 3; ++d) s = s + w[d] * ((double)((ind >> d) & 1) - 0.5);

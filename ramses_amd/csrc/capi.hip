@@ -376,8 +376,10 @@ size_t amr_work(int noct, int nv) {
 }  // namespace
 
 int ramses_amr_config(int interpol_type, int interpol_var) {
-  if (interpol_type < 0 || interpol_type > 3) return fail(-2, "interpol_type must be 0..3");
+  if (interpol_type < 0 || interpol_type > 4) return fail(-2, "interpol_type must be 0..4");
   if (interpol_var < 0 || interpol_var > 2) return fail(-2, "interpol_var must be 0..2");
+  // the reference stops on this pair (hydro/interpol_hydro.f90:369-372)
+  if (interpol_type == 4 && interpol_var != 2) return fail(-2, "interpol_type=4 is designed for interpol_var=2");
   std::lock_guard<std::mutex> lk(g_cfg_mu);
   g_cfg.interpol_type = interpol_type;
   g_cfg.interpol_var = interpol_var;

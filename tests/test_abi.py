@@ -198,3 +198,18 @@ def test_refinement_refuses_nvector_other_than_32():
         assert rc == -8 and "nvector = 32" in lib.ramses_last_error().decode()
     finally:
         ramses_amd.configure()
+
+
+def test_interpolation_config_validates():
+    """ramses_amr_config accepts interpol_type 0..4 and interpol_var 0..2,
+    and refuses type 4 with any interpol_var but 2, where the reference
+    stops (hydro/interpol_hydro.f90:367-372)."""
+    from ramses_amd import amr
+    lib = ramses_amd.load_library()
+    amr._bind(lib)
+    for t, v in [(0, 0), (1, 1), (2, 2), (3, 0), (4, 2)]:
+        assert lib.ramses_amr_config(t, v) == 0, (t, v)
+    for t, v in [(4, 0), (4, 1), (5, 2), (-1, 0), (1, 3)]:
+        assert lib.ramses_amr_config(t, v) < 0, (t, v)
+        assert len(lib.ramses_last_error()) > 0
+    assert lib.ramses_amr_config(1, 0) == 0

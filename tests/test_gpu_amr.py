@@ -89,7 +89,59 @@ def test_amr_sweeps_vs_reference(gpu, riemann, slope):
 
 
 @needs_ref
-@pytest.mark.parametrize("itype,ivar", [(0, 0), (2, 0), (3, 0), (1, 1), (2, 2), (3, 2)])
+@pytest.mark.parametrize("which,riemann,slope", [("dropin", "llf", 2), ("dropin", "hllc", 1),
+                                                 ("patched", "hllc", 1)])
+def test_reference_godunov_through_library_unsplit(gpu, which, riemann, slope):
+    """The drop-in end to end: the reference's own godunov_fine
+    (hydro/godunov_fine.f90), built without umuscl.o so that godfine1's
+    `call unsplit` (:681) reaches this library -- through its link-level
+    symbol unsplit_ (dropin), or through the ISO_C_BINDING patch file of
+    INTEGRATION.md section 3 forwarding to ramses_unsplit (patched, compiled
+    from that document by oracle/Makefile) -- gives the all-reference
+    build's unew bit for bit on levels 3-5 of the sphere tree, coarse/fine
+    buffer octs included.  The patch configures the library once per
+    process from the reference's module globals, so it runs one case."""
+    import ramses_amd as ra
+    path = os.path.join(os.path.dirname(oracle.REF_LIB), f"libramses_ref_{which}.so")
+    assert os.path.exists(path), f"{path} is missing (make -C oracle dropin)"
+    ra.load_library()  # the HIP library (and torch's runtime) before the drop-in build that needs it
+    # the host-memory unsplit the drop-ins reach must work here first: on a
+    # failure the link-level symbol aborts and the patch stops the process
+    ra.configure(gamma=5 / 3, riemann=riemann, slope_type=slope, courant_factor=0.5)
+    uin = np.zeros(ra.stencil_shape(32))
+    uin[0] = 1.0
+    uin[4] = 1.0
+    ra.unsplit(uin, None, np.zeros(ra.flux_shape(32)), np.zeros(ra.tmp_shape(32)), 0.1, 0.1, 0.1, 1e-3, 32)
+    tree = sphere_tree()
+    u = smooth_state(tree, seed=11)
+    out = {}
+    for name, lib in (("reference", oracle.REF_LIB), (which, path)):
+        R = oracle.Reference(lib)
+        A = oracle.ReferenceAMR(R)
+        A.install(tree)
+        A.set_interpol(1, 0)
+        R.set_params(gamma=5 / 3, riemann=riemann, slope_type=slope, courant_factor=0.5)
+        ra.configure(gamma=5 / 3, riemann=riemann, slope_type=slope, courant_factor=0.5)
+        res = []
+        for l in (3, 4, 5):
+            A.put(0, u)
+            A.put(1, u)
+            dt = A.courant(l, 1e10)[0]
+            if l > 3:
+                A.set_unew(l - 1)
+            A.set_unew(l)
+            A.godunov(l, dt)
+            res.append((dt, A.get(1)))
+        out[name] = res
+    for l, (a, b) in zip((3, 4, 5), zip(out["reference"], out[which])):
+        assert a[0] == b[0], (which, l)
+        assert np.array_equal(a[1], b[1]), (which, l)
+    assert not np.array_equal(out["reference"][0][1], u)
+    ra.configure()
+
+
+@needs_ref
+@pytest.mark.parametrize("itype,ivar", [(0, 0), (2, 0), (3, 0), (1, 1), (2, 2), (3, 2), (4, 2)])
 def test_amr_interpolation_vs_reference(gpu, itype, ivar):
     import ramses_amd as ra
     from ramses_amd.amr import configure_interpolation
@@ -395,6 +447,76 @@ def test_sedov_amr_init_and_steps_vs_reference(gpu, graph):
 
 
 @needs_ref
+def test_sedov_amr_levels_6_10_vs_reference(gpu):
+    """The adaptive Sedov workload at its own size: levelmin 6, levelmax 10,
+    err_grad_d = err_grad_p = 0.1, interpol_type 1, gamma 1.4, hllc, minmod,
+    CFL 0.8, nsubcycle 2 -- init_refine, then four coarse steps (64 finest
+    steps) in direct and captured mode against the reference's own
+    init_refine sequence and coarse step: tree, grid lists, free list,
+    refinement map, state, grid centres, level dts, t and nstep, bitwise
+    after every step."""
+    import ramses_amd as ra
+    from amr_cases import mesh_digests, reference_driver_step, reference_init_refine
+    from ramses_amd.amr import AmrSolver, configure_interpolation, configure_refinement
+    from ramses_amd.ics import sedov_regions
+    from ramses_amd.tree import OctTree
+    lmin, L, ngm = 6, 10, 120_000
+    tree = OctTree(lmin, lmin, extra_grids=ngm - sum(8 ** k for k in range(lmin)))
+    ra.configure(gamma=1.4, riemann="hllc", slope_type=1, courant_factor=0.8)
+    configure_interpolation(1, 0)
+    configure_refinement(err_grad_d=0.1, err_grad_p=0.1)
+
+    def snap_native(s):
+        son, father, nbor, flag1 = s.tree_arrays()
+        return mesh_digests(son, father, nbor, flag1, {l: s.grids(l) for l in range(1, L + 1)}, s.free_list(),
+                            s.state("old"), s.state("new"))
+
+    native = {}
+    for graph in (False, True):
+        s = AmrSolver(tree, None, refine=True, nlevelmax=L, graph=graph)
+        s.init_refine(lambda x, dx: sedov_regions(x, dx, gamma=1.4))
+        seq = [(snap_native(s), s.xg[1:].cpu().numpy().copy())]
+        for _ in range(4):
+            s.coarse_step()
+            s.check()
+            seq.append((snap_native(s), None, s.dt.cpu().numpy()[:L + 1].copy(), float(s.time.item()),
+                        s.fine_steps))
+        assert s.fine_steps == 64 and s.noct(L) > 0
+        native[graph] = seq
+        del s
+    R = oracle.Reference()
+    A = oracle.ReferenceAMR(R)
+    R.set_params(gamma=1.4, riemann="hllc", slope_type=1, courant_factor=0.8)
+    A.install(tree, nlevelmax=L)
+    A.set_interpol(1, 0)
+    A.set_refine(lmin, err_grad_d=0.1, err_grad_p=0.1)
+    A.set_xg(native[False][0][1])
+    A.set_sedov(1.0, 1e-5, 0.4, (0.5, 0.5, 0.5))
+    reference_init_refine(A, lmin, L)
+
+    def snap_ref():
+        son, father, nbor, flag1 = A.tree_arrays()
+        return mesh_digests(son, father, nbor, flag1, {l: A.grids(l) for l in range(1, L + 1)}, A.free_list(),
+                            A.get(0), A.get(1))
+    for graph in (False, True):
+        assert native[graph][0][0] == snap_ref(), ("init", graph)
+    A.flag_coarse()
+    dt, dtold, clock = [0.0] * (L + 2), [0.0] * (L + 2), {}
+    for k in range(1, 5):
+        reference_driver_step(A, dt, dtold, clock=clock)
+        ref = snap_ref()
+        for graph in (False, True):
+            snap, _, got_dt, t, nstep = native[graph][k]
+            assert snap == ref, (k, graph)
+            assert (t, nstep) == (clock["t"], clock["nstep"]), (k, graph)
+            for l in range(lmin, L + 1):
+                assert got_dt[l] == dt[l], (k, l, graph)
+    A.install(OctTree(1, 1), nlevelmax=1)  # release the reference's arrays
+    configure_refinement()
+    ra.configure()
+
+
+@needs_ref
 def test_snapshot_after_adaptive_steps_vs_reference(gpu):
     """Two adaptive coarse steps, then dump_all on both sides: the GPU
     defrag + snapshot writer give the reference's files byte for byte, and
@@ -593,7 +715,7 @@ def test_namelist_run_matches_reference(gpu, graph):
 
 
 @needs_ref
-@pytest.mark.parametrize("itype,ivar", [(0, 0), (2, 1), (3, 2)])
+@pytest.mark.parametrize("itype,ivar", [(0, 0), (2, 1), (3, 2), (4, 2)])
 def test_refine_interpolation_variants_vs_reference(gpu, itype, ivar):
     """make_grid_fine prolongates new octs with interpol_hydro: every
     interpolation type / variable choice gives the reference's new cells,

@@ -43,6 +43,38 @@ def test_random_grid_vs_oracle(gpu, oracle_lib, riemann, slope):
         assert np.array_equal(lvl.u.cpu().numpy(), ref), (riemann, slope, s)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_every_tile_shape_same_bits(gpu, oracle_lib, precision):
+    """Every launch shape the autotuner may pick (tiles 32x8, 16x8, 8x8 and
+    32x16 with z chunks of 4..64 planes) gives the same bits: equal to the
+    oracle in FP64, and to each other in FP32-mixed."""
+    from ramses_amd import autotune
+    rng = np.random.default_rng(9)
+    n = 64
+    u = random_grid(rng, n, mach=2.0)
+    ra.configure(gamma=5 / 3, riemann="hllc", slope_type=1, courant_factor=0.5, precision=precision)
+    oracle_lib.set_params(gamma=5 / 3, riemann="hllc", slope_type=1, courant_factor=0.5)
+    ref = None
+    if precision == "fp64":
+        ref, dt_ref = oracle_lib.uniform_step(u, 1.0 / n)
+    try:
+        for tile in (1, 2, 3, 4):
+            for tz in (4, 16, 64):
+                autotune._set(autotune.LEVEL_TILE, tile)
+                autotune._set(autotune.LEVEL_TZ, tz)
+                lvl = ra.UniformLevel(u)
+                dt = float(lvl.step())
+                out = lvl.u.cpu().numpy()
+                if ref is None:
+                    ref, dt_ref = out, dt
+                assert dt == dt_ref, (tile, tz)
+                assert np.array_equal(out, ref), (precision, tile, tz)
+    finally:
+        autotune._set(autotune.LEVEL_TILE, 0)
+        autotune._set(autotune.LEVEL_TZ, 0)
+        ra.configure()
+
+
 def test_conservation_and_determinism(gpu):
     rng = np.random.default_rng(5)
     n = 64
