@@ -661,6 +661,43 @@ __device__ __forceinline__ double cmpdt_w(double u0, double u1, double u2, doubl
   return w;
 }
 
+// cmpdt_w of the FP32-mixed path: the same FP64 expression with the
+// divisions and the square root to about 1 ulp (hardware reciprocal and
+// reciprocal square root refined by Newton steps) instead of correctly
+// rounded.  The dt stays an FP64 reduction; only the FP64 path needs the
+// reference's bits.
+__device__ __forceinline__ double sqrt_fast(double x) {  // x > 0, normal
+  const double y = __builtin_amdgcn_rsq(x);
+  double s = x * y;
+  double h = 0.5 * y;
+  double r = __builtin_fma(-s, h, 0.5);
+  s = __builtin_fma(s, r, s);
+  h = __builtin_fma(h, r, h);
+  r = __builtin_fma(-s, s, x);
+  return __builtin_fma(r, h, s);
+}
+
+__device__ __forceinline__ double cmpdt_w_fast(double u0, double u1, double u2, double u3, double u4,
+                                               const HydroParams& hp) {
+  const double gamma = hp.gamma;
+  const double smallp = hp.smallc * hp.smallc / gamma;
+  const double rho = fmax_f(u0, hp.smallr);
+  const double ir = rcp_fast(rho);
+  const double vx = u1 * ir, vy = u2 * ir, vz = u3 * ir;
+  double e = u4;
+  e = e - 0.5 * rho * (vx * vx);
+  e = e - 0.5 * rho * (vy * vy);
+  e = e - 0.5 * rho * (vz * vz);
+  double c = fmax_f((gamma - 1.0) * e, rho * smallp);
+  c = gamma * c;
+  c = sqrt_fast(c * ir);
+  double w = 3.0 * c;
+  w = w + __builtin_fabs(vx);
+  w = w + __builtin_fabs(vy);
+  w = w + __builtin_fabs(vz);
+  return w;
+}
+
 __device__ __forceinline__ double cmpdt_from_w(double w, double dx, const HydroParams& hp) {
   double r = 0.0;
   r = r * dx / (w * w);

@@ -169,10 +169,21 @@ __device__ __forceinline__ void level_tile(const double* __restrict__ u, double*
   for (int p = z0 - 1; p <= z0 + TZ; ++p) {
     const int scur = (p - z0 + 2) & 1;  // slot of plane p (z0-2 -> 0, z0-1 -> 1)
     const int snext = scur ^ 1;
+    const bool faces_xy = (p >= z0) && (p < z0 + TZ);
+    // FP32-mixed: the own cell's conserved state for the x/y update is
+    // loaded before the next plane, so its latency hides behind the plane's
+    // work (in FP64 the 10 more live registers would spill)
+    double upre[5] = {0, 0, 0, 0, 0};
+    if constexpr (sizeof(R) == 4) {
+      if (interior && faces_xy) {
+        const long o = (long)wrapi(p, n) * plane + (long)yg * n + xg;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) upre[v] = u[o + v * vstride];
+      }
+    }
     __syncthreads();
     load_plane(p + 1, snext);
     __syncthreads();
-    const bool faces_xy = (p >= z0) && (p < z0 + TZ);
     Prim<R> qnext{};
     Prim<R> qp_x{}, qm_x{}, qp_y{}, qm_y{}, qp_z{}, qm_z{};
     if (active) {
@@ -214,30 +225,53 @@ __device__ __forceinline__ void level_tile(const double* __restrict__ u, double*
         for (int v = 0; v < 5; ++v) fys[v] = scale_flux(fc[v], fs);  // :129
       }
       __syncthreads();
-      // fluxes are stored in double in the exchange area reinterpreted per variable pair
+      // The fluxes, in double, go through the exchange area reinterpreted:
+      // in FP64 it holds the x and the y fluxes side by side (one exchange),
+      // in FP32 one direction at a time.
       double* fx_sh = reinterpret_cast<double*>(&xch[0][0]);
       constexpr int FSTRIDE = NE;  // doubles per variable
       static_assert(sizeof(R) * 10 * NE >= sizeof(double) * 5 * NE, "exchange area too small");
-      if (interior || kind == kRight) {
+      if constexpr (sizeof(R) == 8) {
+        double* fy_sh = fx_sh + 5 * FSTRIDE;
+        if (interior || kind == kRight) {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) fx_sh[v * FSTRIDE + e] = fxs[v];
-      }
-      __syncthreads();
-      if (interior) {
-        // hydro/godunov_fine.f90:770-772, idim=1: unew = uold + (flux(i) - flux(i+1))
-        const long o = (long)wrapi(p, n) * plane + (long)yg * n + xg;
+          for (int v = 0; v < 5; ++v) fx_sh[v * FSTRIDE + e] = fxs[v];
+        }
+        if (interior || kind == kTop) {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) part[v] = u[o + v * vstride] + (fx_sh[v * FSTRIDE + e] - fx_sh[v * FSTRIDE + e + 1]);
-      }
-      __syncthreads();
-      if (interior || kind == kTop) {
+          for (int v = 0; v < 5; ++v) fy_sh[v * FSTRIDE + e] = fys[v];
+        }
+        __syncthreads();
+        if (interior) {
+          // hydro/godunov_fine.f90:770-772: unew = uold + (flux(i) - flux(i+1)), idim=1 then 2
+          const long o = (long)wrapi(p, n) * plane + (long)yg * n + xg;
 #pragma unroll
-        for (int v = 0; v < 5; ++v) fx_sh[v * FSTRIDE + e] = fys[v];
-      }
-      __syncthreads();
-      if (interior) {
+          for (int v = 0; v < 5; ++v)
+            part[v] = u[o + v * vstride] + (fx_sh[v * FSTRIDE + e] - fx_sh[v * FSTRIDE + e + 1]);
 #pragma unroll
-        for (int v = 0; v < 5; ++v) part[v] = part[v] + (fx_sh[v * FSTRIDE + e] - fx_sh[v * FSTRIDE + e + EX]);  // idim=2
+          for (int v = 0; v < 5; ++v) part[v] = part[v] + (fy_sh[v * FSTRIDE + e] - fy_sh[v * FSTRIDE + e + EX]);
+        }
+      } else {
+        if (interior || kind == kRight) {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) fx_sh[v * FSTRIDE + e] = fxs[v];
+        }
+        __syncthreads();
+        if (interior) {
+          // hydro/godunov_fine.f90:770-772, idim=1: unew = uold + (flux(i) - flux(i+1))
+#pragma unroll
+          for (int v = 0; v < 5; ++v) part[v] = upre[v] + (fx_sh[v * FSTRIDE + e] - fx_sh[v * FSTRIDE + e + 1]);
+        }
+        __syncthreads();
+        if (interior || kind == kTop) {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) fx_sh[v * FSTRIDE + e] = fys[v];
+        }
+        __syncthreads();
+        if (interior) {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) part[v] = part[v] + (fx_sh[v * FSTRIDE + e] - fx_sh[v * FSTRIDE + e + EX]);  // idim=2
+        }
       }
     }
     // z face below plane p (between p-1 and p)
@@ -260,7 +294,10 @@ __device__ __forceinline__ void level_tile(const double* __restrict__ u, double*
         un[v] = s;
         unew[o + v * vstride] = s;
       }
-      if (d_dt_next) wmax = fmax_f(wmax, cmpdt_w(un[0], un[1], un[2], un[3], un[4], hp));
+      if (d_dt_next) {
+        if constexpr (sizeof(R) == 8) wmax = fmax_f(wmax, cmpdt_w(un[0], un[1], un[2], un[3], un[4], hp));
+        else wmax = fmax_f(wmax, cmpdt_w_fast(un[0], un[1], un[2], un[3], un[4], hp));
+      }
       if constexpr (SUMS) cons_add(cs, un[0], un[1], un[2], un[3], un[4], vol, hp.smallr);
     }
     // carry to the next plane
