@@ -130,7 +130,12 @@ __device__ __forceinline__ void level_tile(const double* __restrict__ u, double*
   // cooperative ctoprim of plane zp (periodic) into slot s
   auto load_plane = [&](int zp, int s) {
     const int zg = wrapi(zp, n);
-    for (int c = t; c < NQ; c += NT) {
+    static_assert(NQ <= 2 * NT, "a plane is at most two cells per thread");
+    // cell t for every thread; the cells beyond NT go to the last threads
+    // (the ring's wave, which solves fewer faces), in reverse order
+    for (int k = 0; k < 2; ++k) {
+      const int c = k == 0 ? t : 2 * NT - 1 - t;
+      if (c >= NQ) continue;
       const int qx = c % QX, qy = c / QX;
       const int gx = wrapi(x0 - 2 + qx, n), gy = wrapi(y0 - 2 + qy, n);
       const long o = (long)zg * plane + (long)gy * n + gx;
@@ -196,7 +201,7 @@ __device__ __forceinline__ void level_tile(const double* __restrict__ u, double*
       const Source<R> src = trace_sources(qcur, dqx, dqy, dqz, sc.gamma);
       trace_dir(qcur, dqx, src, dtdx, sc.smallr, qp_x, qm_x);
       trace_dir(qcur, dqy, src, dtdx, sc.smallr, qp_y, qm_y);
-      trace_dir(qcur, dqz, src, dtdx, sc.smallr, qp_z, qm_z);
+      if (interior) trace_dir(qcur, dqz, src, dtdx, sc.smallr, qp_z, qm_z);  // ring cells solve no z face
     }
     double part[5] = {0, 0, 0, 0, 0};
     if (faces_xy) {
