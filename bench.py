@@ -73,7 +73,9 @@ def time_variant(torch, ra, uin, flux, tmp, dt, riemann, precision, steps, warmu
     ra.configure(gamma=GAMMA, riemann=riemann, slope_type=1, courant_factor=CFL, precision=precision)
     stream = torch.cuda.current_stream()
     tuned = None
-    if not tune:
+    if precision == "fp32":  # nvector 32 runs the LDS-staged kernel: fixed 192-thread workgroups
+        tuned = (192, None)
+    elif not tune:
         autotune._set(autotune.UNSPLIT_BLOCK, block)
         tuned = (block or 256, None)
     else:  # untimed: pick the workgroup size on these inputs

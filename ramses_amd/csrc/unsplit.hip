@@ -17,6 +17,11 @@
 // the oct depends on (the 6^3 corners) are never read.  Because every value
 // is a deterministic function of its inputs, computing it once per row
 // instead of once per 6^3 sweep gives the reference's values bit for bit.
+//
+// FP32-mixed with nvector 32 stages instead: a workgroup takes half a batch
+// (16 octs x 12 rows), converts the 80 stencil cells its faces read to
+// primitives once, into LDS, and its rows read them from there.  A row then
+// evaluates no primitive itself, against 22 in the row kernel.
 #include <type_traits>
 
 #include "hydro_device.hpp"
@@ -43,29 +48,132 @@ __device__ __forceinline__ unsigned xcd_contiguous(unsigned b, unsigned G) {
   return x * q + (x < r ? x : r) + b / X;
 }
 
-// One thread per (group, row, lane) task (SH: partner rows share
-// primitives across the wavefront, nvector 32 only).
-template <typename R, int RS, bool S27, bool SH>
+// One thread per (group, row, lane) task.
+template <typename R, int RS, bool S27>
 __global__ __launch_bounds__(768) void unsplit_rows(UnsplitArgs a) {
   const long t = (long)xcd_contiguous(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= a.ngroups * 12L * a.nv) return;
-  unsplit_task<R, RS, S27, false, SH>(a, t, a.ngroups, a.ngrid_last);
+  unsplit_task<R, RS, S27>(a, t, a.ngroups, a.ngrid_last);
+}
+
+// The stencil cells some face of an oct depends on: a cell (i,j,k) in
+// -1..4 is read when at most one coordinate lies outside 1..2, or when two
+// do and both are 0 or 3 (a transverse neighbour of a row's end cell).
+// That is 8 + 48 + 24 = 80 cells.  slot[cell6] numbers them in cell6 order
+// (-1 for the unread cells) and cell[slot] inverts it.
+struct StageMap {
+  signed char slot[216];
+  unsigned char cell[80];
+};
+
+constexpr StageMap make_stage_map() {
+  StageMap m{};
+  int n = 0;
+  for (int c = 0; c < 216; ++c) {
+    const int x[3] = {c % 6 - 1, (c / 6) % 6 - 1, c / 36 - 1};
+    int out = 0, edge = 0;
+    for (int a = 0; a < 3; ++a) {
+      if (x[a] < 1 || x[a] > 2) ++out;
+      if (x[a] == 0 || x[a] == 3) ++edge;
+    }
+    const bool read = out <= 1 || (out == 2 && edge == 2);
+    m.slot[c] = read ? (signed char)n : (signed char)-1;
+    if (read) m.cell[n++] = (unsigned char)c;
+  }
+  return m;
+}
+
+constexpr StageMap kStageHost = make_stage_map();
+static_assert(kStageHost.slot[215] == -1 && kStageHost.cell[79] == 201, "80 stencil cells");
+__constant__ StageMap kStage = make_stage_map();
+
+constexpr int kStageLanes = 16;                  // octs per workgroup
+constexpr int kStageCells = 80 * kStageLanes;    // (cell, lane) pairs staged
+constexpr int kStageThreads = 12 * kStageLanes;  // one thread per (row, lane)
+
+// Primitives of the staged cells, variable-major: [5][80][kStageLanes].
+template <typename R>
+struct StagedCtx {
+  const R* pr;  // + lane
+  int nv;
+  __device__ __forceinline__ Prim<R> prim(int i, int j, int k) const {
+    const R* q = pr + kStage.slot[unsplit_dev::cell6(i, j, k)] * kStageLanes;
+    return Prim<R>{q[0], q[kStageCells], q[2 * kStageCells], q[3 * kStageCells], q[4 * kStageCells]};
+  }
+};
+
+// Half a batch per workgroup (nv == 32, 7-point slopes): stage, then one
+// row of faces per thread.  A workgroup's rows read only its own lanes.
+template <typename R, int RS>
+__global__ __launch_bounds__(kStageThreads) void unsplit_staged(UnsplitArgs a) {
+  __shared__ R pr[5 * kStageCells];
+  constexpr int nv = 32;
+  const long gidx = blockIdx.x >> 1;
+  const int l0 = (blockIdx.x & 1) * kStageLanes;
+  const int t = threadIdx.x;
+  const HydroParams hp = a.hp;
+  const double dt = a.d_dt ? *a.d_dt : a.dt;
+  const double dtxhalf = dt * 0.5;
+  const double smalle = hp.smallc * hp.smallc / hp.gamma / (hp.gamma - 1.0);  // hydro/umuscl.f90:884
+  const size_t vs = (size_t)nv * 216;
+  const double* u = a.uin + gidx * vs * 5 + l0;
+  const double* g = a.grav ? a.grav + gidx * vs * 3 + l0 : nullptr;
+  for (int e = t; e < kStageCells; e += kStageThreads) {
+    const int cell = e / kStageLanes, l = e - cell * kStageLanes;
+    const size_t o = (size_t)kStage.cell[cell] * nv + l;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    if (g) {
+      gx = g[o];
+      gy = g[o + vs];
+      gz = g[o + 2 * vs];
+    }
+    const Prim<R> q = ctoprim_cell<R>(u[o], u[o + vs], u[o + 2 * vs], u[o + 3 * vs], u[o + 4 * vs], gx, gy, gz,
+                                      dtxhalf, smalle, hp);
+    pr[e] = q.r;
+    pr[kStageCells + e] = q.u;
+    pr[2 * kStageCells + e] = q.v;
+    pr[3 * kStageCells + e] = q.w;
+    pr[4 * kStageCells + e] = q.p;
+  }
+  __syncthreads();
+  const int row = t / kStageLanes, l = t - row * kStageLanes;
+  const int ngrid = (gidx == a.ngroups - 1) ? a.ngrid_last : nv;
+  if (l0 + l >= ngrid) return;
+  const int d = row >> 2, a1 = 1 + (row & 1), b1 = 1 + ((row >> 1) & 1);
+  const StagedCtx<R> c{pr + l, nv};
+  const SolverConsts<R> sc = make_consts<R>(hp);
+  double* flux = a.flux + gidx * (size_t)nv * 27 * 15 + l0 + l;
+  double* tmp = a.tmp + gidx * (size_t)nv * 27 * 6 + l0 + l;
+  const size_t fvs = (size_t)nv * 27;
+  using unsplit_dev::row_faces;
+  if (d == 0)
+    row_faces<R, 0, RS, false, false>(c, sc, a1, b1, (R)(dt / a.dx[0]), flux, tmp, fvs, make_scale(dt, a.dx[0]),
+                                      nullptr);
+  else if (d == 1)
+    row_faces<R, 1, RS, false, false>(c, sc, a1, b1, (R)(dt / a.dx[1]), flux, tmp, fvs, make_scale(dt, a.dx[1]),
+                                      nullptr);
+  else
+    row_faces<R, 2, RS, false, false>(c, sc, a1, b1, (R)(dt / a.dx[2]), flux, tmp, fvs, make_scale(dt, a.dx[2]),
+                                      nullptr);
 }
 
 }  // namespace
 
-// Partner sharing is used in FP32 only: there it saves 4% (1.38 -> 1.33 ms
-// on 2^20 octs); in FP64 it measured 1% slower (1.74 -> 1.76 ms) although
-// it removes a quarter of the transverse primitive evaluations.
+// Staging is used in FP32-mixed only.  On 2^20 octs it measured 1.27 ms
+// against 1.32 ms for the row kernel (hllc and llf alike, the same bits);
+// in FP64 it measured 2.25 ms against 1.72 ms: 51 KB of FP64 primitives per
+// workgroup leave 2.25 waves per SIMD, and the FP64 kernel is not bound by
+// its primitive evaluations (a staged workgroup of 8 or 32 octs was slower
+// still in both precisions).
 template <typename R, int RS, bool S27>
 static void launch_t(const UnsplitArgs& a, long grid, int block, hipStream_t stream) {
   if constexpr (!S27 && std::is_same<R, float>::value) {
     if (a.nv == 32) {
-      hipLaunchKernelGGL((unsplit_rows<R, RS, false, true>), dim3((unsigned)grid), dim3(block), 0, stream, a);
+      hipLaunchKernelGGL((unsplit_staged<R, RS>), dim3((unsigned)(a.ngroups * 2)), dim3(kStageThreads), 0, stream, a);
       return;
     }
   }
-  hipLaunchKernelGGL((unsplit_rows<R, RS, S27, false>), dim3((unsigned)grid), dim3(block), 0, stream, a);
+  hipLaunchKernelGGL((unsplit_rows<R, RS, S27>), dim3((unsigned)grid), dim3(block), 0, stream, a);
 }
 
 template <typename R>

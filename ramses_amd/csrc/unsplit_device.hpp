@@ -108,35 +108,15 @@ __device__ __forceinline__ void store_face(double* flux, double* tmp, size_t fba
   tmp[tbase + fvs] = scale_flux(g.f[5], fs);
 }
 
-// Value of the lane 32 away in the same wavefront: v_permlane32_swap of x
-// with itself leaves lanes 0-31's upper partners in the second result and
-// lanes 32-63's lower partners in the first (a VALU swap, no LDS crossbar).
-__device__ __forceinline__ unsigned partner_u32(unsigned x) {
-  const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);
-  return (threadIdx.x & 32) ? r[0] : r[1];
-}
-__device__ __forceinline__ float partner_val(float x) { return __uint_as_float(partner_u32(__float_as_uint(x))); }
-__device__ __forceinline__ double partner_val(double x) {
-  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
-  const unsigned lo = partner_u32((unsigned)b), hi = partner_u32((unsigned)(b >> 32));
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-template <typename R>
-__device__ __forceinline__ Prim<R> partner_prim(const Prim<R>& q) {
-  return Prim<R>{partner_val(q.r), partner_val(q.u), partner_val(q.v), partner_val(q.w), partner_val(q.p)};
-}
-
 // One row of faces along direction D of one oct (see the file comment).
 // With ZR, ok points at the batch's refinement flags (+ lane) and a face
 // with a flagged cell on either side stores a zero flux: the value the
 // reference's reset leaves there (hydro/godunov_fine.f90:720-749).
-// With SH (nvector 32, wavefront-aligned tasks), the two rows of a
-// wavefront are rows a1 = 1 and a1 = 2 of the same direction and b1 for the
-// same 32 octs, lanes l and l+32: the inner first-transverse neighbour of a
-// row's cell at position s is the partner row's own cell at s, taken from
-// the partner lane instead of evaluated again (the same value).
-template <typename R, int D, int RS, bool S27, bool ZR, bool SH>
-__device__ __forceinline__ void row_faces(const RowCtx<R>& c, const SolverConsts<R>& sc, int a1, int b1,
+// Ctx supplies prim(i,j,k) and nv: RowCtx evaluates the primitives from
+// uin, and the staged kernel of unsplit.hip reads them from LDS (the 27-point
+// slopes need RowCtx).
+template <typename R, int D, int RS, bool S27, bool ZR, typename Ctx>
+__device__ __forceinline__ void row_faces(const Ctx& c, const SolverConsts<R>& sc, int a1, int b1,
                                           R dtdx, double* flux, double* tmp, size_t fvs, const FluxScale fs,
                                           const unsigned char* ok) {
   constexpr int T1 = (D == 0) ? 1 : 0;  // first transverse axis
@@ -155,7 +135,7 @@ __device__ __forceinline__ void row_faces(const RowCtx<R>& c, const SolverConsts
     int i, j, k;
     row_cell(D, s, a1, b1, i, j, k);
     Prim<R> dqd, dq1, dq2;
-    if (S27) {
+    if constexpr (S27) {
       Prim<R> gx, gy, gz;
       slope_27(c, i, j, k, gx, gy, gz);
       dqd = (D == 0) ? gx : ((D == 1) ? gy : gz);
@@ -163,17 +143,9 @@ __device__ __forceinline__ void row_faces(const RowCtx<R>& c, const SolverConsts
       dq2 = (T2 == 1) ? gy : gz;
     } else {
       dqd = slope_prim(qa, qb, qc, sc.slope_type, sc.theta);
-      if (SH) {
-        const bool lo = a1 == 1;
-        const Prim<R> inner = partner_prim(qb);
-        const Prim<R> outer = (T1 == 0) ? c.prim(lo ? i - 1 : i + 1, j, k) : c.prim(i, lo ? j - 1 : j + 1, k);
-        dq1 = lo ? slope_prim(outer, qb, inner, sc.slope_type, sc.theta)
-                 : slope_prim(inner, qb, outer, sc.slope_type, sc.theta);
-      } else {
-        const Prim<R> m1 = (T1 == 0) ? c.prim(i - 1, j, k) : c.prim(i, j - 1, k);
-        const Prim<R> p1 = (T1 == 0) ? c.prim(i + 1, j, k) : c.prim(i, j + 1, k);
-        dq1 = slope_prim(m1, qb, p1, sc.slope_type, sc.theta);
-      }
+      const Prim<R> m1 = (T1 == 0) ? c.prim(i - 1, j, k) : c.prim(i, j - 1, k);
+      const Prim<R> p1 = (T1 == 0) ? c.prim(i + 1, j, k) : c.prim(i, j + 1, k);
+      dq1 = slope_prim(m1, qb, p1, sc.slope_type, sc.theta);
       const Prim<R> m2 = (T2 == 1) ? c.prim(i, j - 1, k) : c.prim(i, j, k - 1);
       const Prim<R> p2 = (T2 == 1) ? c.prim(i, j + 1, k) : c.prim(i, j, k + 1);
       dq2 = slope_prim(m2, qb, p2, sc.slope_type, sc.theta);
@@ -292,13 +264,9 @@ __device__ __forceinline__ void face_task(const UnsplitArgs& a, long t, long ngr
     face_dir<R, 2, RS, S27, ZR>(c, sc, i, j, k, (R)(dt / a.dx[2]), flux, tmp, fvs, make_scale(dt, a.dx[2]), ok);
 }
 
-// One (group, row, lane) task of the flat decomposition (ZR, SH: see
-// row_faces; SH needs nv == 32, no 27-point slopes, and tasks that start a
-// wavefront at a multiple of 64 -- true of every launch here, whose
-// workgroup sizes are multiples of 64).
-template <typename R, int RS, bool S27, bool ZR = false, bool SH = false>
+// One (group, row, lane) task of the flat decomposition (ZR: see row_faces).
+template <typename R, int RS, bool S27, bool ZR = false>
 __device__ __forceinline__ void unsplit_task(const UnsplitArgs& a, long t, long ngroups, int ngrid_last) {
-  static_assert(!(SH && S27), "partner sharing is for the 7-point slopes");
   const int nv = a.nv;
   const long per_group = 12L * nv;
   const long gidx = t / per_group;
@@ -334,11 +302,11 @@ __device__ __forceinline__ void unsplit_task(const UnsplitArgs& a, long t, long 
   const size_t fvs = (size_t)nv * 27;
   const unsigned char* ok = ZR ? a.ok + gidx * (size_t)nv * 216 + l : nullptr;
   if (d == 0)
-    row_faces<R, 0, RS, S27, ZR, SH>(c, sc, a1, b1, (R)(dt / a.dx[0]), flux, tmp, fvs, make_scale(dt, a.dx[0]), ok);  // :516
+    row_faces<R, 0, RS, S27, ZR>(c, sc, a1, b1, (R)(dt / a.dx[0]), flux, tmp, fvs, make_scale(dt, a.dx[0]), ok);  // :516
   else if (d == 1)
-    row_faces<R, 1, RS, S27, ZR, SH>(c, sc, a1, b1, (R)(dt / a.dx[1]), flux, tmp, fvs, make_scale(dt, a.dx[1]), ok);
+    row_faces<R, 1, RS, S27, ZR>(c, sc, a1, b1, (R)(dt / a.dx[1]), flux, tmp, fvs, make_scale(dt, a.dx[1]), ok);
   else
-    row_faces<R, 2, RS, S27, ZR, SH>(c, sc, a1, b1, (R)(dt / a.dx[2]), flux, tmp, fvs, make_scale(dt, a.dx[2]), ok);
+    row_faces<R, 2, RS, S27, ZR>(c, sc, a1, b1, (R)(dt / a.dx[2]), flux, tmp, fvs, make_scale(dt, a.dx[2]), ok);
 }
 
 }  // namespace unsplit_dev

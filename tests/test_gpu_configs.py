@@ -5,7 +5,7 @@
   courant_fine / godunov_fine (tests/golden/sedov64.npz).
 * Synthetic oct batches, 2^20 octs: the full batched launch the benchmark
   times, a strided sample of its batches bitwise against the oracle; the
-  FP32-mixed partner-row kernel with a ragged last batch.
+  FP32-mixed LDS-staged kernel with a ragged last batch.
 * 512^3 Mach-5 k^-4 field (rho lognormal 0.5, P = rho/gamma, gamma 5/3,
   hllc, minmod, CFL 0.5): the same field at 64^3 bitwise against the
   reference's own level routines and at 128^3 against the oracle; at 512^3
@@ -93,9 +93,10 @@ def test_unsplit_full_launch_strided_vs_oracle(gpu, oracle_lib, riemann):
 
 
 @pytest.mark.parametrize("last", [1, 17, 31])
-def test_fp32_partner_rows_ragged_last_batch(gpu, oracle_lib, last):
-    """The FP32-mixed kernel exchanges primitives between the two rows of an
-    oct held 32 lanes apart.  With a ragged last batch its live lanes give
+def test_fp32_staged_ragged_last_batch(gpu, oracle_lib, last):
+    """The FP32-mixed kernel (nvector 32) stages the primitives of half a
+    batch per workgroup in LDS, so a last batch of 1, 17 or 31 octs leaves a
+    workgroup with no live lane or a partly live one.  Its live lanes give
     the same bits as the same octs in a full batch, its dead lanes keep the
     caller's values, and the result is within 1e-5 of the FP64 oracle."""
     import torch

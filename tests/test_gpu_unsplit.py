@@ -119,9 +119,9 @@ def test_cmpdt_golden(gpu, golden_cmpdt):
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
 def test_unsplit_nvector_layouts_agree(gpu, precision):
     """The same octs laid out in batches of 32 and of 16 give the same bits.
-    With 32 the FP32-mixed kernel takes the inner transverse neighbour's
-    primitives from the partner row 32 lanes away; with 16 it evaluates
-    them itself, so this pins the two forms against each other."""
+    With 32 the FP32-mixed kernel stages each cell's primitives once in LDS;
+    with 16 the row kernel evaluates them per row, so this pins the two
+    forms against each other."""
     import torch
     rng = np.random.default_rng(77)
     G = 8
