@@ -543,7 +543,7 @@ def main():
     ap.add_argument("--steps-per-tick", type=int, default=8, help="serve: level steps per scheduler tick")
     ap.add_argument("--no-tune", action="store_true", help="use the shapes given below instead of autotuning")
     ap.add_argument("--unsplit-block", type=int, default=0, help="with --no-tune: unsplit workgroup size")
-    ap.add_argument("--level-tile", type=int, default=0, help="with --no-tune: 1=32x8, 2=16x8, 3=8x8, 4=32x16")
+    ap.add_argument("--level-tile", type=int, default=0, help="with --no-tune: 1=32x8, 2=16x8, 3=8x8, 4=32x16, 5=16x12")
     ap.add_argument("--level-tz", type=int, default=0, help="with --no-tune: z planes per workgroup")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="run only a device copy of this many MiB (for HBM counter scaling)")

@@ -401,7 +401,7 @@ int ramses_amr_mesh_release(ramses_amr_mesh *m);
  * (ramses_amd/autotune.py).  value 0 restores the built-in default.
  * --------------------------------------------------------------------- */
 #define RAMSES_TUNE_UNSPLIT_BLOCK 1 /* threads per unsplit workgroup: multiple of 64, <= 768 */
-#define RAMSES_TUNE_LEVEL_TILE 2    /* level tile: 1 = 32x8, 2 = 16x8, 3 = 8x8, 4 = 32x16 columns */
+#define RAMSES_TUNE_LEVEL_TILE 2    /* level tile: 1 = 32x8, 2 = 16x8, 3 = 8x8, 4 = 32x16, 5 = 16x12 columns */
 #define RAMSES_TUNE_LEVEL_TZ 3      /* z planes per level workgroup: a divisor of n, >= 4 */
 #define RAMSES_TUNE_PERSIST_BLOCKS 4 /* workgroups of a captured AMR step's fixed grids (0: 1024) */
 #define RAMSES_TUNE_AMR_FACE_OCTS 5  /* levels of at most this many octs run the AMR unsplit one face per

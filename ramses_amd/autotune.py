@@ -58,6 +58,12 @@ def tune_unsplit(run, key, blocks=(128, 192, 256, 320, 384, 512, 768), verbose=F
     return best
 
 
+def tile_16x12_fits(n):
+    """16x12 tiles cover rows [0, m) for the largest multiple m of 12 that
+    leaves a multiple of 8 rows, which 16x8 tiles cover (n a multiple of 16)."""
+    return n % 16 == 0 and any((n - m) % 8 == 0 for m in range(n // 12 * 12, 11, -12))
+
+
 def tune_level(run, n, key, verbose=False):
     """run(): one level update on the real state.  Returns ((tile, tz), ms)."""
     if key in _cache:
@@ -65,7 +71,7 @@ def tune_level(run, n, key, verbose=False):
         _set(LEVEL_TILE, tile)
         _set(LEVEL_TZ, tz)
         return _cache[key]
-    tiles = [t for t, w in ((1, 32), (2, 16), (3, 8), (4, 32)) if n % w == 0]
+    tiles = [t for t, w in ((1, 32), (2, 16), (3, 8), (4, 32)) if n % w == 0] + ([5] if tile_16x12_fits(n) else [])
     tzs = [0] + [z for z in (8, 16, 32, 64, 128) if n % z == 0 and z < n]
     best = None
     for t in tiles:

@@ -1032,7 +1032,7 @@ int ramses_tune(int knob, int value) {
       t.unsplit_block = value;
       return 0;
     case RAMSES_TUNE_LEVEL_TILE:
-      if (value < 0 || value > 4) return fail(-2, "level tile must be 0..4");
+      if (value < 0 || value > 5) return fail(-2, "level tile must be 0..5");
       t.level_tile = value;
       return 0;
     case RAMSES_TUNE_LEVEL_TZ:

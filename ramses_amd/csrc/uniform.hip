@@ -338,12 +338,14 @@ __device__ __forceinline__ void level_tile(const double* __restrict__ u, double*
 
 template <typename R, int TX, int TY, int RS, bool SUMS>
 __global__ __launch_bounds__((tile_threads<TX, TY>()))
-__attribute__((amdgpu_waves_per_eu(3, 8))) void uniform_godunov_kernel(UniformArgs a, int TZ) {
+__attribute__((amdgpu_waves_per_eu(3, 8))) void uniform_godunov_kernel(UniformArgs a, int TZ, int y0, int boff) {
   // XCD-aware tile order: the hardware deals workgroups round-robin to the
   // 8 XCDs, so workgroup b runs on XCD b%8.  Giving each XCD a contiguous
   // range of tiles (x fastest, then y, then z chunk) keeps the tiles that run
   // side by side on one XCD spatial neighbours, and their shared halo
-  // columns are served by that XCD's L2 instead of HBM.
+  // columns are served by that XCD's L2 instead of HBM.  The launch covers
+  // rows [y0, y0 + TY * gridDim.y); its workgroups' courant sums go to
+  // partials[boff + block].
   const int ntx = gridDim.x, nty = gridDim.y;
   const int nb = ntx * nty * gridDim.z;
   const int b = blockIdx.x + ntx * (blockIdx.y + nty * blockIdx.z);
@@ -351,8 +353,8 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void uniform_godunov_kernel(UniformAr
   int logical = (b & 7) * per_xcd + (b >> 3);
   if ((nb & 7) != 0) logical = b;  // ragged grids keep the plain order
   const int tx = logical % ntx, ty = (logical / ntx) % nty, tzc = logical / (ntx * nty);
-  level_tile<R, TX, TY, RS, SUMS>(a.u, a.unew, a.n, a.dx, *a.d_dt, a.d_dt_next, a.hp, tx * TX, ty * TY, tzc * TZ,
-                                  TZ, a.partials, b);
+  level_tile<R, TX, TY, RS, SUMS>(a.u, a.unew, a.n, a.dx, *a.d_dt, a.d_dt_next, a.hp, tx * TX, y0 + ty * TY,
+                                  tzc * TZ, TZ, a.partials, boff + b);
 }
 
 // Many independent levels advanced by one launch (continuous batching).
@@ -506,32 +508,61 @@ __global__ void cmpdt_group(const double* uu, const double* gg, int nv, int ncel
   }
 }
 
+// Tiles of rows [y0, y0 + ny), ny a multiple of TY; returns the number of
+// workgroups launched.
 template <typename R, int TX, int TY, bool SUMS>
-hipError_t launch_tile_s(const UniformArgs& a, int TZ, hipStream_t s) {
+long launch_band(const UniformArgs& a, int TZ, int y0, int ny, int boff, hipStream_t s) {
   constexpr int NT = tile_threads<TX, TY>();
-  dim3 grid(a.n / TX, a.n / TY, a.n / TZ);
+  const dim3 grid(a.n / TX, ny / TY, a.n / TZ);
   switch (a.hp.riemann) {
-    case kHLLC: hipLaunchKernelGGL((uniform_godunov_kernel<R, TX, TY, kHLLC, SUMS>), grid, dim3(NT), 0, s, a, TZ); break;
-    case kHLL: hipLaunchKernelGGL((uniform_godunov_kernel<R, TX, TY, kHLL, SUMS>), grid, dim3(NT), 0, s, a, TZ); break;
+    case kHLLC:
+      hipLaunchKernelGGL((uniform_godunov_kernel<R, TX, TY, kHLLC, SUMS>), grid, dim3(NT), 0, s, a, TZ, y0, boff);
+      break;
+    case kHLL:
+      hipLaunchKernelGGL((uniform_godunov_kernel<R, TX, TY, kHLL, SUMS>), grid, dim3(NT), 0, s, a, TZ, y0, boff);
+      break;
     case kACOUSTIC:
     case kEXACT:
-      hipLaunchKernelGGL((uniform_godunov_kernel<R, TX, TY, kDynamicRiemann, SUMS>), grid, dim3(NT), 0, s, a, TZ);
+      hipLaunchKernelGGL((uniform_godunov_kernel<R, TX, TY, kDynamicRiemann, SUMS>), grid, dim3(NT), 0, s, a, TZ, y0,
+                         boff);
       break;
-    default: hipLaunchKernelGGL((uniform_godunov_kernel<R, TX, TY, kLLF, SUMS>), grid, dim3(NT), 0, s, a, TZ); break;
+    default:
+      hipLaunchKernelGGL((uniform_godunov_kernel<R, TX, TY, kLLF, SUMS>), grid, dim3(NT), 0, s, a, TZ, y0, boff);
+      break;
   }
+  return (long)grid.x * grid.y * grid.z;
+}
+
+// Rows [0, ny12) in 16x12 tiles, the rest in 16x8 tiles: ny12 is the
+// largest multiple of 12 that leaves a multiple of 8, or -1 if there is
+// none (n must be a multiple of 16).
+inline int rows_in_16x12(int n) {
+  for (int ny = n / 12 * 12; ny >= 12; ny -= 12)
+    if ((n - ny) % 8 == 0) return ny;
+  return -1;
+}
+
+// Launch the level kernel (16x12 tiles: two bands, see rows_in_16x12); with
+// sums, then add the workgroups' conservation sums into d_sums[0..2] in
+// block order.
+template <typename R, int TX, int TY, bool SUMS>
+hipError_t launch_tile_s(const UniformArgs& a, int TZ, double* d_sums, hipStream_t s) {
+  long nblocks;
+  if constexpr (TY == 12) {
+    const int ny = rows_in_16x12(a.n);
+    nblocks = launch_band<R, 16, 12, SUMS>(a, TZ, 0, ny, 0, s);
+    if (ny < a.n) nblocks += launch_band<R, 16, 8, SUMS>(a, TZ, ny, a.n - ny, (int)nblocks, s);
+  } else {
+    nblocks = launch_band<R, TX, TY, SUMS>(a, TZ, 0, a.n, 0, s);
+  }
+  if constexpr (SUMS)
+    hipLaunchKernelGGL(cons_finish, dim3(1), dim3(256), 0, s, (const double*)a.partials, nblocks, d_sums, 0);
   return hipGetLastError();
 }
 
-// Launch the level kernel; with sums, then add the workgroups' conservation
-// sums into d_sums[0..2] in block order.
 template <typename R, int TX, int TY>
 hipError_t launch_tile(const UniformArgs& a, int TZ, double* d_sums, hipStream_t s) {
-  if (!a.partials) return launch_tile_s<R, TX, TY, false>(a, TZ, s);
-  hipError_t e = launch_tile_s<R, TX, TY, true>(a, TZ, s);
-  if (e != hipSuccess) return e;
-  const long nblocks = (long)(a.n / TX) * (a.n / TY) * (a.n / TZ);
-  hipLaunchKernelGGL(cons_finish, dim3(1), dim3(256), 0, s, (const double*)a.partials, nblocks, d_sums, 0);
-  return hipGetLastError();
+  return a.partials ? launch_tile_s<R, TX, TY, true>(a, TZ, d_sums, s) : launch_tile_s<R, TX, TY, false>(a, TZ, d_sums, s);
 }
 
 template <typename R, int RS>
@@ -624,18 +655,22 @@ size_t uniform_sums_work_bytes(int n) {
   return (size_t)(blocks > 1 ? blocks : 1) * 3 * sizeof(double);
 }
 
-// Tile choice: the largest supported interior tile dividing n, and a z chunk
-// that leaves enough workgroups to fill the chip.
+// Tile choice: the launch-shape knob when it fits n, else 16x12 where n
+// allows it (three interior waves and one ring wave: the ring's work is a
+// third of a tile's, against a half for 16x8), else the largest tile
+// dividing n; and a z chunk that leaves enough workgroups to fill the chip.
 hipError_t launch_uniform_godunov(const double* u, double* unew, int n, double dx, const double* d_dt,
                                   double* d_dt_next, const HydroParams& hp, bool fp32, hipStream_t s,
                                   double* partials, double* d_sums) {
   UniformArgs a{u, unew, d_dt, d_dt_next, partials, n, dx, hp};
   int TX, TY;
   const int tile = tuning().level_tile;
+  const bool fits12 = n % 16 == 0 && rows_in_16x12(n) > 0;
   if (tile == 1 && n % 32 == 0) { TX = 32; TY = 8; }
   else if (tile == 2 && n % 16 == 0) { TX = 16; TY = 8; }
   else if (tile == 3) { TX = 8; TY = 8; }
   else if (tile == 4 && n % 32 == 0) { TX = 32; TY = 16; }
+  else if (fits12) { TX = 16; TY = 12; }  // tile 5, and the default
   else if (n % 32 == 0) { TX = 32; TY = 8; }
   else if (n % 16 == 0) { TX = 16; TY = 8; }
   else { TX = 8; TY = 8; }
@@ -645,11 +680,13 @@ hipError_t launch_uniform_godunov(const double* u, double* unew, int n, double d
   const int tz = tuning().level_tz;
   if (tz >= 4 && tz <= n && n % tz == 0) TZ = tz;
   if (fp32) {
+    if (TY == 12) return launch_tile<float, 16, 12>(a, TZ, d_sums, s);
     if (TX == 32 && TY == 16) return launch_tile<float, 32, 16>(a, TZ, d_sums, s);
     if (TX == 32) return launch_tile<float, 32, 8>(a, TZ, d_sums, s);
     if (TX == 16) return launch_tile<float, 16, 8>(a, TZ, d_sums, s);
     return launch_tile<float, 8, 8>(a, TZ, d_sums, s);
   }
+  if (TY == 12) return launch_tile<double, 16, 12>(a, TZ, d_sums, s);
   if (TX == 32 && TY == 16) return launch_tile<double, 32, 16>(a, TZ, d_sums, s);
   if (TX == 32) return launch_tile<double, 32, 8>(a, TZ, d_sums, s);
   if (TX == 16) return launch_tile<double, 16, 8>(a, TZ, d_sums, s);

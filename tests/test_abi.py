@@ -127,7 +127,7 @@ def test_tuning_knobs_validate_and_read_back():
     lib = ramses_amd.load_library()
     lib.ramses_tune.argtypes = [ctypes.c_int, ctypes.c_int]
     lib.ramses_tune_get.argtypes = [ctypes.c_int]
-    bad = [(1, 100), (1, 832), (2, 5), (3, 2), (4, 16), (4, 9000), (5, -2), (99, 0)]
+    bad = [(1, 100), (1, 832), (2, 6), (2, -1), (3, 2), (4, 16), (4, 9000), (5, -2), (99, 0)]
     for knob, value in bad:
         assert lib.ramses_tune(knob, value) < 0, (knob, value)
         assert len(lib.ramses_last_error()) > 0
