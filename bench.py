@@ -364,6 +364,7 @@ def amr_bench(torch, ra, args):
         out[prec] = {"cell_updates_per_s": cu / wall, "wall_s": wall, "coarse_steps": ncoarse,
                      "fine_steps": s.fine_steps, "cell_updates": cu,
                      "ms_per_coarse_step": 1e3 * wall / ncoarse,
+                     "ms_each_coarse_step": [round(1e3 * x, 3) for x in per_step],
                      "octs_initial": init_octs, "octs_final": {l: s.noct(l) for l in range(lmin, lmax + 1)},
                      "roofline": {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                                   "frac": ach / HBM_PEAK, "traffic": None}}

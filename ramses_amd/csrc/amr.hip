@@ -481,51 +481,99 @@ __global__ void amr_dt_kernel(int op, double* dt, double* dtold, double* time, i
 // down to the first level with no finer octs, each level's sum added as one
 // term (hydro/courant_fine.f90:152-153).  A level's sum runs in the
 // reference's serial order -- batches of nvector grids, cell position ind,
-// grid in the batch (:44-102) -- so the values have its rounding: one
-// wavefront loads 64 cells of that order at a time and adds them in lane
-// order.  A cell that is not a leaf adds +0.0, which leaves the positive
+// grid in the batch (:44-102) -- so the values have its rounding.  The
+// workgroup gathers chunks of kConsChunk terms of that order into LDS, many
+// threads loading in parallel, so the tree walks and loads of a chunk
+// overlap.  Thread 0 adds a chunk's terms in order while the wavefronts
+// after the first gather the next chunk into the second buffer.  A cell that is not a leaf adds +0.0, which leaves the positive
 // running sum unchanged.  With dn the level counts are read from device
 // memory and `armed` (nonzero: nstep_coarse changed since the last
 // update_time) decides at run time.
-__global__ __launch_bounds__(64) void amr_cons0_kernel(ConsLevels lv, const int* dn, AmrTreeView t, const double* u,
-                                                      long ustride, int levelmin, int nlevelmax, double boxlen, int nv,
-                                                      const int* armed, double* cons0) {
+constexpr int kConsChunk = 2048;
+constexpr int kConsThreads = 1024;
+
+
+// Terms [q0, q0 + kConsChunk) of a level's serial order into tp (mass,
+// energy), loaded by threads [first, kConsThreads); past the level's end,
+// zeros.
+__device__ __forceinline__ void cons_gather(const int* active, int n, int nv, AmrTreeView t, const double* u,
+                                            long ustride, double vol, long q0, int first, double2* tp) {
+  // term q: batch b0 = q / (8 nv) * nv (only the last batch is short, so
+  // 8 b0 terms precede it), then slot s = q - 8 b0 = (ind - 1) * ng + grid
+  const long nterm = 8L * n;
+  for (int k = (int)threadIdx.x - first; k < kConsChunk; k += kConsThreads - first) {
+    double vm = 0.0, ve = 0.0;
+    if (q0 + k < nterm) {
+      const int q = (int)(q0 + k);  // 8 n <= 8 ngridmax fits the reference's 32-bit cell indices
+      const int b0 = q / (8 * nv) * nv;
+      const int ng = min(nv, n - b0);
+      const int sl = q - 8 * b0;
+      const long c = cell_of(t, sl / ng + 1, active[b0 + sl % ng]);
+      if (t.son[c] == 0) {
+        vm = u[c] * vol;                 // hydro/courant_fine.f90:95-97
+        ve = u[4 * ustride + c] * vol;   // :100-102
+      }
+    }
+    tp[k] = make_double2(vm, ve);
+  }
+}
+
+// Adds the first cnt terms of tp to m, e in order (one thread); the LDS
+// loads run ahead of the two dependent chains of additions, which bound the
+// kernel (about 20 cycles a term).
+__device__ __forceinline__ void cons_add(const double2* tp, int cnt, double& m, double& e) {
+  int k0 = 0;
+  for (; k0 + 16 <= cnt; k0 += 16) {
+    double2 v[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) v[j] = tp[k0 + j];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      m = m + v[j].x;
+      e = e + v[j].y;
+    }
+  }
+  for (; k0 < cnt; ++k0) {
+    const double2 v = tp[k0];
+    m = m + v.x;
+    e = e + v.y;
+  }
+}
+
+__global__ __launch_bounds__(kConsThreads) void amr_cons0_kernel(ConsLevels lv, const int* dn, AmrTreeView t,
+                                                                const double* u, long ustride, int levelmin,
+                                                                int nlevelmax, double boxlen, int nv, const int* armed,
+                                                                double* cons0) {
+  __shared__ double2 tp[2][kConsChunk];  // double-buffered chunks of (mass, energy) terms
   if (armed && *armed == 0) return;
   if (cons0[0] != 0.0 && cons0[1] != 0.0) return;
-  const int lane = threadIdx.x;
-  double mass_tot = 0.0, ekin_tot = 0.0;
+  const bool first_wave = threadIdx.x < 64;
+  double mass_tot = 0.0, ekin_tot = 0.0;  // thread 0's
   for (int l = levelmin; l <= nlevelmax; ++l) {
     const int n = dn ? dn[l] : lv.noct[l];
     if (n <= 0) break;
     const int* active = lv.active[l];
     const double dx = __builtin_ldexp(1.0, -l) * boxlen;
     const double vol = dx * dx * dx;
+    const long nterm = 8L * n;
+    const long nchunk = (nterm + kConsChunk - 1) / kConsChunk;
     double m = 0.0, e = 0.0;
-    for (int b0 = 0; b0 < n; b0 += nv) {
-      const int ng = min(nv, n - b0);
-      const int nslot = 8 * ng;
-      for (int s0 = 0; s0 < nslot; s0 += 64) {
-        const int sl = s0 + lane;
-        double vm = 0.0, ve = 0.0;
-        if (sl < nslot) {
-          const long c = cell_of(t, sl / ng + 1, active[b0 + sl % ng]);
-          if (t.son[c] == 0) {
-            vm = u[c] * vol;                 // :95-97
-            ve = u[4 * ustride + c] * vol;   // :100-102
-          }
-        }
-        const int cnt = min(64, nslot - s0);
-        for (int k = 0; k < cnt; ++k) {
-          m = m + __shfl(vm, k, 64);
-          e = e + __shfl(ve, k, 64);
-        }
-      }
+    cons_gather(active, n, nv, t, u, ustride, vol, 0, 0, tp[0]);
+    __syncthreads();
+    // thread 0 adds chunk c while the wavefronts after the first gather c + 1
+    for (long c = 0; c < nchunk; ++c) {
+      const int b = (int)(c & 1);
+      if (threadIdx.x == 0)
+        cons_add(tp[b], (int)min((long)kConsChunk, nterm - c * kConsChunk), m, e);
+      else if (!first_wave && c + 1 < nchunk)
+        cons_gather(active, n, nv, t, u, ustride, vol, (c + 1) * kConsChunk, 64, tp[b ^ 1]);
+      __syncthreads();
     }
     mass_tot = mass_tot + m;
     ekin_tot = ekin_tot + e;
     if (l == nlevelmax || (dn ? dn[l + 1] : lv.noct[l + 1]) == 0) break;
   }
-  if (lane == 0) {
+  if (threadIdx.x == 0) {
     if (cons0[1] == 0.0) cons0[1] = mass_tot;        // mass_tot_0
     if (cons0[0] == 0.0) cons0[0] = 0.0 + ekin_tot;  // einit = epot_tot + ekin_tot
   }
@@ -655,8 +703,8 @@ hipError_t launch_amr_dt(int op, double* dt, double* dtold, double* time, int l,
 hipError_t launch_amr_cons0(const ConsLevels& lv, const int* dn, const AmrTreeView& t, const double* u, long ustride,
                             int levelmin, int nlevelmax, double boxlen, int nv, const int* armed, double* cons0,
                             hipStream_t s) {
-  hipLaunchKernelGGL(amr_cons0_kernel, dim3(1), dim3(64), 0, s, lv, dn, t, u, ustride, levelmin, nlevelmax, boxlen, nv,
-                     armed, cons0);
+  hipLaunchKernelGGL(amr_cons0_kernel, dim3(1), dim3(kConsThreads), 0, s, lv, dn, t, u, ustride, levelmin, nlevelmax,
+                     boxlen, nv, armed, cons0);
   return hipGetLastError();
 }
 

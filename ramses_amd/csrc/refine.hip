@@ -11,10 +11,11 @@
 //                            (hydro/hydro_flag.f90:1-180,
 //                            hydro/godunov_utils.f90:125-263)
 //                flag_ensure ensure_ref_rules (:213-276)
-// refine_fine    refine_mark one thread per (batch, ind, lane) slot, the
+// refine_fine    MarkSrc     one mark per (batch, ind, lane) slot, the
 //                            order in which the reference visits cells, so
 //                            an exclusive scan of the marks gives every new
 //                            or killed oct its rank in the reference's order
+//                            (the marks are computed inside their scan)
 //                refine_make make_grid_fine (:590-948): the k-th new oct takes
 //                            the k-th free grid from the head of the free
 //                            list, links to its father and neighbour cells,
@@ -230,18 +231,31 @@ struct DevLen {
   }
 };
 
-// out[i] = sum of in[0..i) within the tile; sums[tile] = tile total.  Block b
+// The scanned values: an array, or computed by the kernel that scans them
+// (the producer stores each value too, for the scan's consumers).
+struct ArraySrc {
+  const long long* in;
+  __device__ __forceinline__ bool skip() const { return false; }
+  __device__ __forceinline__ long long operator()(long i) const { return in[i]; }
+};
+
+// Exclusive scan in two launches: scan_tiles leaves out[i] = the sum of the
+// values before i within its tile of kScanTile and sums[tile] = the tile's
+// total; scan_sums turns sums[] into the tiles' exclusive offsets (and
+// sums[ntiles] = the grand total).  The consumers read out[i] + sums[i /
+// kScanTile] (scan_offset), so no third pass adds the offsets.  Block b
 // scans tiles b, b + gridDim.x, ...
-__global__ __launch_bounds__(kScanThreads) void scan_tiles(const long long* in, long long* out, long long* sums,
-                                                            DevLen len) {
+template <class Src>
+__global__ __launch_bounds__(kScanThreads) void scan_tiles(Src src, long long* out, long long* sums, DevLen len) {
   __shared__ long long sh[kScanThreads / 64];
+  if (src.skip()) return;
   const long n = len.get();
   const long ntiles = (n + kScanTile - 1) / kScanTile;
   for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const long base = tile * kScanTile + (long)threadIdx.x * kScanItems;
     long long v[kScanItems], s = 0;
     for (int k = 0; k < kScanItems; ++k) {
-      v[k] = base + k < n ? in[base + k] : 0;
+      v[k] = base + k < n ? src(base + k) : 0;
       s += v[k];
     }
     long long total;
@@ -276,20 +290,25 @@ __global__ __launch_bounds__(1024) void scan_sums(long long* sums, DevLen len) {
   if (threadIdx.x == 0) sums[ntiles] = carry;
 }
 
-__global__ void scan_add(long long* out, const long long* sums, DevLen len) {
-  const long n = len.get(), stride = (long)gridDim.x * blockDim.x;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) out[i] += sums[i / kScanTile];
+__device__ __forceinline__ long long scan_offset(const long long* out, const long long* sums, long i) {
+  return out[i] + sums[i / kScanTile];
 }
 
 // --- refine_fine ----------------------------------------------------------
 
 // Slot p = batch*256 + (ind-1)*32 + lane over the level's active list in
-// nvector=32 batches.  mark = create (low 32 bits) | kill << 32.
-__global__ void refine_mark(const int* active, int noct_h, const int* dn, AmrTreeView t, const int* flag1,
-                            long long* mark) {
-  const int noct = level_count(noct_h, dn);
-  const long nslot = (long)((noct + 31) / 32) * 256, stride = (long)gridDim.x * blockDim.x;
-  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < nslot; p += stride) {
+// nvector=32 batches.  mark = create (low 32 bits) | kill << 32, computed
+// and stored by the scan of the marks.
+struct MarkSrc {
+  const int* active;
+  int noct_h;
+  const int* dn;
+  AmrTreeView t;
+  const int* flag1;
+  long long* mark;
+  __device__ __forceinline__ bool skip() const { return false; }
+  __device__ __forceinline__ long long operator()(long p) const {
+    const int noct = level_count(noct_h, dn);
     const int b = (int)(p >> 8), r = (int)(p & 255);
     const int ind = (r >> 5) + 1, lane = r & 31;
     const int a = b * 32 + lane;
@@ -301,8 +320,9 @@ __global__ void refine_mark(const int* active, int noct_h, const int* dn, AmrTre
       if (f == 0 && s > 0) m = 1LL << 32;
     }
     mark[p] = m;
+    return m;
   }
-}
+};
 
 struct RefineRun {
   AmrTreeView t;      // read view of the same arrays as m
@@ -310,7 +330,8 @@ struct RefineRun {
   const int* active;  // level ilevel
   int noct;
   const long long* mark;
-  const long long* off;  // exclusive scan of mark
+  const long long* off;   // exclusive scan of mark within tiles
+  const long long* sums;  // the tiles' offsets (scan_offset)
   long nslot;
   int* ring;
   int cap;          // ngridmax
@@ -346,7 +367,7 @@ __device__ void make_slot(const RefineRun& r, long p) {
   const int ind = (q >> 5) + 1, lane = q & 31;
   const int igrid = r.active[b * 32 + lane];
   const long c = cell_of(r.t, ind, igrid);
-  const int k = (int)(r.off[p] & 0xffffffffLL);
+  const int k = (int)(scan_offset(r.off, r.sums, p) & 0xffffffffLL);
   const int g = r.ring[(int)(((long)r.head_() + k) % r.cap)];
   r.newg[k] = g;
   r.m.son[c] = g;
@@ -386,7 +407,7 @@ __device__ void kill_slot(const RefineRun& r, long p) {
   const int b = (int)(p >> 8), q = (int)(p & 255);
   const int ind = (q >> 5) + 1, lane = q & 31;
   const long c = cell_of(r.t, ind, r.active[b * 32 + lane]);
-  const int k = (int)(r.off[p] >> 32);
+  const int k = (int)(scan_offset(r.off, r.sums, p) >> 32);
   const int g = r.m.son[c];
   r.m.son[c] = 0;
   r.m.father[g] = 0;
@@ -437,15 +458,27 @@ __global__ void refine_decide(const long long* sums, DevLen len, const int* dfre
   ctrl[2] = ok;
 }
 
-// Survivors of the next level's list (father still set) in list order.
-__global__ void list_alive(const int* list, int n, const int* father, long long* mark) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) mark[i] = father[list[i]] != 0 ? 1 : 0;
-}
+// Survivors of the next level's list (father still set) in list order,
+// computed and stored by their scan.  With ctrl (the device-decided
+// refinement), nothing is scanned unless octs were killed: the list then
+// keeps its order.
+struct AliveSrc {
+  const int* list;
+  const int* father;
+  long long* mark;
+  const int* ctrl;
+  __device__ __forceinline__ bool skip() const { return ctrl && (!ctrl[2] || ctrl[1] == 0); }
+  __device__ __forceinline__ long long operator()(long i) const {
+    const long long m = father[list[i]] != 0 ? 1 : 0;
+    mark[i] = m;
+    return m;
+  }
+};
 
-__global__ void list_compact(const int* list, int n, const long long* mark, const long long* off, int* tmp) {
+__global__ void list_compact(const int* list, int n, const long long* mark, const long long* off,
+                             const long long* sums, int* tmp) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && mark[i]) tmp[off[i]] = list[i];
+  if (i < n && mark[i]) tmp[scan_offset(off, sums, i)] = list[i];
 }
 
 // list[0..nalive) = tmp[0..nalive) (if tmp), list[nalive..) = newg; pos follows.
@@ -460,19 +493,12 @@ __global__ void list_assemble(int* list, const int* tmp, int nalive, const int* 
 
 // The same list maintenance with the counts and the decision in device
 // memory: dnf = the next level's count before this refinement.
-__global__ void list_alive_d(const int* list, const int* dnf, const int* father, long long* mark, const int* ctrl) {
+__global__ void list_compact_d(const int* list, const int* dnf, const long long* mark, const long long* off,
+                               const long long* sums, int* tmp, const int* ctrl) {
   if (!ctrl[2] || ctrl[1] == 0) return;  // nothing killed: the list keeps its order
   const long n = *dnf, stride = (long)gridDim.x * blockDim.x;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    mark[i] = father[list[i]] != 0 ? 1 : 0;
-}
-
-__global__ void list_compact_d(const int* list, const int* dnf, const long long* mark, const long long* off, int* tmp,
-                               const int* ctrl) {
-  if (!ctrl[2] || ctrl[1] == 0) return;
-  const long n = *dnf, stride = (long)gridDim.x * blockDim.x;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    if (mark[i]) tmp[off[i]] = list[i];
+    if (mark[i]) tmp[scan_offset(off, sums, i)] = list[i];
 }
 
 __global__ void list_assemble_d(int* list, const int* tmp, const int* dnf, const int* newg, int* pos,
@@ -502,24 +528,24 @@ __global__ void refine_commit(int* dnf, int* dfree, const int* ctrl, int cap) {
 
 inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
 
-hipError_t scan64(const long long* in, long long* out, long long* sums, long n, hipStream_t s) {
+// The scan of n values from src (host length): one tile scans into its
+// final offsets (sums = {0, total}); more tiles also scan the tile totals.
+template <class Src>
+hipError_t scan64(Src src, long long* out, long long* sums, long n, hipStream_t s) {
   const long nt = (n + kScanTile - 1) / kScanTile;
   if (nt == 0) return hipSuccess;
   const DevLen len{n, nullptr, 0};
-  hipLaunchKernelGGL(scan_tiles, dim3((unsigned)nt), dim3(kScanThreads), 0, s, in, out, sums, len);
-  if (nt > 1) {
-    hipLaunchKernelGGL(scan_sums, dim3(1), dim3(1024), 0, s, sums, len);
-    hipLaunchKernelGGL(scan_add, dim3(nblk(n, 256)), dim3(256), 0, s, out, (const long long*)sums, len);
-  }
+  hipLaunchKernelGGL(scan_tiles<Src>, dim3((unsigned)nt), dim3(kScanThreads), 0, s, src, out, sums, len);
+  if (nt > 1) hipLaunchKernelGGL(scan_sums, dim3(1), dim3(1024), 0, s, sums, len);
   return hipGetLastError();
 }
 
-// The scan with its length in device memory: persistent grids, and the
+// The scan with its length in device memory: a persistent grid, and the
 // tile totals always scanned so that sums[ntiles] is the grand total.
-hipError_t scan64_dev(const long long* in, long long* out, long long* sums, DevLen len, hipStream_t s) {
-  hipLaunchKernelGGL(scan_tiles, dim3(persist_blocks() / 2), dim3(kScanThreads), 0, s, in, out, sums, len);
+template <class Src>
+hipError_t scan64_dev(Src src, long long* out, long long* sums, DevLen len, hipStream_t s) {
+  hipLaunchKernelGGL(scan_tiles<Src>, dim3(persist_blocks() / 2), dim3(kScanThreads), 0, s, src, out, sums, len);
   hipLaunchKernelGGL(scan_sums, dim3(1), dim3(1024), 0, s, sums, len);
-  hipLaunchKernelGGL(scan_add, dim3(persist_blocks()), dim3(256), 0, s, out, (const long long*)sums, len);
   return hipGetLastError();
 }
 
@@ -590,9 +616,7 @@ hipError_t launch_refine_fine(const int* active, int noct, int ilevel, const Amr
   long long* sums = off + nmax;  // nt entries
   int* newg = (int*)(sums + nt);
   int* tmp = newg + nslot;
-  hipLaunchKernelGGL(refine_mark, dim3(nblk(nslot, 256)), dim3(256), 0, s, active, noct, (const int*)nullptr, t,
-                     (const int*)flag1, mark);
-  hipError_t e = scan64(mark, off, sums, nslot, s);
+  hipError_t e = scan64(MarkSrc{active, noct, nullptr, t, flag1, mark}, off, sums, nslot, s);
   if (e != hipSuccess) return e;
   const long ntiles = (nslot + kScanTile - 1) / kScanTile;
   long long tot = 0;
@@ -611,6 +635,7 @@ hipError_t launch_refine_fine(const int* active, int noct, int ilevel, const Amr
   r.noct = noct;
   r.mark = mark;
   r.off = off;
+  r.sums = sums;
   r.nslot = nslot;
   r.ring = ring;
   r.cap = t.ngridmax;
@@ -636,11 +661,9 @@ hipError_t launch_refine_fine(const int* active, int noct, int ilevel, const Amr
   if ((e = hipGetLastError()) != hipSuccess) return e;
   int nalive = nf;
   if (nkill > 0 && nf > 0) {
-    hipLaunchKernelGGL(list_alive, dim3(nblk(nf, 256)), dim3(256), 0, s, (const int*)active_fine, nf,
-                       (const int*)m.father, mark);
-    if ((e = scan64(mark, off, sums, nf, s)) != hipSuccess) return e;
+    if ((e = scan64(AliveSrc{active_fine, m.father, mark, nullptr}, off, sums, nf, s)) != hipSuccess) return e;
     hipLaunchKernelGGL(list_compact, dim3(nblk(nf, 256)), dim3(256), 0, s, (const int*)active_fine, nf,
-                       (const long long*)mark, (const long long*)off, tmp);
+                       (const long long*)mark, (const long long*)off, (const long long*)sums, tmp);
     nalive = nf - nkill;
     if (nalive + ncreate > 0)
       hipLaunchKernelGGL(list_assemble, dim3(nblk(nalive + ncreate, 256)), dim3(256), 0, s, active_fine,
@@ -680,9 +703,8 @@ hipError_t launch_refine_fine_dev(const int* active, const int* dn_l, int ilevel
   long long* sums = off + nmax;
   int* newg = (int*)(sums + nt);
   int* tmp = newg + nslot;
-  hipLaunchKernelGGL(refine_mark, dim3(persist_blocks()), dim3(256), 0, s, active, 0, dn_l, t, (const int*)flag1, mark);
   const DevLen slots{0, dn_l, 2};
-  hipError_t e = scan64_dev(mark, off, sums, slots, s);
+  hipError_t e = scan64_dev(MarkSrc{active, 0, dn_l, t, flag1, mark}, off, sums, slots, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(refine_decide, dim3(1), dim3(1), 0, s, (const long long*)sums, slots, (const int*)dfree,
                      (const int*)dn_f, capacity, ctrl, d_error);
@@ -693,6 +715,7 @@ hipError_t launch_refine_fine_dev(const int* active, const int* dn_l, int ilevel
   r.noct = 0;
   r.mark = mark;
   r.off = off;
+  r.sums = sums;
   r.nslot = 0;
   r.ring = ring;
   r.cap = cap;
@@ -715,11 +738,10 @@ hipError_t launch_refine_fine_dev(const int* active, const int* dn_l, int ilevel
   r.ctrl = ctrl;
   hipLaunchKernelGGL(refine_make, dim3(persist_blocks()), dim3(128), 0, s, r);
   hipLaunchKernelGGL(refine_kill, dim3(persist_blocks()), dim3(256), 0, s, r);
-  hipLaunchKernelGGL(list_alive_d, dim3(persist_blocks()), dim3(256), 0, s, (const int*)active_fine, (const int*)dn_f,
-                     (const int*)m.father, mark, (const int*)ctrl);
-  if ((e = scan64_dev(mark, off, sums, DevLen{0, dn_f, 1}, s)) != hipSuccess) return e;
+  if ((e = scan64_dev(AliveSrc{active_fine, m.father, mark, ctrl}, off, sums, DevLen{0, dn_f, 1}, s)) != hipSuccess)
+    return e;
   hipLaunchKernelGGL(list_compact_d, dim3(persist_blocks()), dim3(256), 0, s, (const int*)active_fine, (const int*)dn_f,
-                     (const long long*)mark, (const long long*)off, tmp, (const int*)ctrl);
+                     (const long long*)mark, (const long long*)off, (const long long*)sums, tmp, (const int*)ctrl);
   hipLaunchKernelGGL(list_assemble_d, dim3(persist_blocks()), dim3(256), 0, s, active_fine, (const int*)tmp,
                      (const int*)dn_f, (const int*)newg, pos, (const int*)ctrl);
   hipLaunchKernelGGL(refine_commit, dim3(1), dim3(1), 0, s, dn_f, dfree, (const int*)ctrl, cap);
