@@ -137,7 +137,7 @@ __device__ __forceinline__ void update_cell(const AmrLevelArgs& a, int b, int q,
 // face, lane) task per thread on levels of at most u.face_octs octs, where
 // the chain of dependent steps matters more than the arithmetic, else one
 // (batch, row, lane) task per thread.  Small levels spread over every CU.
-template <typename R, int RS, bool S27>
+template <typename R, int RS, bool S27, int ST>
 __global__ __launch_bounds__(768) void amr_unsplit(AmrLevelArgs a, UnsplitArgs u) {
   const int noct = level_count(a.noct, a.dnoct);
   const int nv = a.nv;
@@ -147,11 +147,11 @@ __global__ __launch_bounds__(768) void amr_unsplit(AmrLevelArgs a, UnsplitArgs u
   if (noct <= u.face_octs) {
     const long ntask = (long)nb * 36 * nv;
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < ntask; t += stride)
-      unsplit_dev::face_task<R, RS, S27, true>(u, t, nb, ngrid_last);
+      unsplit_dev::face_task<R, RS, S27, true, ST>(u, t, nb, ngrid_last);
   } else {
     const long ntask = (long)nb * 12 * nv;
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < ntask; t += stride)
-      unsplit_dev::unsplit_task<R, RS, S27, true>(u, t, nb, ngrid_last);
+      unsplit_dev::unsplit_task<R, RS, S27, true, ST>(u, t, nb, ngrid_last);
   }
 }
 
@@ -169,9 +169,14 @@ __global__ __launch_bounds__(256) void amr_update(AmrLevelArgs a) {
   }
 }
 
+// Minmod (the reference's default slope) with the limiter fixed at compile
+// time; other slope types take it at run time.
 template <typename R, int RS, bool S27>
 void launch_solve_t(const AmrLevelArgs& a, const UnsplitArgs& u, unsigned grid, unsigned block, hipStream_t s) {
-  hipLaunchKernelGGL((amr_unsplit<R, RS, S27>), dim3(grid), dim3(block), 0, s, a, u);
+  if (!S27 && a.hp.slope_type == 1)
+    hipLaunchKernelGGL((amr_unsplit<R, RS, S27, 1>), dim3(grid), dim3(block), 0, s, a, u);
+  else
+    hipLaunchKernelGGL((amr_unsplit<R, RS, S27, -1>), dim3(grid), dim3(block), 0, s, a, u);
 }
 
 template <typename R>

@@ -125,8 +125,11 @@ __device__ __forceinline__ Prim<R> ctoprim_cell(double u0, double u1, double u2,
 // One-dimensional slope of the 3D uslope branches with axial stencils
 // (hydro/umuscl.f90:1241-1285 minmod, :1286-1327 moncen, :1393-1431 van
 // Leer, :1432-1473 generalised moncen).
-template <typename R>
-__device__ __forceinline__ R slope1(R qm1, R q0, R qp1, int st, R theta) {
+// ST >= 0 fixes the slope type at compile time (the kernel then holds one
+// limiter and no branches on it); ST = -1 takes it from st at run time.
+template <typename R, int ST = -1>
+__device__ __forceinline__ R slope1(R qm1, R q0, R qp1, int st_run, R theta) {
+  const int st = ST >= 0 ? ST : st_run;
   const R zero = (R)0;
   if (st == 1) {
     const R dlft = q0 - qm1;
@@ -159,16 +162,16 @@ __device__ __forceinline__ R slope1(R qm1, R q0, R qp1, int st, R theta) {
   return zero;  // slope_type 0 (:1002-1005)
 }
 
-// Slopes of the five primitives in one direction.
-template <typename R>
+// Slopes of the five primitives in one direction (ST: see slope1).
+template <typename R, int ST = -1>
 __device__ __forceinline__ Prim<R> slope_prim(const Prim<R>& a, const Prim<R>& b, const Prim<R>& c,
                                               int st, R theta) {
   Prim<R> d;
-  d.r = slope1(a.r, b.r, c.r, st, theta);
-  d.u = slope1(a.u, b.u, c.u, st, theta);
-  d.v = slope1(a.v, b.v, c.v, st, theta);
-  d.w = slope1(a.w, b.w, c.w, st, theta);
-  d.p = slope1(a.p, b.p, c.p, st, theta);
+  d.r = slope1<R, ST>(a.r, b.r, c.r, st, theta);
+  d.u = slope1<R, ST>(a.u, b.u, c.u, st, theta);
+  d.v = slope1<R, ST>(a.v, b.v, c.v, st, theta);
+  d.w = slope1<R, ST>(a.w, b.w, c.w, st, theta);
+  d.p = slope1<R, ST>(a.p, b.p, c.p, st, theta);
   return d;
 }
 

@@ -49,11 +49,11 @@ __device__ __forceinline__ unsigned xcd_contiguous(unsigned b, unsigned G) {
 }
 
 // One thread per (group, row, lane) task.
-template <typename R, int RS, bool S27>
+template <typename R, int RS, bool S27, int ST>
 __global__ __launch_bounds__(768) void unsplit_rows(UnsplitArgs a) {
   const long t = (long)xcd_contiguous(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= a.ngroups * 12L * a.nv) return;
-  unsplit_task<R, RS, S27>(a, t, a.ngroups, a.ngrid_last);
+  unsplit_task<R, RS, S27, false, ST>(a, t, a.ngroups, a.ngrid_last);
 }
 
 // The stencil cells some face of an oct depends on: a cell (i,j,k) in
@@ -104,7 +104,7 @@ struct StagedCtx {
 
 // Half a batch per workgroup (nv == 32, 7-point slopes): stage, then one
 // row of faces per thread.  A workgroup's rows read only its own lanes.
-template <typename R, int RS>
+template <typename R, int RS, int ST>
 __global__ __launch_bounds__(kStageThreads) void unsplit_staged(UnsplitArgs a) {
   __shared__ R pr[5 * kStageCells];
   constexpr int nv = 32;
@@ -147,13 +147,13 @@ __global__ __launch_bounds__(kStageThreads) void unsplit_staged(UnsplitArgs a) {
   const size_t fvs = (size_t)nv * 27;
   using unsplit_dev::row_faces;
   if (d == 0)
-    row_faces<R, 0, RS, false, false>(c, sc, a1, b1, (R)(dt / a.dx[0]), flux, tmp, fvs, make_scale(dt, a.dx[0]),
+    row_faces<R, 0, RS, false, false, ST>(c, sc, a1, b1, (R)(dt / a.dx[0]), flux, tmp, fvs, make_scale(dt, a.dx[0]),
                                       nullptr);
   else if (d == 1)
-    row_faces<R, 1, RS, false, false>(c, sc, a1, b1, (R)(dt / a.dx[1]), flux, tmp, fvs, make_scale(dt, a.dx[1]),
+    row_faces<R, 1, RS, false, false, ST>(c, sc, a1, b1, (R)(dt / a.dx[1]), flux, tmp, fvs, make_scale(dt, a.dx[1]),
                                       nullptr);
   else
-    row_faces<R, 2, RS, false, false>(c, sc, a1, b1, (R)(dt / a.dx[2]), flux, tmp, fvs, make_scale(dt, a.dx[2]),
+    row_faces<R, 2, RS, false, false, ST>(c, sc, a1, b1, (R)(dt / a.dx[2]), flux, tmp, fvs, make_scale(dt, a.dx[2]),
                                       nullptr);
 }
 
@@ -165,15 +165,28 @@ __global__ __launch_bounds__(kStageThreads) void unsplit_staged(UnsplitArgs a) {
 // workgroup leave 2.25 waves per SIMD, and the FP64 kernel is not bound by
 // its primitive evaluations (a staged workgroup of 8 or 32 octs was slower
 // still in both precisions).
-template <typename R, int RS, bool S27>
-static void launch_t(const UnsplitArgs& a, long grid, int block, hipStream_t stream) {
+//
+// Minmod, the reference's default slope (hydro/hydro_parameters.f90:76),
+// has kernels with the limiter fixed at compile time (ST = 1); the other
+// 7-point types take it at run time (ST = -1).
+template <typename R, int RS, bool S27, int ST>
+static void launch_st(const UnsplitArgs& a, long grid, int block, hipStream_t stream) {
   if constexpr (!S27 && std::is_same<R, float>::value) {
     if (a.nv == 32) {
-      hipLaunchKernelGGL((unsplit_staged<R, RS>), dim3((unsigned)(a.ngroups * 2)), dim3(kStageThreads), 0, stream, a);
+      hipLaunchKernelGGL((unsplit_staged<R, RS, ST>), dim3((unsigned)(a.ngroups * 2)), dim3(kStageThreads), 0, stream,
+                         a);
       return;
     }
   }
-  hipLaunchKernelGGL((unsplit_rows<R, RS, S27>), dim3((unsigned)grid), dim3(block), 0, stream, a);
+  hipLaunchKernelGGL((unsplit_rows<R, RS, S27, ST>), dim3((unsigned)grid), dim3(block), 0, stream, a);
+}
+
+template <typename R, int RS, bool S27>
+static void launch_t(const UnsplitArgs& a, long grid, int block, hipStream_t stream) {
+  if (!S27 && a.hp.slope_type == 1)
+    launch_st<R, RS, S27, 1>(a, grid, block, stream);
+  else
+    launch_st<R, RS, S27, -1>(a, grid, block, stream);
 }
 
 template <typename R>

@@ -114,8 +114,9 @@ __device__ __forceinline__ void store_face(double* flux, double* tmp, size_t fba
 // reference's reset leaves there (hydro/godunov_fine.f90:720-749).
 // Ctx supplies prim(i,j,k) and nv: RowCtx evaluates the primitives from
 // uin, and the staged kernel of unsplit.hip reads them from LDS (the 27-point
-// slopes need RowCtx).
-template <typename R, int D, int RS, bool S27, bool ZR, typename Ctx>
+// slopes need RowCtx).  ST: the 7-point slope type fixed at compile time, or
+// -1 (see slope1).
+template <typename R, int D, int RS, bool S27, bool ZR, int ST, typename Ctx>
 __device__ __forceinline__ void row_faces(const Ctx& c, const SolverConsts<R>& sc, int a1, int b1,
                                           R dtdx, double* flux, double* tmp, size_t fvs, const FluxScale fs,
                                           const unsigned char* ok) {
@@ -142,13 +143,13 @@ __device__ __forceinline__ void row_faces(const Ctx& c, const SolverConsts<R>& s
       dq1 = (T1 == 0) ? gx : gy;
       dq2 = (T2 == 1) ? gy : gz;
     } else {
-      dqd = slope_prim(qa, qb, qc, sc.slope_type, sc.theta);
+      dqd = slope_prim<R, ST>(qa, qb, qc, sc.slope_type, sc.theta);
       const Prim<R> m1 = (T1 == 0) ? c.prim(i - 1, j, k) : c.prim(i, j - 1, k);
       const Prim<R> p1 = (T1 == 0) ? c.prim(i + 1, j, k) : c.prim(i, j + 1, k);
-      dq1 = slope_prim(m1, qb, p1, sc.slope_type, sc.theta);
+      dq1 = slope_prim<R, ST>(m1, qb, p1, sc.slope_type, sc.theta);
       const Prim<R> m2 = (T2 == 1) ? c.prim(i, j - 1, k) : c.prim(i, j, k - 1);
       const Prim<R> p2 = (T2 == 1) ? c.prim(i, j + 1, k) : c.prim(i, j, k + 1);
-      dq2 = slope_prim(m2, qb, p2, sc.slope_type, sc.theta);
+      dq2 = slope_prim<R, ST>(m2, qb, p2, sc.slope_type, sc.theta);
     }
     const Prim<R>& dqx = (D == 0) ? dqd : dq1;
     const Prim<R>& dqy = (D == 1) ? dqd : ((D == 0) ? dq1 : dq2);
@@ -180,7 +181,7 @@ __device__ __forceinline__ void row_faces(const Ctx& c, const SolverConsts<R>& s
 // Slopes of cell (i,j,k) and its traced states along D (qp on the minus
 // side, qm on the plus side), exactly as row_faces computes them for a cell
 // of a row: the same routines on the same primitives.
-template <typename R, int D, bool S27>
+template <typename R, int D, bool S27, int ST>
 __device__ __forceinline__ void trace_cell(const RowCtx<R>& c, const SolverConsts<R>& sc, int i, int j, int k,
                                            R dtdx, Prim<R>& qp, Prim<R>& qm) {
   const Prim<R> q = c.prim(i, j, k);
@@ -188,23 +189,23 @@ __device__ __forceinline__ void trace_cell(const RowCtx<R>& c, const SolverConst
   if (S27) {
     slope_27(c, i, j, k, dqx, dqy, dqz);
   } else {
-    dqx = slope_prim(c.prim(i - 1, j, k), q, c.prim(i + 1, j, k), sc.slope_type, sc.theta);
-    dqy = slope_prim(c.prim(i, j - 1, k), q, c.prim(i, j + 1, k), sc.slope_type, sc.theta);
-    dqz = slope_prim(c.prim(i, j, k - 1), q, c.prim(i, j, k + 1), sc.slope_type, sc.theta);
+    dqx = slope_prim<R, ST>(c.prim(i - 1, j, k), q, c.prim(i + 1, j, k), sc.slope_type, sc.theta);
+    dqy = slope_prim<R, ST>(c.prim(i, j - 1, k), q, c.prim(i, j + 1, k), sc.slope_type, sc.theta);
+    dqz = slope_prim<R, ST>(c.prim(i, j, k - 1), q, c.prim(i, j, k + 1), sc.slope_type, sc.theta);
   }
   const Source<R> src = trace_sources(q, dqx, dqy, dqz, sc.gamma);
   trace_dir(q, D == 0 ? dqx : (D == 1 ? dqy : dqz), src, dtdx, sc.smallr, qp, qm);
 }
 
 // The face of direction D whose right cell is (i,j,k) (ZR: see row_faces).
-template <typename R, int D, int RS, bool S27, bool ZR>
+template <typename R, int D, int RS, bool S27, bool ZR, int ST>
 __device__ __forceinline__ void face_dir(const RowCtx<R>& c, const SolverConsts<R>& sc, int i, int j, int k,
                                          R dtdx, double* flux, double* tmp, size_t fvs, const FluxScale fs,
                                          const unsigned char* ok) {
   constexpr int di = D == 0, dj = D == 1, dk = D == 2;
   Prim<R> qpL, qmL, qpR, qmR;
-  trace_cell<R, D, S27>(c, sc, i - di, j - dj, k - dk, dtdx, qpL, qmL);
-  trace_cell<R, D, S27>(c, sc, i, j, k, dtdx, qpR, qmR);
+  trace_cell<R, D, S27, ST>(c, sc, i - di, j - dj, k - dk, dtdx, qpL, qmL);
+  trace_cell<R, D, S27, ST>(c, sc, i, j, k, dtdx, qpR, qmR);
   const Face<R> fl = rotate(qmL, D), fr = rotate(qpR, D);
   const Flux6<R> g = riemann_fixed<R, RS>(fl, fr, sc);
   const size_t fslot = (size_t)((i - 1) + 3 * ((j - 1) + 3 * (k - 1))) * c.nv;
@@ -223,7 +224,7 @@ __device__ __forceinline__ void face_dir(const RowCtx<R>& c, const SolverConsts<
 // decomposition for a third of its chain of dependent steps, which is the
 // better trade where a launch holds few octs.  Same routines, same inputs,
 // same bits.
-template <typename R, int RS, bool S27, bool ZR = false>
+template <typename R, int RS, bool S27, bool ZR = false, int ST = -1>
 __device__ __forceinline__ void face_task(const UnsplitArgs& a, long t, long ngroups, int ngrid_last) {
   const int nv = a.nv;
   const long per_group = 36L * nv;
@@ -257,15 +258,15 @@ __device__ __forceinline__ void face_task(const UnsplitArgs& a, long t, long ngr
   const size_t fvs = (size_t)nv * 27;
   const unsigned char* ok = ZR ? a.ok + gidx * (size_t)nv * 216 + l : nullptr;
   if (d == 0)
-    face_dir<R, 0, RS, S27, ZR>(c, sc, i, j, k, (R)(dt / a.dx[0]), flux, tmp, fvs, make_scale(dt, a.dx[0]), ok);
+    face_dir<R, 0, RS, S27, ZR, ST>(c, sc, i, j, k, (R)(dt / a.dx[0]), flux, tmp, fvs, make_scale(dt, a.dx[0]), ok);
   else if (d == 1)
-    face_dir<R, 1, RS, S27, ZR>(c, sc, i, j, k, (R)(dt / a.dx[1]), flux, tmp, fvs, make_scale(dt, a.dx[1]), ok);
+    face_dir<R, 1, RS, S27, ZR, ST>(c, sc, i, j, k, (R)(dt / a.dx[1]), flux, tmp, fvs, make_scale(dt, a.dx[1]), ok);
   else
-    face_dir<R, 2, RS, S27, ZR>(c, sc, i, j, k, (R)(dt / a.dx[2]), flux, tmp, fvs, make_scale(dt, a.dx[2]), ok);
+    face_dir<R, 2, RS, S27, ZR, ST>(c, sc, i, j, k, (R)(dt / a.dx[2]), flux, tmp, fvs, make_scale(dt, a.dx[2]), ok);
 }
 
-// One (group, row, lane) task of the flat decomposition (ZR: see row_faces).
-template <typename R, int RS, bool S27, bool ZR = false>
+// One (group, row, lane) task of the flat decomposition (ZR, ST: see row_faces).
+template <typename R, int RS, bool S27, bool ZR = false, int ST = -1>
 __device__ __forceinline__ void unsplit_task(const UnsplitArgs& a, long t, long ngroups, int ngrid_last) {
   const int nv = a.nv;
   const long per_group = 12L * nv;
@@ -302,11 +303,11 @@ __device__ __forceinline__ void unsplit_task(const UnsplitArgs& a, long t, long 
   const size_t fvs = (size_t)nv * 27;
   const unsigned char* ok = ZR ? a.ok + gidx * (size_t)nv * 216 + l : nullptr;
   if (d == 0)
-    row_faces<R, 0, RS, S27, ZR>(c, sc, a1, b1, (R)(dt / a.dx[0]), flux, tmp, fvs, make_scale(dt, a.dx[0]), ok);  // :516
+    row_faces<R, 0, RS, S27, ZR, ST>(c, sc, a1, b1, (R)(dt / a.dx[0]), flux, tmp, fvs, make_scale(dt, a.dx[0]), ok);  // :516
   else if (d == 1)
-    row_faces<R, 1, RS, S27, ZR>(c, sc, a1, b1, (R)(dt / a.dx[1]), flux, tmp, fvs, make_scale(dt, a.dx[1]), ok);
+    row_faces<R, 1, RS, S27, ZR, ST>(c, sc, a1, b1, (R)(dt / a.dx[1]), flux, tmp, fvs, make_scale(dt, a.dx[1]), ok);
   else
-    row_faces<R, 2, RS, S27, ZR>(c, sc, a1, b1, (R)(dt / a.dx[2]), flux, tmp, fvs, make_scale(dt, a.dx[2]), ok);
+    row_faces<R, 2, RS, S27, ZR, ST>(c, sc, a1, b1, (R)(dt / a.dx[2]), flux, tmp, fvs, make_scale(dt, a.dx[2]), ok);
 }
 
 }  // namespace unsplit_dev
