@@ -174,23 +174,20 @@ __device__ __forceinline__ void level_tile(const double* __restrict__ u, double*
   __syncthreads();
   if (active) qcur = read_q(1, qo);
 
-  // One plane of the march.  kSteady: a plane with x/y faces, a z face and
-  // a plane below to finish (every plane but z0-1, z0 and z0+TZ), so those
-  // tests fold away.  kWaves: 0 for a wave of interior cells and 1 for the
-  // ring's wave, whose cell kinds are then known at compile time (interior
-  // or not) -- the cells of a wave are all interior or all ring, NI being a
-  // multiple of 64 -- or 2 to test them at run time.
-  const bool active_run = active, interior_run = interior;
+  // One plane of the march.  kWaves: 0 for a wave of interior cells, 1 for
+  // the ring's wave; the cell kinds of a wave are then known at compile time
+  // (the cells of a wave are all interior or all ring, NI being a multiple
+  // of 64).
+  const bool active_run = active;
   const int kind_run = kind;
-  auto plane_step = [&](auto steady_tag, auto waves_tag, int p) {
-    constexpr bool kSteady = decltype(steady_tag)::value;
+  auto plane_step = [&](auto waves_tag, int p) {
     constexpr int kWaves = decltype(waves_tag)::value;
-    const bool interior = kWaves == 0 ? true : (kWaves == 1 ? false : interior_run);
+    const bool interior = kWaves == 0;
     const bool active = kWaves == 0 ? true : active_run;
     const int kind = kWaves == 0 ? (int)kInterior : kind_run;
     const int scur = (p - z0 + 2) & 1;  // slot of plane p (z0-2 -> 0, z0-1 -> 1)
     const int snext = scur ^ 1;
-    const bool faces_xy = kSteady || ((p >= z0) && (p < z0 + TZ));
+    const bool faces_xy = (p >= z0) && (p < z0 + TZ);
     // FP32-mixed: the own cell's conserved state for the x/y update is
     // loaded before the next plane, so its latency hides behind the plane's
     // work (in FP64 the 10 more live registers would spill)
@@ -297,7 +294,7 @@ __device__ __forceinline__ void level_tile(const double* __restrict__ u, double*
     }
     // z face below plane p (between p-1 and p)
     double fz[5] = {0, 0, 0, 0, 0};
-    if (interior && (kSteady || p >= z0)) {
+    if (interior && p >= z0) {
       const Flux6<R> g = riemann_fixed<R, RS>(fz_left, rotate(qp_z, 2), sc);
       R fc[5];
       unrotate(g, 2, fc);
@@ -305,7 +302,7 @@ __device__ __forceinline__ void level_tile(const double* __restrict__ u, double*
       for (int v = 0; v < 5; ++v) fz[v] = scale_flux(fc[v], fs);  // :153
     }
     // plane p-1 is complete: hydro/godunov_fine.f90:751-792 in idim order x, y, z
-    if (interior && (kSteady || p >= z0 + 1)) {
+    if (interior && p >= z0 + 1) {
       const int zg = wrapi(p - 1, n);
       const long o = (long)zg * plane + (long)yg * n + xg;
       double un[5];
@@ -331,24 +328,18 @@ __device__ __forceinline__ void level_tile(const double* __restrict__ u, double*
     qprev = qcur;
     qcur = qnext;
   };
-  // FP64: the interior waves and the ring's wave run copies of the march
-  // with their cell kinds fixed, both meeting the same barriers in the same
-  // order (5.5% faster at 512^3).  FP32-mixed keeps one march with the tests:
-  // there the two copies took 149 VGPRs, below 4 waves per SIMD, and ran
-  // slower, and a loop of its own for the steady planes (1-3% faster on
-  // 16x12 tiles) took the 16x8 and serving kernels past 128 VGPRs.  (In FP64
-  // that steady loop spilled 48 B and ran 9% slower.)
+  // The interior waves and the ring's wave run copies of the march with
+  // their cell kinds fixed, both meeting the same barriers in the same order
+  // (at 512^3: FP64 5.5% faster, FP32-mixed 3.5% once held to 128 VGPRs,
+  // where it takes 117).  A further copy for the planes between the first
+  // two and the last, with their tests folded, spilled 48 B in FP64 and ran
+  // 9% slower.
   using kInteriorWaves = std::integral_constant<int, 0>;
   using kRingWave = std::integral_constant<int, 1>;
-  using kAnyWave = std::integral_constant<int, 2>;
-  if constexpr (sizeof(R) == 8) {
-    if (t >= NI) {
-      for (int p = z0 - 1; p <= z0 + TZ; ++p) plane_step(std::false_type{}, kRingWave{}, p);
-    } else {
-      for (int p = z0 - 1; p <= z0 + TZ; ++p) plane_step(std::false_type{}, kInteriorWaves{}, p);
-    }
+  if (t >= NI) {
+    for (int p = z0 - 1; p <= z0 + TZ; ++p) plane_step(kRingWave{}, p);
   } else {
-    for (int p = z0 - 1; p <= z0 + TZ; ++p) plane_step(std::false_type{}, kAnyWave{}, p);
+    for (int p = z0 - 1; p <= z0 + TZ; ++p) plane_step(kInteriorWaves{}, p);
   }
 
   if (d_dt_next) {
@@ -372,8 +363,11 @@ __device__ __forceinline__ void level_tile(const double* __restrict__ u, double*
 
 
 template <typename R, int TX, int TY, int RS, bool SUMS, int ST>
+// The register budget: 3 waves per SIMD in FP64 (166 VGPRs), 4 in FP32-mixed
+// (at most 128).
 __global__ __launch_bounds__((tile_threads<TX, TY>()))
-__attribute__((amdgpu_waves_per_eu(3, 8))) void uniform_godunov_kernel(UniformArgs a, int TZ, int y0, int boff) {
+__attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 3, 8))) void uniform_godunov_kernel(UniformArgs a, int TZ,
+                                                                                           int y0, int boff) {
   // XCD-aware tile order: the hardware deals workgroups round-robin to the
   // 8 XCDs, so workgroup b runs on XCD b%8.  Giving each XCD a contiguous
   // range of tiles (x fastest, then y, then z chunk) keeps the tiles that run
@@ -397,7 +391,7 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void uniform_godunov_kernel(UniformAr
 // `done` is skipped.  The slot's buffer parity selects input and output.
 template <typename R, int TX, int TY, int RS, int ST>
 __global__ __launch_bounds__((tile_threads<TX, TY>()))
-__attribute__((amdgpu_waves_per_eu(3, 8))) void batch_godunov_kernel(BatchArgs a) {
+__attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 3, 8))) void batch_godunov_kernel(BatchArgs a) {
   const int b = blockIdx.x;
   int s = 0;
   while (s + 1 < a.nslots && b >= a.blk0[s + 1]) ++s;
