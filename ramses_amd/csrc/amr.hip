@@ -490,10 +490,11 @@ __global__ void amr_dt_kernel(int op, double* dt, double* dtold, double* time, i
 // workgroup gathers chunks of kConsChunk terms of that order into LDS, many
 // threads loading in parallel, so the tree walks and loads of a chunk
 // overlap.  Thread 0 adds a chunk's terms in order while the wavefronts
-// after the first gather the next chunk into the second buffer.  A cell that is not a leaf adds +0.0, which leaves the positive
-// running sum unchanged.  With dn the level counts are read from device
-// memory and `armed` (nonzero: nstep_coarse changed since the last
-// update_time) decides at run time.
+// after the first gather the next chunk into the second buffer.  A cell
+// that is not a leaf adds +0.0, which leaves the positive running sum
+// unchanged.  With dn the level counts are read from device memory and
+// `armed` (nonzero: nstep_coarse changed since the last update_time)
+// decides at run time.
 constexpr int kConsChunk = 2048;
 constexpr int kConsThreads = 1024;
 
