@@ -109,15 +109,14 @@ def time_variant(torch, ra, uin, flux, tmp, dt, riemann, precision, steps, warmu
                 ms_per_step=1e3 * wall / steps, block=tuned[0] if tuned else None)
 
 
-def unsplit_cpu_baseline(budget_s=12.0):
-    """The reference's own unsplit (oracle/_ref) on one core, bounded sample
-    of the same input distribution (numpy default_rng(1) stencils)."""
+def unsplit_cpu_baseline(dt, budget_s=12.0):
+    """The reference's own unsplit (oracle/_ref) on one core over the first
+    32 batches of the benchmark's own input (the first chunk of
+    ics.oct_batches_host with seed 1) with its dt, a bounded sample."""
     import numpy as np
     import oracle
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from helpers import random_stencils
-    rng = np.random.default_rng(1)
-    groups = random_stencils(rng, 32)
+    from ramses_amd import ics
+    groups = ics.oct_batches_host_chunk(np.random.default_rng(1), 1024, NV, GAMMA, DX, CFL)[0][:32]
     if oracle.reference_available():
         impl, kind = oracle.Reference(), "reference"
     else:
@@ -128,7 +127,7 @@ def unsplit_cpu_baseline(budget_s=12.0):
     n = 0
     t0 = time.perf_counter()
     while True:
-        impl.unsplit(np.ascontiguousarray(groups[n % 32]), None, flux, tmp, DX, 1e-4, NV)
+        impl.unsplit(np.ascontiguousarray(groups[n % 32]), None, flux, tmp, DX, dt, NV)
         n += 1
         el = time.perf_counter() - t0
         if el > budget_s and n >= 32:
@@ -136,7 +135,8 @@ def unsplit_cpu_baseline(budget_s=12.0):
     cells = 8 * NV * n
     return {"value": cells / el, "unit": "cell-updates/s", "cores": 1, "kind": kind,
             "sample": f"{n} unsplit calls of {NV} octs ({cells} cell-updates) in {el:.1f} s, hllc minmod, "
-                      f"same input distribution; {'flang -O2 build of hydro/umuscl.f90' if kind == 'reference' else 'C restatement'}"}
+                      f"the first 32 batches of the benchmark's input, cycled; "
+                      f"{'flang -O2 build of hydro/umuscl.f90' if kind == 'reference' else 'C restatement'}"}
 
 
 def progress(msg):
@@ -204,6 +204,24 @@ def profiled_traffic(workload, precision, tile, tz):
         if m and m.group(1) == workload and m.group(2) == precision and int(m.group(3)) == tile \
                 and int(m.group(4)) == tz:
             return int(m.group(7))
+    return None
+
+
+def profiled_unsplit_traffic(variant, block):
+    """HBM bytes per launch of the batched unsplit kernel from the committed
+    counter summary (profiles/unsplit_rows_2p20_octs.md): lines 'traffic
+    unsplit <variant> block=<b> fetch_kib=<F> write_kib=<W> bytes=<B>', with
+    B as in profiled_traffic.  None unless a line matches the variant and
+    its workgroup size."""
+    import re
+    path = os.path.join(ROOT, "profiles", "unsplit_rows_2p20_octs.md")
+    if not os.path.exists(path):
+        return None
+    pat = re.compile(r"^traffic unsplit (\S+) block=(\d+) fetch_kib=(\d+) write_kib=(\d+) bytes=(\d+)\s*$")
+    for line in open(path):
+        m = pat.match(line.strip())
+        if m and m.group(1) == variant and int(m.group(2)) == block:
+            return int(m.group(5))
     return None
 
 
@@ -580,7 +598,8 @@ def main():
 
     noct = args.octs
     from ramses_amd import ics
-    uin, dt = ics.oct_batches(noct, NV, seed=1, gamma=GAMMA, dx=DX, cfl=CFL)
+    uin, dt = ics.oct_batches_host(noct, NV, seed=1, gamma=GAMMA, dx=DX, cfl=CFL, progress=progress)
+    progress("oct batches resident")
     G = noct // NV
     flux = torch.zeros((G, 3, 5, 3, 3, 3, NV), dtype=torch.float64, device="cuda")
     tmp = torch.zeros((G, 3, 2, 3, 3, 3, NV), dtype=torch.float64, device="cuda")
@@ -594,13 +613,15 @@ def main():
     ra.configure()
 
     cells = 8 * noct
-    head = results.get("fp64-hllc") or next(iter(results.values()))
+    head_key = "fp64-hllc" if "fp64-hllc" in results else next(iter(results))
+    head = results[head_key]
     alg_bytes = BYTES_PER_OCT * noct
 
-    def roof(r):
+    def roof(r, variant):
         ach = alg_bytes / (r["kernel_ms_avg"] * 1e-3)
+        traffic = profiled_unsplit_traffic(variant, r["block"]) if noct == NOCT else None
         return {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": ach / HBM_PEAK, "traffic": None,
+                "frac": ach / HBM_PEAK, "traffic": traffic,
                 # the same time against the bytes the kernel needs: the 80 of
                 # 216 stencil cells that some face depends on, and the 36 faces
                 "frac_needed_bytes": NEEDED_BYTES_PER_OCT * noct / (r["kernel_ms_avg"] * 1e-3) / HBM_PEAK}
@@ -609,8 +630,8 @@ def main():
     for k, r in results.items():
         variants[k] = {"cell_updates_per_s": cells / (r["kernel_ms_avg"] * 1e-3),
                        "kernel_ms_avg": r["kernel_ms_avg"], "p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"],
-                       "roofline_frac": roof(r)["frac"], "roofline_frac_needed_bytes": roof(r)["frac_needed_bytes"],
-                       "workgroup": r["block"]}
+                       "roofline_frac": roof(r, k)["frac"], "roofline_frac_needed_bytes": roof(r, k)["frac_needed_bytes"],
+                       "traffic": roof(r, k)["traffic"], "workgroup": r["block"]}
     line = {
         "metric": "cell-updates/s FP64 vs FP32-mixed, % HBM roofline; p50/p99 solve latency",
         "value": cells / (head["ms_per_step"] * 1e-3),
@@ -623,12 +644,12 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (device Philox, seed 1)",
+        "data": "synthetic: numpy default_rng(1) stencils (ics.oct_batches_host), resident in HBM",
         "config": {"workload": "unsplit oct batches: 2^20 octs (8.4M cells) as 6x6x6 stencils, nvector=32, "
                                "NVAR=5, hllc, minmod, gamma=5/3, dx=1/128, dt at CFL 0.5",
                    "octs": noct, "nvector": NV, "riemann": "hllc", "slope_type": 1, "parallelism": "1 GPU"},
         "latency_ms": {"p50": head["p50_ms"], "p99": head["p99_ms"]},
-        "roofline": roof(head),
+        "roofline": roof(head, head_key),
         "variants": variants,
     }
     if "fp32-hllc" in variants:
@@ -636,7 +657,7 @@ def main():
         line["fp32_over_fp64"] = variants["fp32-hllc"]["cell_updates_per_s"] / variants["fp64-hllc"]["cell_updates_per_s"] \
             if "fp64-hllc" in variants else None
     if not args.no_cpu:
-        cb = unsplit_cpu_baseline()
+        cb = unsplit_cpu_baseline(dt)
         line["cpu_baseline"] = cb
     print(json.dumps(line), flush=True)
 

@@ -16,6 +16,10 @@
   draws torch Philox noise on the GPU instead (fast enough to create a
   request's field inside a serving loop, but only reproducible with torch).
 Both return conserved state u[var][z][y][x] as a float64 tensor.
+* ``oct_batches(noct)`` / ``oct_batches_host(noct)`` -- synthetic gathered
+  oct stencils for the batched unsplit boundary: torch Philox noise drawn on
+  the device, or numpy ``default_rng(seed)`` noise drawn on the host in
+  chunks and copied over (the benchmark's input, regenerable without torch).
 * ``region_condinit(x, dx, regions)`` -- the &INIT_PARAMS regions evaluated
   at arbitrary cell centres x (N, 3) with the cell size dx of their level,
   as init_flow_fine calls condinit on every level of an oct tree; returns
@@ -126,8 +130,9 @@ def oct_batches(noct, nv=32, seed=1, gamma=5.0 / 3.0, dx=1.0 / 128, cfl=0.5, dev
     to conserved variables; and dt = the smallest cmpdt time step at CFL
     `cfl` without gravity (hydro/godunov_utils.f90:5-120) over the 8 centre
     cells of every oct.  The noise is torch Philox drawn on the device in
-    chunks of `chunk` batches (2^20 octs are 9 GB of stencils, too many to
-    draw on the host and upload in a benchmark's set-up)."""
+    chunks of `chunk` batches (2^20 octs, 9 GB of stencils, in well under a
+    second; oct_batches_host draws the same distribution with numpy in about
+    30 s)."""
     import torch
     G = noct // nv
     uin = torch.empty((G, 5, 6, 6, 6, nv), dtype=torch.float64, device=device)
@@ -150,6 +155,51 @@ def oct_batches(noct, nv=32, seed=1, gamma=5.0 / 3.0, dx=1.0 / 128, cfl=0.5, dev
         dtc = dx / w * ((1.0 + 2.0 * cfl * r) ** 0.5 - 1.0) / r
         dtmin = min(dtmin, float(dtc.min()))
         del rho, v, p, c, w, dtc
+    return uin, dtmin
+
+
+def oct_batches_host_chunk(rng, ngroups, nv=32, gamma=5.0 / 3.0, dx=1.0 / 128, cfl=0.5):
+    """ngroups batches of the distribution of oct_batches drawn with numpy:
+    rho = exp(N(0,1)), then the three velocity components, then P =
+    exp(-0.5 + 0.5 N(0,1)), each as one standard_normal call of shape
+    (ngroups, 6, 6, 6, nv) in that order from `rng`.  Returns the conserved
+    batches uin[g][5][6][6][6][nv] and the chunk's smallest cmpdt time step
+    (see oct_batches)."""
+    import numpy as np
+    shp = (ngroups, 6, 6, 6, nv)
+    rho = np.exp(rng.standard_normal(shp))
+    v = rng.standard_normal((3,) + shp)
+    p = np.exp(-0.5 + 0.5 * rng.standard_normal(shp))
+    u = np.empty((ngroups, 5) + shp[1:])
+    u[:, 0] = rho
+    u[:, 1:4] = np.moveaxis(rho[None] * v, 0, 1)
+    u[:, 4] = p / (gamma - 1.0) + 0.5 * rho * (v * v).sum(0)
+    c = np.sqrt(gamma * p[:, 1:3, 1:3, 1:3] / rho[:, 1:3, 1:3, 1:3])
+    w = 3.0 * c + np.abs(v[:, :, 1:3, 1:3, 1:3]).sum(0)
+    r = 1e-4
+    dtc = dx / w * ((1.0 + 2.0 * cfl * r) ** 0.5 - 1.0) / r
+    return u, float(dtc.min())
+
+
+def oct_batches_host(noct, nv=32, seed=1, gamma=5.0 / 3.0, dx=1.0 / 128, cfl=0.5, device="cuda", chunk=1024,
+                     progress=None):
+    """oct_batches with numpy default_rng(seed) noise, drawn on the host in
+    chunks of `chunk` batches (oct_batches_host_chunk, chunk after chunk from
+    one generator) and copied to `device`, so the inputs can be regenerated
+    without torch.  The first chunk alone gives the first `chunk` batches."""
+    import numpy as np
+    import torch
+    G = noct // nv
+    uin = torch.empty((G, 5, 6, 6, 6, nv), dtype=torch.float64, device=device)
+    rng = np.random.default_rng(seed)
+    dtmin = float("inf")
+    for g0 in range(0, G, chunk):
+        g1 = min(G, g0 + chunk)
+        u, dt = oct_batches_host_chunk(rng, g1 - g0, nv, gamma, dx, cfl)
+        uin[g0:g1].copy_(torch.from_numpy(u))
+        dtmin = min(dtmin, dt)
+        if progress and (g0 // chunk) % 8 == 7:
+            progress(f"{g1} of {G} batches drawn")
     return uin, dtmin
 
 
