@@ -78,7 +78,7 @@ def tune_level(run, n, key, verbose=False):
         for z in tzs:
             _set(LEVEL_TILE, t)
             _set(LEVEL_TZ, z)
-            ms = _time(run, reps=3)
+            ms = _time(run, reps=5)  # the median of 5: shapes within a few percent of each other
             if verbose:
                 print(f"autotune level tile={t} tz={z}: {ms:.3f} ms")
             if best is None or ms < best[1]:
