@@ -489,8 +489,9 @@ __global__ void amr_dt_kernel(int op, double* dt, double* dtold, double* time, i
 // grid in the batch (:44-102) -- so the values have its rounding.  The
 // workgroup gathers chunks of kConsChunk terms of that order into LDS, many
 // threads loading in parallel, so the tree walks and loads of a chunk
-// overlap.  Thread 0 adds a chunk's terms in order while the wavefronts
-// after the first gather the next chunk into the second buffer.  A cell
+// overlap.  Threads 0 and 64 add a chunk's mass and energy terms in order
+// while the wavefronts after the first two gather the next chunk into the
+// second buffer.  A cell
 // that is not a leaf adds +0.0, which leaves the positive running sum
 // unchanged.  With dn the level counts are read from device memory and
 // `armed` (nonzero: nstep_coarse changed since the last update_time)
@@ -524,26 +525,22 @@ __device__ __forceinline__ void cons_gather(const int* active, int n, int nv, Am
   }
 }
 
-// Adds the first cnt terms of tp to m, e in order (one thread); the LDS
-// loads run ahead of the two dependent chains of additions, which bound the
-// kernel (about 20 cycles a term).
-__device__ __forceinline__ void cons_add(const double2* tp, int cnt, double& m, double& e) {
+// Adds component `which` (0: mass, 1: energy) of the first cnt terms of tp
+// to acc, in order, by one thread.  The LDS loads run ahead of the chain of
+// additions.  Mass and energy are summed by threads of two different
+// wavefronts: an FP64 addition occupies its SIMD for 8 cycles whatever the
+// number of active lanes, so the two chains in one wavefront shared one
+// SIMD (2.5 ms for the adaptive Sedov run's baselines, 2.0 ms split).
+__device__ __forceinline__ void cons_add(const double2* tp, int cnt, int which, double& acc) {
   int k0 = 0;
   for (; k0 + 16 <= cnt; k0 += 16) {
-    double2 v[16];
+    double v[16];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) v[j] = tp[k0 + j];
+    for (int j = 0; j < 16; ++j) v[j] = which == 0 ? tp[k0 + j].x : tp[k0 + j].y;
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      m = m + v[j].x;
-      e = e + v[j].y;
-    }
+    for (int j = 0; j < 16; ++j) acc = acc + v[j];
   }
-  for (; k0 < cnt; ++k0) {
-    const double2 v = tp[k0];
-    m = m + v.x;
-    e = e + v.y;
-  }
+  for (; k0 < cnt; ++k0) acc = acc + (which == 0 ? tp[k0].x : tp[k0].y);
 }
 
 __global__ __launch_bounds__(kConsThreads) void amr_cons0_kernel(ConsLevels lv, const int* dn, AmrTreeView t,
@@ -553,8 +550,9 @@ __global__ __launch_bounds__(kConsThreads) void amr_cons0_kernel(ConsLevels lv, 
   __shared__ double2 tp[2][kConsChunk];  // double-buffered chunks of (mass, energy) terms
   if (armed && *armed == 0) return;
   if (cons0[0] != 0.0 && cons0[1] != 0.0) return;
-  const bool first_wave = threadIdx.x < 64;
-  double mass_tot = 0.0, ekin_tot = 0.0;  // thread 0's
+  const bool adder_waves = threadIdx.x < 128;
+  const int which = threadIdx.x == 0 ? 0 : 1;  // thread 0 sums the mass, thread 64 the energy
+  double tot = 0.0;                            // threads 0 and 64: mass_tot, ekin_tot
   for (int l = levelmin; l <= nlevelmax; ++l) {
     const int n = dn ? dn[l] : lv.noct[l];
     if (n <= 0) break;
@@ -563,26 +561,24 @@ __global__ __launch_bounds__(kConsThreads) void amr_cons0_kernel(ConsLevels lv, 
     const double vol = dx * dx * dx;
     const long nterm = 8L * n;
     const long nchunk = (nterm + kConsChunk - 1) / kConsChunk;
-    double m = 0.0, e = 0.0;
+    double acc = 0.0;
     cons_gather(active, n, nv, t, u, ustride, vol, 0, 0, tp[0]);
     __syncthreads();
-    // thread 0 adds chunk c while the wavefronts after the first gather c + 1
+    // threads 0 and 64 add chunk c while the wavefronts after the first two
+    // gather c + 1
     for (long c = 0; c < nchunk; ++c) {
       const int b = (int)(c & 1);
-      if (threadIdx.x == 0)
-        cons_add(tp[b], (int)min((long)kConsChunk, nterm - c * kConsChunk), m, e);
-      else if (!first_wave && c + 1 < nchunk)
-        cons_gather(active, n, nv, t, u, ustride, vol, (c + 1) * kConsChunk, 64, tp[b ^ 1]);
+      if (threadIdx.x == 0 || threadIdx.x == 64)
+        cons_add(tp[b], (int)min((long)kConsChunk, nterm - c * kConsChunk), which, acc);
+      else if (!adder_waves && c + 1 < nchunk)
+        cons_gather(active, n, nv, t, u, ustride, vol, (c + 1) * kConsChunk, 128, tp[b ^ 1]);
       __syncthreads();
     }
-    mass_tot = mass_tot + m;
-    ekin_tot = ekin_tot + e;
+    tot = tot + acc;
     if (l == nlevelmax || (dn ? dn[l + 1] : lv.noct[l + 1]) == 0) break;
   }
-  if (threadIdx.x == 0) {
-    if (cons0[1] == 0.0) cons0[1] = mass_tot;        // mass_tot_0
-    if (cons0[0] == 0.0) cons0[0] = 0.0 + ekin_tot;  // einit = epot_tot + ekin_tot
-  }
+  if (threadIdx.x == 0 && cons0[1] == 0.0) cons0[1] = tot;        // mass_tot_0
+  if (threadIdx.x == 64 && cons0[0] == 0.0) cons0[0] = 0.0 + tot;  // einit = epot_tot + ekin_tot
 }
 
 __global__ void clear_bits(int* p, int mask) { p[0] &= ~mask; }
