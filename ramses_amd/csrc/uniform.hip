@@ -212,9 +212,18 @@ __device__ __forceinline__ void level_tile(const double* __restrict__ u, double*
       const Prim<R> dqy = slope_prim<R, ST>(ym, qcur, yp, sc.slope_type, sc.theta);
       const Prim<R> dqz = slope_prim<R, ST>(qprev, qcur, qnext, sc.slope_type, sc.theta);
       const Source<R> src = trace_sources(qcur, dqx, dqy, dqz, sc.gamma);
-      trace_dir(qcur, dqx, src, dtdx, sc.smallr, qp_x, qm_x);
-      trace_dir(qcur, dqy, src, dtdx, sc.smallr, qp_y, qm_y);
-      if (interior) trace_dir(qcur, dqz, src, dtdx, sc.smallr, qp_z, qm_z);  // ring cells solve no z face
+      if constexpr (kWaves == 0) {
+        trace_dir(qcur, dqx, src, dtdx, sc.smallr, qp_x, qm_x);
+        trace_dir(qcur, dqy, src, dtdx, sc.smallr, qp_y, qm_y);
+        trace_dir(qcur, dqz, src, dtdx, sc.smallr, qp_z, qm_z);
+      } else {
+        // a left or right ring cell feeds only an x face, a bottom or top one
+        // only a y face, and none a z face
+        const bool xdir = kind == kLeft || kind == kRight;
+        trace_dir(qcur, xdir ? dqx : dqy, src, dtdx, sc.smallr, qp_x, qm_x);
+        qp_y = qp_x;
+        qm_y = qm_x;
+      }
     }
     double part[5] = {0, 0, 0, 0, 0};
     if (faces_xy) {
