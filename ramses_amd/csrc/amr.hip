@@ -137,6 +137,8 @@ __device__ __forceinline__ void update_cell(const AmrLevelArgs& a, int b, int q,
 // face, lane) task per thread on levels of at most u.face_octs octs, where
 // the chain of dependent steps matters more than the arithmetic, else one
 // (batch, row, lane) task per thread.  Small levels spread over every CU.
+// The levels carry no gravity array (u.grav is null), so the tasks are the
+// gravity-free ones.
 template <typename R, int RS, bool S27, int ST>
 __global__ __launch_bounds__(768) void amr_unsplit(AmrLevelArgs a, UnsplitArgs u) {
   const int noct = level_count(a.noct, a.dnoct);
@@ -147,11 +149,11 @@ __global__ __launch_bounds__(768) void amr_unsplit(AmrLevelArgs a, UnsplitArgs u
   if (noct <= u.face_octs) {
     const long ntask = (long)nb * 36 * nv;
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < ntask; t += stride)
-      unsplit_dev::face_task<R, RS, S27, true, ST>(u, t, nb, ngrid_last);
+      unsplit_dev::face_task<R, RS, S27, true, ST, false>(u, t, nb, ngrid_last);
   } else {
     const long ntask = (long)nb * 12 * nv;
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < ntask; t += stride)
-      unsplit_dev::unsplit_task<R, RS, S27, true, ST>(u, t, nb, ngrid_last);
+      unsplit_dev::unsplit_task<R, RS, S27, true, ST, false>(u, t, nb, ngrid_last);
   }
 }
 

@@ -12,7 +12,13 @@ namespace unsplit_dev {
 // Fortran stencil index (i,j,k in -1..4) -> cell offset in units of nv.
 __device__ __forceinline__ int cell6(int i, int j, int k) { return (i + 1) + 6 * ((j + 1) + 6 * (k + 1)); }
 
-template <typename R>
+// G: whether the call may have a gravity array.  The kernels without one
+// (G = false) carry no test of g, which frees registers (FP64 minmod rows:
+// 126 VGPRs against 162, four waves per SIMD instead of three); zero
+// acceleration enters ctoprim_cell either way, the same operations on the
+// same values.  With G the test stays: loading g unconditionally made the
+// FP64 kernels spill.
+template <typename R, bool G>
 struct RowCtx {
   const double* u;     // group base + lane
   const double* g;     // group gravity base + lane, or null
@@ -25,7 +31,7 @@ struct RowCtx {
     const size_t o = (size_t)cell6(i, j, k) * nv;
     const double u0 = u[o], u1 = u[o + vs], u2 = u[o + 2 * vs], u3 = u[o + 3 * vs], u4 = u[o + 4 * vs];
     double gx = 0.0, gy = 0.0, gz = 0.0;
-    if (g) {
+    if (G && g) {
       gx = g[o];
       gy = g[o + vs];
       gz = g[o + 2 * vs];
@@ -44,8 +50,8 @@ __device__ __forceinline__ void row_cell(int d, int s, int a, int b, int& i, int
 
 // Positivity-preserving 27-point slope (hydro/umuscl.f90:1328-1392) of the
 // cell (i,j,k) for all five variables.
-template <typename R>
-__device__ void slope_27(const RowCtx<R>& c, int i, int j, int k, Prim<R>& dqx, Prim<R>& dqy, Prim<R>& dqz) {
+template <typename R, typename Ctx>
+__device__ void slope_27(const Ctx& c, int i, int j, int k, Prim<R>& dqx, Prim<R>& dqy, Prim<R>& dqz) {
   const Prim<R> q0 = c.prim(i, j, k);
   R vmin[5], vmax[5];
   const R c0[5] = {q0.r, q0.u, q0.v, q0.w, q0.p};
@@ -181,8 +187,8 @@ __device__ __forceinline__ void row_faces(const Ctx& c, const SolverConsts<R>& s
 // Slopes of cell (i,j,k) and its traced states along D (qp on the minus
 // side, qm on the plus side), exactly as row_faces computes them for a cell
 // of a row: the same routines on the same primitives.
-template <typename R, int D, bool S27, int ST>
-__device__ __forceinline__ void trace_cell(const RowCtx<R>& c, const SolverConsts<R>& sc, int i, int j, int k,
+template <typename R, int D, bool S27, int ST, typename Ctx>
+__device__ __forceinline__ void trace_cell(const Ctx& c, const SolverConsts<R>& sc, int i, int j, int k,
                                            R dtdx, Prim<R>& qp, Prim<R>& qm) {
   const Prim<R> q = c.prim(i, j, k);
   Prim<R> dqx, dqy, dqz;
@@ -198,8 +204,8 @@ __device__ __forceinline__ void trace_cell(const RowCtx<R>& c, const SolverConst
 }
 
 // The face of direction D whose right cell is (i,j,k) (ZR: see row_faces).
-template <typename R, int D, int RS, bool S27, bool ZR, int ST>
-__device__ __forceinline__ void face_dir(const RowCtx<R>& c, const SolverConsts<R>& sc, int i, int j, int k,
+template <typename R, int D, int RS, bool S27, bool ZR, int ST, typename Ctx>
+__device__ __forceinline__ void face_dir(const Ctx& c, const SolverConsts<R>& sc, int i, int j, int k,
                                          R dtdx, double* flux, double* tmp, size_t fvs, const FluxScale fs,
                                          const unsigned char* ok) {
   constexpr int di = D == 0, dj = D == 1, dk = D == 2;
@@ -223,8 +229,8 @@ __device__ __forceinline__ void face_dir(const RowCtx<R>& c, const SolverConsts<
 // sharing a row's sliding window: about 1.3x the arithmetic of the row
 // decomposition for a third of its chain of dependent steps, which is the
 // better trade where a launch holds few octs.  Same routines, same inputs,
-// same bits.
-template <typename R, int RS, bool S27, bool ZR = false, int ST = -1>
+// same bits.  G: a.grav is non-null (see RowCtx).
+template <typename R, int RS, bool S27, bool ZR, int ST, bool G>
 __device__ __forceinline__ void face_task(const UnsplitArgs& a, long t, long ngroups, int ngrid_last) {
   const int nv = a.nv;
   const long per_group = 36L * nv;
@@ -244,9 +250,9 @@ __device__ __forceinline__ void face_task(const UnsplitArgs& a, long t, long ngr
   const size_t gsz_t = (size_t)nv * 27 * 2 * 3;
   const HydroParams hp = a.hp;
   const double dt = a.d_dt ? *a.d_dt : a.dt;
-  RowCtx<R> c;
+  RowCtx<R, G> c;
   c.u = a.uin + gidx * gsz_u + l;
-  c.g = a.grav ? a.grav + gidx * gsz_g + l : nullptr;
+  c.g = G ? a.grav + gidx * gsz_g + l : nullptr;
   c.vs = (size_t)nv * 216;
   c.nv = nv;
   c.dtxhalf = dt * 0.5;
@@ -265,8 +271,9 @@ __device__ __forceinline__ void face_task(const UnsplitArgs& a, long t, long ngr
     face_dir<R, 2, RS, S27, ZR, ST>(c, sc, i, j, k, (R)(dt / a.dx[2]), flux, tmp, fvs, make_scale(dt, a.dx[2]), ok);
 }
 
-// One (group, row, lane) task of the flat decomposition (ZR, ST: see row_faces).
-template <typename R, int RS, bool S27, bool ZR = false, int ST = -1>
+// One (group, row, lane) task of the flat decomposition (ZR, ST: see
+// row_faces; G: a.grav is non-null, see RowCtx).
+template <typename R, int RS, bool S27, bool ZR, int ST, bool G>
 __device__ __forceinline__ void unsplit_task(const UnsplitArgs& a, long t, long ngroups, int ngrid_last) {
   const int nv = a.nv;
   const long per_group = 12L * nv;
@@ -288,9 +295,9 @@ __device__ __forceinline__ void unsplit_task(const UnsplitArgs& a, long t, long 
 
   const HydroParams hp = a.hp;
   const double dt = a.d_dt ? *a.d_dt : a.dt;
-  RowCtx<R> c;
+  RowCtx<R, G> c;
   c.u = a.uin + gidx * gsz_u + l;
-  c.g = a.grav ? a.grav + gidx * gsz_g + l : nullptr;
+  c.g = G ? a.grav + gidx * gsz_g + l : nullptr;
   c.vs = (size_t)nv * 216;
   c.nv = nv;
   c.dtxhalf = dt * 0.5;
