@@ -48,17 +48,20 @@ def test_unsplit_golden_fortran_symbol(gpu, golden_unsplit):
         assert np.array_equal(flux[..., :2], d[f"c{ci}_flux"], equal_nan=True), c
 
 
+@pytest.mark.parametrize("gravity", [False, True])
 @pytest.mark.parametrize("riemann", ["llf", "hllc", "hll", "acoustic", "exact"])
 @pytest.mark.parametrize("slope", [0, 1, 2, 3, 7, 8])
-def test_unsplit_device_batched_vs_oracle(gpu, oracle_lib, riemann, slope):
+def test_unsplit_device_batched_vs_oracle(gpu, oracle_lib, riemann, slope, gravity):
     """Bitwise against the oracle; the exact solver's rarefaction branches
     call pow, whose device and host libraries may round the last bit apart,
-    so it is held to the FP64 bar of 1e-12 relative instead."""
+    so it is held to the FP64 bar of 1e-12 relative instead.  Calls with and
+    without a gravity array run different kernel instances (the ones without
+    carry no gravity loads), so every solver and slope type runs both."""
     import torch
     rng = np.random.default_rng(1000 + 10 * slope + len(riemann))
     G = 40
     uin = random_stencils(rng, G)
-    grav = rng.normal(0, 1, (G,) + ra.grav_shape(NV)) if slope in (1, 2) else None
+    grav = rng.normal(0, 1, (G,) + ra.grav_shape(NV)) if gravity else None
     dx, dt, last = 1 / 128, 2e-4, 7
     ra.configure(gamma=5 / 3, riemann=riemann, slope_type=slope)
     oracle_lib.set_params(gamma=5 / 3, riemann=riemann, slope_type=slope)
@@ -81,28 +84,32 @@ def test_unsplit_device_batched_vs_oracle(gpu, oracle_lib, riemann, slope):
 
 
 def test_unsplit_fp32_mixed_error(gpu, oracle_lib):
-    """FP32-mixed fluxes against the FP64 oracle: relative L1 per variable."""
+    """FP32-mixed fluxes against the FP64 oracle: relative L1 per variable,
+    without and with a gravity array."""
     import torch
     rng = np.random.default_rng(55)
     G = 64
     uin = random_stencils(rng, G)
+    grav = rng.normal(0, 1, (G,) + ra.grav_shape(NV))
     dx, dt = 1 / 128, 2e-4
     m = defined_face_mask(NV)
     for riemann in ("hllc", "llf"):
-        ra.configure(gamma=5 / 3, riemann=riemann, slope_type=1, precision="fp32")
-        oracle_lib.set_params(gamma=5 / 3, riemann=riemann, slope_type=1)
-        f_ref = np.zeros((G,) + ra.flux_shape(NV))
-        t_ref = np.zeros((G,) + ra.tmp_shape(NV))
-        oracle_lib.unsplit_batched(uin, None, f_ref, t_ref, dx, dt)
-        d_f = torch.zeros(f_ref.shape, dtype=torch.float64, device="cuda")
-        d_t = torch.zeros(t_ref.shape, dtype=torch.float64, device="cuda")
-        ra.unsplit_device(torch.from_numpy(uin).cuda(), None, d_f, d_t, dx, dt)
-        torch.cuda.synchronize()
-        f = d_f.cpu().numpy()
-        for v in range(5):
-            a, b = f[:, :, v][:, m[:, v]], f_ref[:, :, v][:, m[:, v]]
-            rel_l1 = np.abs(a - b).sum() / np.abs(b).sum()
-            assert rel_l1 < 1e-5, (riemann, v, rel_l1)
+        for g in (None, grav):
+            ra.configure(gamma=5 / 3, riemann=riemann, slope_type=1, precision="fp32")
+            oracle_lib.set_params(gamma=5 / 3, riemann=riemann, slope_type=1)
+            f_ref = np.zeros((G,) + ra.flux_shape(NV))
+            t_ref = np.zeros((G,) + ra.tmp_shape(NV))
+            oracle_lib.unsplit_batched(uin, g, f_ref, t_ref, dx, dt)
+            d_f = torch.zeros(f_ref.shape, dtype=torch.float64, device="cuda")
+            d_t = torch.zeros(t_ref.shape, dtype=torch.float64, device="cuda")
+            d_g = None if g is None else torch.from_numpy(g).cuda()
+            ra.unsplit_device(torch.from_numpy(uin).cuda(), d_g, d_f, d_t, dx, dt)
+            torch.cuda.synchronize()
+            f = d_f.cpu().numpy()
+            for v in range(5):
+                a, b = f[:, :, v][:, m[:, v]], f_ref[:, :, v][:, m[:, v]]
+                rel_l1 = np.abs(a - b).sum() / np.abs(b).sum()
+                assert rel_l1 < 1e-5, (riemann, g is None, v, rel_l1)
     ra.configure()
 
 
@@ -116,24 +123,30 @@ def test_cmpdt_golden(gpu, golden_cmpdt):
     ra.configure()
 
 
+@pytest.mark.parametrize("gravity", [False, True])
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
-def test_unsplit_nvector_layouts_agree(gpu, precision):
+def test_unsplit_nvector_layouts_agree(gpu, precision, gravity):
     """The same octs laid out in batches of 32 and of 16 give the same bits.
     With 32 the FP32-mixed kernel stages each cell's primitives once in LDS;
     with 16 the row kernel evaluates them per row, so this pins the two
-    forms against each other."""
+    forms against each other, without and with a gravity array."""
     import torch
     rng = np.random.default_rng(77)
     G = 8
     u32 = random_stencils(rng, G, nv=32)
     u16 = u32.reshape(G, 5, 6, 6, 6, 2, 16).transpose(0, 5, 1, 2, 3, 4, 6).reshape(2 * G, 5, 6, 6, 6, 16)
+    g32 = g16 = None
+    if gravity:
+        g32 = rng.normal(0, 1, (G,) + ra.grav_shape(32))
+        g16 = g32.reshape(G, 3, 6, 6, 6, 2, 16).transpose(0, 5, 1, 2, 3, 4, 6).reshape(2 * G, 3, 6, 6, 6, 16)
     dx, dt = 1 / 128, 2e-4
     out = {}
-    for nv, u in ((32, u32), (16, u16)):
+    for nv, u, g in ((32, u32, g32), (16, u16, g16)):
         ra.configure(gamma=5 / 3, riemann="hllc", slope_type=1, precision=precision, nvector=nv)
         f = torch.zeros((u.shape[0],) + ra.flux_shape(nv), dtype=torch.float64, device="cuda")
         t = torch.zeros((u.shape[0],) + ra.tmp_shape(nv), dtype=torch.float64, device="cuda")
-        ra.unsplit_device(torch.from_numpy(np.ascontiguousarray(u)).cuda(), None, f, t, dx, dt)
+        d_g = None if g is None else torch.from_numpy(np.ascontiguousarray(g)).cuda()
+        ra.unsplit_device(torch.from_numpy(np.ascontiguousarray(u)).cuda(), d_g, f, t, dx, dt)
         torch.cuda.synchronize()
         out[nv] = (f.cpu().numpy(), t.cpu().numpy())
     ra.configure()
