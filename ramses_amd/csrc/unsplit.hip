@@ -170,8 +170,9 @@ __global__ __launch_bounds__(kStageThreads) void unsplit_staged(UnsplitArgs a) {
 // has kernels with the limiter fixed at compile time (ST = 1); the other
 // 7-point types take it at run time (ST = -1).  The row kernels without a
 // gravity array (G = false) read no gravity and test no pointer per cell:
-// 6% of the FP64 kernel's time on 2^20 octs.  Calls with gravity take the
-// row kernel with the slope type at run time.
+// 8% of the FP64 kernel's time on 2^20 octs.  The reference always passes
+// a gravity array (zeros without Poisson), so minmod has a gravity instance
+// too.
 template <typename R, int RS, bool S27>
 static void launch_t(const UnsplitArgs& a, long grid, int block, hipStream_t stream) {
   const bool st1 = !S27 && a.hp.slope_type == 1;
@@ -187,6 +188,12 @@ static void launch_t(const UnsplitArgs& a, long grid, int block, hipStream_t str
   }
   const dim3 g((unsigned)grid), b(block);
   if (a.grav) {
+    if constexpr (!S27) {
+      if (st1) {
+        hipLaunchKernelGGL((unsplit_rows<R, RS, S27, 1, true>), g, b, 0, stream, a);
+        return;
+      }
+    }
     hipLaunchKernelGGL((unsplit_rows<R, RS, S27, -1, true>), g, b, 0, stream, a);
     return;
   }
