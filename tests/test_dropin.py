@@ -43,8 +43,8 @@ def test_link_level_dropin_resolves_unsplit_to_library():
 
 @needs_ref
 def test_integration_patch_compiles_and_forwards():
-    """INTEGRATION.md section 3's patch file, compiled by flang against the
-    reference's modules, is what the patched build links: it defines
+    """The integration guide's patch file (its first fortran block), compiled
+    by flang against the reference's modules, is what the patched build links: it defines
     `unsplit_` itself and calls the library's ramses_unsplit and
     ramses_hydro_config."""
     text = open(os.path.join(ROOT, "INTEGRATION.md")).read()

@@ -95,9 +95,9 @@ def test_reference_godunov_through_library_unsplit(gpu, which, riemann, slope):
     """The drop-in end to end: the reference's own godunov_fine
     (hydro/godunov_fine.f90), built without umuscl.o so that godfine1's
     `call unsplit` (:681) reaches this library -- through its link-level
-    symbol unsplit_ (dropin), or through the ISO_C_BINDING patch file of
-    INTEGRATION.md section 3 forwarding to ramses_unsplit (patched, compiled
-    from that document by oracle/Makefile) -- gives the all-reference
+    symbol unsplit_ (dropin), or through the integration guide's
+    ISO_C_BINDING patch file forwarding to ramses_unsplit (patched, compiled
+    from the guide by oracle/Makefile) -- gives the all-reference
     build's unew bit for bit on levels 3-5 of the sphere tree, coarse/fine
     buffer octs included.  The patch configures the library once per
     process from the reference's module globals, so it runs one case."""
