@@ -32,7 +32,8 @@ from helpers import defined_face_mask, random_stencils
 pytestmark = pytest.mark.gpu
 needs_ref = pytest.mark.needs_ref
 
-C4 = dict(gamma=5.0 / 3.0, riemann="hllc", slope_type=1, courant_factor=0.5)
+# hydro parameters of the Mach-5 k^-4 turbulence level
+TURB_HYDRO = dict(gamma=5.0 / 3.0, riemann="hllc", slope_type=1, courant_factor=0.5)
 REL_L1_FP32 = 1e-5  # FP32-mixed vs FP64, relative L1 per conserved variable
 
 
@@ -72,7 +73,7 @@ def test_unsplit_full_launch_strided_vs_oracle(gpu, oracle_lib, riemann):
     print(f"full launch: {G} batches, dt={dt!r}, {riemann}", flush=True)
     out = {}
     for prec in ("fp64", "fp32"):
-        ra.configure(**dict(C4, riemann=riemann), precision=prec)
+        ra.configure(**dict(TURB_HYDRO, riemann=riemann), precision=prec)
         flux = torch.zeros((G,) + ra.flux_shape(32), dtype=torch.float64, device=uin.device)
         tmp = torch.zeros((G,) + ra.tmp_shape(32), dtype=torch.float64, device=uin.device)
         ra.unsplit_device(uin, None, flux, tmp, dx, dt)
@@ -81,7 +82,7 @@ def test_unsplit_full_launch_strided_vs_oracle(gpu, oracle_lib, riemann):
         del flux, tmp
     u_s = np.ascontiguousarray(uin[sample].cpu().numpy())
     del uin
-    oracle_lib.set_params(**dict(C4, riemann=riemann))
+    oracle_lib.set_params(**dict(TURB_HYDRO, riemann=riemann))
     f_ref = np.zeros((len(sample),) + ra.flux_shape(32))
     t_ref = np.zeros((len(sample),) + ra.tmp_shape(32))
     oracle_lib.unsplit_batched(u_s, None, f_ref, t_ref, dx, dt)
@@ -103,7 +104,7 @@ def test_fp32_staged_ragged_last_batch(gpu, oracle_lib, last):
     rng = np.random.default_rng(11 + last)
     uin = random_stencils(rng, 5)
     dx, dt = 1.0 / 128, 2e-4
-    ra.configure(**C4, precision="fp32")
+    ra.configure(**TURB_HYDRO, precision="fp32")
     d_u = torch.from_numpy(uin).cuda()
     res = {}
     for ngl in (32, last):
@@ -116,7 +117,7 @@ def test_fp32_staged_ragged_last_batch(gpu, oracle_lib, last):
         assert np.array_equal(a[:4], b[:4])
         assert np.array_equal(a[4, ..., :last], b[4, ..., :last])
         assert np.all(b[4, ..., last:] == 7.0)
-    oracle_lib.set_params(**C4)
+    oracle_lib.set_params(**TURB_HYDRO)
     f_ref = np.full((5,) + ra.flux_shape(32), 7.0)
     t_ref = np.full((5,) + ra.tmp_shape(32), 7.0)
     oracle_lib.unsplit_batched(uin, None, f_ref, t_ref, dx, dt, ngrid_last=last)
@@ -135,10 +136,10 @@ def test_c4_field_vs_reference_routines_64(gpu):
     bitwise against the reference's own courant_fine and godunov_fine."""
     from oracle.ref_uniform import ReferenceUniformLevel
     from ramses_amd import ics
-    u0 = ics.turbulent_host(64, 5.0, gamma=C4["gamma"], rho_sigma=0.5, seed=4)
-    R = ReferenceUniformLevel(64, C4["gamma"], riemann="hllc", slope_type=1, courant_factor=0.5)
+    u0 = ics.turbulent_host(64, 5.0, gamma=TURB_HYDRO["gamma"], rho_sigma=0.5, seed=4)
+    R = ReferenceUniformLevel(64, TURB_HYDRO["gamma"], riemann="hllc", slope_type=1, courant_factor=0.5)
     R.load(u0)
-    ra.configure(**C4)
+    ra.configure(**TURB_HYDRO)
     lvl = ra.UniformLevel(u0)
     for s in range(3):
         dt_ref = R.step()
@@ -150,9 +151,9 @@ def test_c4_field_vs_reference_routines_64(gpu):
 
 def test_c4_field_vs_oracle_128(gpu, oracle_lib):
     from ramses_amd import ics
-    u0 = ics.turbulent_host(128, 5.0, gamma=C4["gamma"], rho_sigma=0.5, seed=4)
-    ra.configure(**C4)
-    oracle_lib.set_params(**C4)
+    u0 = ics.turbulent_host(128, 5.0, gamma=TURB_HYDRO["gamma"], rho_sigma=0.5, seed=4)
+    ra.configure(**TURB_HYDRO)
+    oracle_lib.set_params(**TURB_HYDRO)
     lvl = ra.UniformLevel(u0)
     ref = u0
     for s in range(3):
@@ -165,7 +166,7 @@ def test_c4_field_vs_oracle_128(gpu, oracle_lib):
 @pytest.fixture(scope="module")
 def c4_512():
     from ramses_amd import ics
-    return ics.turbulent_host(512, 5.0, gamma=C4["gamma"], rho_sigma=0.5, seed=4)
+    return ics.turbulent_host(512, 5.0, gamma=TURB_HYDRO["gamma"], rho_sigma=0.5, seed=4)
 
 
 def test_c4_512_full_size(gpu, oracle_lib, c4_512):
@@ -178,9 +179,9 @@ def test_c4_512_full_size(gpu, oracle_lib, c4_512):
     import torch
     n, dx, steps = 512, 1.0 / 512, 10
     u0 = c4_512
-    ra.configure(**C4)
-    oracle_lib.set_params(**C4)
-    start = min(1.0 / 1e-10, C4["courant_factor"] * dx / 1e-10)
+    ra.configure(**TURB_HYDRO)
+    oracle_lib.set_params(**TURB_HYDRO)
+    start = min(1.0 / 1e-10, TURB_HYDRO["courant_factor"] * dx / 1e-10)
     dt0_ref = oracle_lib.uniform_courant(u0, dx, start)[0]
     sb = slice(192, 320)
     lvl = ra.UniformLevel(u0)
@@ -209,7 +210,7 @@ def test_c4_512_full_size(gpu, oracle_lib, c4_512):
         m = 4 * (k + 1)
         inner = slice(m, 128 - m)
         assert np.array_equal(ref[:, inner, inner, inner], subs[k][:, inner, inner, inner]), k
-    ra.configure(**C4, precision="fp32")
+    ra.configure(**TURB_HYDRO, precision="fp32")
     lvl32 = ra.UniformLevel(u0)
     for _ in range(steps):
         lvl32.step()
@@ -221,7 +222,7 @@ def test_c4_512_full_size(gpu, oracle_lib, c4_512):
 
 
 # ------------------------------------------------------------------- serving
-C3 = dict(gamma=5.0 / 3.0, riemann="hllc", slope_type=1, courant_factor=0.5)
+SERVE_HYDRO = dict(gamma=5.0 / 3.0, riemann="hllc", slope_type=1, courant_factor=0.5)
 C3_REQUESTS = [0, 1, 2, 3, 4, 7, 10, 11]  # indices into the benchmark's request stream
 
 
@@ -235,7 +236,7 @@ def _oracle_serve(O, u0, n, t_end, boxlen=1.0, smallc=1e-10):
     """The oracle's level step repeated until t >= t_end with each dt capped
     at t_end - t, as the server steps a request."""
     dx = boxlen / n
-    start = min(boxlen / smallc, C3["courant_factor"] * dx / smallc)
+    start = min(boxlen / smallc, SERVE_HYDRO["courant_factor"] * dx / smallc)
     dtc = O.uniform_courant(u0, dx, start)[0]
     u, t, steps = u0, 0.0, 0
     while not t >= t_end:
@@ -252,12 +253,12 @@ def c3_cases(gpu):
     from ramses_amd import ics
     sides = _c3_sides()
     O = oracle.Oracle()
-    O.set_params(**C3)
+    O.set_params(**SERVE_HYDRO)
     cases = []
     for i in C3_REQUESTS:
         n = sides[i]
         t_end = 0.02 if n == 128 else 0.1
-        u0 = ics.turbulent(n, 2.0, gamma=C3["gamma"], rho_sigma=0.0, seed=1000 + i, generator="device")
+        u0 = ics.turbulent(n, 2.0, gamma=SERVE_HYDRO["gamma"], rho_sigma=0.0, seed=1000 + i, generator="device")
         host = u0.cpu().numpy()
         ref, steps = _oracle_serve(O, host, n, t_end)
         print(f"request {i}: {n}^3 to t={t_end}, {steps} oracle steps", flush=True)
@@ -269,7 +270,7 @@ def c3_cases(gpu):
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_c3_served_vs_oracle(gpu, c3_cases, precision):
     from ramses_amd.serve import HydroServer, Request
-    ra.configure(**C3, precision=precision)
+    ra.configure(**SERVE_HYDRO, precision=precision)
     reqs = [Request(c["i"], c["n"], c["t_end"], arrival=0.001 * k, ic=lambda n, u=c["u0"]: u.clone(),
                     keep_result=True) for k, c in enumerate(c3_cases)]
     srv = HydroServer(hbm_budget=1 << 30, max_slots=4)
