@@ -97,7 +97,10 @@ int ramses_unsplit_noalloc(const double *uin, const double *gravin, double *flux
  * offset g*nvector*216*5 for uin, etc.), every group full except the last
  * which has `ngrid_last` lanes.  Equivalent to the godunov_fine sweep loop
  * (hydro/godunov_fine.f90:24-31) calling unsplit once per batch.  Enqueued
- * on `stream` (a hipStream_t; NULL = default stream), asynchronous. */
+ * on `stream` (a hipStream_t; NULL = default stream), asynchronous.
+ * gravin may be NULL (gravity zero, the same bits as a zero array): that
+ * selects kernels with no gravity loads, 1.51 ms against 1.97 ms with a
+ * zero array for 2^20 octs in FP64. */
 int ramses_unsplit_device(const double *uin, const double *gravin, double *flux, double *tmp, double dx,
                           double dy, double dz, double dt, long ngroups, int ngrid_last, void *stream);
 
