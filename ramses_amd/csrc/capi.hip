@@ -123,25 +123,39 @@ Config current() {
   return g_cfg;
 }
 
-// Device staging for the host-memory entry points.  The reference's unsplit
-// is not reentrant (its `save` scratch arrays, hydro/umuscl.f90:41-53); the
-// host-memory form is likewise serialised on one library-owned stream.
+// Device staging for the host-memory entry points, with a pinned host
+// mirror: the caller's arrays are copied into it on the host and cross PCIe
+// in two DMA transfers each way instead of six pageable ones.  The
+// reference's unsplit is not reentrant (its `save` scratch arrays,
+// hydro/umuscl.f90:41-53); the host-memory form is likewise serialised on
+// one library-owned stream.
 struct Staging {
   std::mutex mu;
   hipStream_t stream = nullptr;
-  double* buf = nullptr;
-  size_t cap = 0;  // doubles
+  double* buf = nullptr;   // device
+  double* host = nullptr;  // pinned host memory of the same size
+  size_t cap = 0;          // doubles
   int ensure(size_t n) {
     if (!stream) {
       hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
       if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
     }
     if (n <= cap) return 0;
+    // every call synchronises its stream before returning, so nothing is in
+    // flight on the old buffers
     if (buf) (void)hipFree(buf);
+    if (host) (void)hipHostFree(host);
     buf = nullptr;
+    host = nullptr;
     cap = 0;
     hipError_t e = hipMalloc(&buf, n * sizeof(double));
     if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    e = hipHostMalloc(reinterpret_cast<void**>(&host), n * sizeof(double), hipHostMallocDefault);
+    if (e != hipSuccess) {
+      (void)hipFree(buf);
+      buf = nullptr;
+      return hip_fail(e, "hipHostMalloc");
+    }
     cap = n;
     return 0;
   }
@@ -232,19 +246,35 @@ int ramses_unsplit(const double* uin, const double* gravin, double* flux, double
   double* d_g = d_u + nu;
   double* d_f = d_g + ng;
   double* d_t = d_f + nf;
+  double* h_u = g_stage.host;  // the same layout in pinned host memory
+  double* h_g = h_u + nu;
+  double* h_f = h_g + ng;
+  double* h_t = h_f + nf;
   hipStream_t s = g_stage.stream;
   hipError_t e;
   // flux/tmp travel both ways so that entries the reference never writes keep the caller's values
-  if ((e = hipMemcpyAsync(d_u, uin, nu * 8, hipMemcpyHostToDevice, s)) != hipSuccess) return hip_fail(e, "H2D uin");
-  if (gravin && (e = hipMemcpyAsync(d_g, gravin, ng * 8, hipMemcpyHostToDevice, s)) != hipSuccess)
-    return hip_fail(e, "H2D gravin");
-  if ((e = hipMemcpyAsync(d_f, flux, nf * 8, hipMemcpyHostToDevice, s)) != hipSuccess) return hip_fail(e, "H2D flux");
-  if ((e = hipMemcpyAsync(d_t, tmp, nt * 8, hipMemcpyHostToDevice, s)) != hipSuccess) return hip_fail(e, "H2D tmp");
+  std::memcpy(h_u, uin, nu * 8);
+  if (gravin) std::memcpy(h_g, gravin, ng * 8);
+  std::memcpy(h_f, flux, nf * 8);
+  std::memcpy(h_t, tmp, nt * 8);
+  if (gravin) {
+    if ((e = hipMemcpyAsync(d_u, h_u, (nu + ng + nf + nt) * 8, hipMemcpyHostToDevice, s)) != hipSuccess)
+      return hip_fail(e, "H2D uin, gravin, flux, tmp");
+  } else {
+    if ((e = hipMemcpyAsync(d_u, h_u, nu * 8, hipMemcpyHostToDevice, s)) != hipSuccess) return hip_fail(e, "H2D uin");
+    if ((e = hipMemcpyAsync(d_f, h_f, (nf + nt) * 8, hipMemcpyHostToDevice, s)) != hipSuccess)
+      return hip_fail(e, "H2D flux, tmp");
+  }
   rc = ramses_unsplit_device(d_u, gravin ? d_g : nullptr, d_f, d_t, dx, dy, dz, dt, 1, ngrid, s);
-  if (rc) return rc;
-  if ((e = hipMemcpyAsync(flux, d_f, nf * 8, hipMemcpyDeviceToHost, s)) != hipSuccess) return hip_fail(e, "D2H flux");
-  if ((e = hipMemcpyAsync(tmp, d_t, nt * 8, hipMemcpyDeviceToHost, s)) != hipSuccess) return hip_fail(e, "D2H tmp");
+  if (rc) {
+    (void)hipStreamSynchronize(s);  // the copies above still read the pinned buffer
+    return rc;
+  }
+  if ((e = hipMemcpyAsync(h_f, d_f, (nf + nt) * 8, hipMemcpyDeviceToHost, s)) != hipSuccess)
+    return hip_fail(e, "D2H flux, tmp");
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "unsplit sync");
+  std::memcpy(flux, h_f, nf * 8);
+  std::memcpy(tmp, h_t, nt * 8);
   return 0;
 }
 

@@ -119,7 +119,6 @@ __device__ __forceinline__ void level_tile(const double* __restrict__ u, double*
     }
   }
   const bool active = kind != kNone;
-  const bool interior = kind == kInterior;
   const int e = ey * EX + ex;  // own slot in the exchange area
   const int xg = wrapi(x0 - 1 + ex, n), yg = wrapi(y0 - 1 + ey, n);
   const int qo = (ey + 1) * QX + (ex + 1);  // own cell in the primitive plane
