@@ -262,16 +262,20 @@ int ramses_unsplit(const double* uin, const double* gravin, double* flux, double
       return hip_fail(e, "H2D uin, gravin, flux, tmp");
   } else {
     if ((e = hipMemcpyAsync(d_u, h_u, nu * 8, hipMemcpyHostToDevice, s)) != hipSuccess) return hip_fail(e, "H2D uin");
-    if ((e = hipMemcpyAsync(d_f, h_f, (nf + nt) * 8, hipMemcpyHostToDevice, s)) != hipSuccess)
+    if ((e = hipMemcpyAsync(d_f, h_f, (nf + nt) * 8, hipMemcpyHostToDevice, s)) != hipSuccess) {
+      (void)hipStreamSynchronize(s);  // uin's copy may still read the pinned buffer
       return hip_fail(e, "H2D flux, tmp");
+    }
   }
   rc = ramses_unsplit_device(d_u, gravin ? d_g : nullptr, d_f, d_t, dx, dy, dz, dt, 1, ngrid, s);
   if (rc) {
     (void)hipStreamSynchronize(s);  // the copies above still read the pinned buffer
     return rc;
   }
-  if ((e = hipMemcpyAsync(h_f, d_f, (nf + nt) * 8, hipMemcpyDeviceToHost, s)) != hipSuccess)
+  if ((e = hipMemcpyAsync(h_f, d_f, (nf + nt) * 8, hipMemcpyDeviceToHost, s)) != hipSuccess) {
+    (void)hipStreamSynchronize(s);
     return hip_fail(e, "D2H flux, tmp");
+  }
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "unsplit sync");
   std::memcpy(flux, h_f, nf * 8);
   std::memcpy(tmp, h_t, nt * 8);
