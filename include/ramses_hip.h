@@ -69,7 +69,9 @@ const char *ramses_last_error(void);
  *   subroutine unsplit(uin,gravin,pin,flux,tmp,dx,dy,dz,dt,ngrid)
  * (hydro/umuscl.f90:22, called from hydro/godunov_fine.f90:681).  All
  * arguments by reference, host memory.  On failure it prints the message and
- * aborts, as the reference's `stop` does. */
+ * aborts, as the reference's `stop` does.  A gravity array of +0.0 words
+ * only (godfine1's without Poisson) is treated as none: the same bits,
+ * without its copy. */
 void unsplit_(const double *uin, const double *gravin, const double *pin, double *flux, double *tmp,
               const double *dx, const double *dy, const double *dz, const double *dt, const int *ngrid);
 

@@ -242,31 +242,25 @@ int ramses_unsplit(const double* uin, const double* gravin, double* flux, double
   std::lock_guard<std::mutex> lk(g_stage.mu);
   int rc = g_stage.ensure(nu + ng + nf + nt);
   if (rc) return rc;
+  // uin, flux, tmp, gravin: one transfer in (gravin last, left out when
+  // null) and one out (flux and tmp)
   double* d_u = g_stage.buf;
-  double* d_g = d_u + nu;
-  double* d_f = d_g + ng;
+  double* d_f = d_u + nu;
   double* d_t = d_f + nf;
+  double* d_g = d_t + nt;
   double* h_u = g_stage.host;  // the same layout in pinned host memory
-  double* h_g = h_u + nu;
-  double* h_f = h_g + ng;
+  double* h_f = h_u + nu;
   double* h_t = h_f + nf;
+  double* h_g = h_t + nt;
   hipStream_t s = g_stage.stream;
   hipError_t e;
   // flux/tmp travel both ways so that entries the reference never writes keep the caller's values
   std::memcpy(h_u, uin, nu * 8);
-  if (gravin) std::memcpy(h_g, gravin, ng * 8);
   std::memcpy(h_f, flux, nf * 8);
   std::memcpy(h_t, tmp, nt * 8);
-  if (gravin) {
-    if ((e = hipMemcpyAsync(d_u, h_u, (nu + ng + nf + nt) * 8, hipMemcpyHostToDevice, s)) != hipSuccess)
-      return hip_fail(e, "H2D uin, gravin, flux, tmp");
-  } else {
-    if ((e = hipMemcpyAsync(d_u, h_u, nu * 8, hipMemcpyHostToDevice, s)) != hipSuccess) return hip_fail(e, "H2D uin");
-    if ((e = hipMemcpyAsync(d_f, h_f, (nf + nt) * 8, hipMemcpyHostToDevice, s)) != hipSuccess) {
-      (void)hipStreamSynchronize(s);  // uin's copy may still read the pinned buffer
-      return hip_fail(e, "H2D flux, tmp");
-    }
-  }
+  if (gravin) std::memcpy(h_g, gravin, ng * 8);
+  if ((e = hipMemcpyAsync(d_u, h_u, (nu + nf + nt + (gravin ? ng : 0)) * 8, hipMemcpyHostToDevice, s)) != hipSuccess)
+    return hip_fail(e, "H2D uin, flux, tmp, gravin");
   rc = ramses_unsplit_device(d_u, gravin ? d_g : nullptr, d_f, d_t, dx, dy, dz, dt, 1, ngrid, s);
   if (rc) {
     (void)hipStreamSynchronize(s);  // the copies above still read the pinned buffer
@@ -353,7 +347,24 @@ int ramses_unsplit_noalloc(const double* uin, const double* gravin, double* flux
 void unsplit_(const double* uin, const double* gravin, const double* pin, double* flux, double* tmp,
               const double* dx, const double* dy, const double* dz, const double* dt, const int* ngrid) {
   (void)pin;  // declared but never read by the reference body (hydro/umuscl.f90:32)
-  const int rc = ramses_unsplit(uin, gravin, flux, tmp, *dx, *dy, *dz, *dt, *ngrid);
+  // godfine1 passes zeros when poisson is off (its gloc starts at 0.0 and
+  // is filled only under `if(poisson)`, hydro/godunov_fine.f90:512,637).
+  // An array of +0.0 words gives
+  // the same bits as no array (zero acceleration enters ctoprim either way),
+  // so it is not copied and the gravity-free kernels run; a -0.0 or any
+  // other word keeps the array.
+  const double* g = gravin;
+  if (g) {
+    const size_t n = (size_t)current().nvector * 216 * 3;
+    size_t i = 0;
+    for (; i < n; ++i) {
+      uint64_t w;
+      std::memcpy(&w, g + i, sizeof w);
+      if (w != 0) break;
+    }
+    if (i == n) g = nullptr;
+  }
+  const int rc = ramses_unsplit(uin, g, flux, tmp, *dx, *dy, *dz, *dt, *ngrid);
   if (rc) {
     std::fprintf(stderr, "unsplit: %s\n", ramses_last_error());
     std::abort();
